@@ -1,0 +1,221 @@
+/*
+ * pastel.h -- C ABI of the MI355X H-LU factor + solve library (libpastel.so).
+ *
+ * Plain pointers and sizes only.  Every function returns PP_OK (0) or a
+ * negative PP_ERR_* code and leaves a message for pp_last_error() (thread
+ * local).  No C++ exception crosses this boundary.
+ *
+ * The reference (QxGbM/Pastel-Palettes) is a C++/CUDA library with no FFI of
+ * its own; each entry point below names the reference interface whose role
+ * it takes over.  Device pointers passed in ("dptr") are HIP device
+ * addresses on the context's device.
+ */
+#ifndef PASTEL_H
+#define PASTEL_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PP_OK 0
+#define PP_ERR_INVALID (-1)  /* bad argument or shape */
+#define PP_ERR_HIP (-2)      /* HIP runtime error */
+#define PP_ERR_TIMEOUT (-3)  /* VM watchdog fired or host wait timed out */
+#define PP_ERR_NOMEM (-4)    /* HBM budget or allocation failure */
+#define PP_ERR_INTERNAL (-5) /* anything else */
+#define PP_ERR_DEVICE (-6)   /* kernel reported a bad opcode / operation */
+
+/* precision modes */
+#define PP_FP64 0  /* FP64 storage, FP64 MFMA everywhere (the drop-in) */
+#define PP_FP32 1  /* FP32 storage, FP32 MFMA */
+#define PP_MIXED 2 /* FP32 storage, FP32 leaf getrf/trsm, BF16 MFMA (FP32 accumulate) for gemm and accum */
+
+const char* pp_last_error(void);
+/* Library version string. */
+const char* pp_version(void);
+
+/* ------------------------------------------------------------------ rts */
+
+/* DataMap2D::checkOverlap (rts/ops.h:18, rts/ops.cpp:7-40).  Byte units.
+ * Returns 1 / 0, or a negative error code. */
+int pp_rts_check_overlap(uint64_t ptr_a, uint64_t pitch_a, uint64_t width_a, uint64_t height_a, uint64_t ptr_b,
+                         uint64_t pitch_b, uint64_t width_b, uint64_t height_b);
+
+/* Operation::checkDependencyFrom (rts/ops.cpp:42-61): maps are n x 4 arrays
+ * (ptr, pitch, width, height), read_only n flags.  Returns the 3-bit mask
+ * (flow 1, anti 2, out 4) of op_to depending on op_from. */
+int pp_rts_check_dependency(int n_from, const uint64_t* maps_from, const int32_t* ro_from, int n_to,
+                            const uint64_t* maps_to, const int32_t* ro_to);
+
+/* instructions_scheduler (rts/scheduler.h:73-126) over an explicit DAG:
+ * n instructions with FLOP estimates, n_edges edges (edges[2e] -> edges[2e+1])
+ * with masks.  From- and to-lists are taken in ascending order. */
+typedef struct pp_schedule pp_schedule;
+int pp_schedule_create(int n, const int64_t* flops, int n_edges, const int32_t* edges, const int32_t* masks,
+                       int workers, pp_schedule** out);
+int pp_schedule_worker_length(const pp_schedule* s, int worker);
+/* Entries of one worker: instruction ids, 1 = execute / 0 = wait, and the
+ * worker's FLOP clock at each entry.  Any output pointer may be NULL. */
+int pp_schedule_worker_entries(const pp_schedule* s, int worker, int32_t* insts, int32_t* ex_w, int64_t* start_flops);
+int pp_schedule_comm_waits(const pp_schedule* s);
+void pp_schedule_destroy(pp_schedule* s);
+/* _MIN_INST_FLOPS used by the scheduler. */
+int64_t pp_min_inst_flops(void);
+
+/* ------------------------------------------------------------------ H-matrix structure */
+
+/* Hierarchical (matrix/Hierarchical.h; API implied by src/launcher.cu:126-139). */
+typedef struct pp_hmatrix pp_hmatrix;
+/* HODLR: `levels` binary splits, off-diagonal blocks of rank `rank`. */
+int pp_hmatrix_create_hodlr(int n, int levels, int rank, pp_hmatrix** out);
+/* All-dense binary partition down to `leaf`. */
+int pp_hmatrix_create_dense(int n, int leaf, pp_hmatrix** out);
+void pp_hmatrix_destroy(pp_hmatrix* h);
+int pp_hmatrix_size(const pp_hmatrix* h);
+/* Arena size in elements. */
+int64_t pp_hmatrix_storage(const pp_hmatrix* h);
+int pp_hmatrix_num_leaves(const pp_hmatrix* h);
+/* info = {type (1 dense, 2 low rank), row0, col0, ny, nx, rank, off (dense) or off_u, off_v} */
+int pp_hmatrix_leaf(const pp_hmatrix* h, int i, int64_t info[8]);
+
+/* ------------------------------------------------------------------ programs */
+
+/* An instruction stream for one launch: h-ops (h_ops tree, src/launcher.cu:139)
+ * -> DAG (h_ops_dag, src/launcher.cu:53) -> schedule (rts/scheduler.cpp)
+ * -> per-worker words (instructions_manager, src/launcher.cu:72).
+ * Pointer-table slots: 0 = matrix arena, 1 = temp arena, 2 = right-hand sides.
+ * Batched programs use slots 3*i, 3*i+1, 3*i+2 for matrix i. */
+typedef struct pp_program pp_program;
+/* H-LU factorization (dev_hierarchical_GETRF, src/launcher.cu:117). */
+int pp_program_factor(const pp_hmatrix* h, int workers, pp_program** out);
+/* Forward + backward substitution on an n x nrhs row-major block (pitch ld). */
+int pp_program_solve(const pp_hmatrix* h, int nrhs, int ld, int rhs_split, int workers, pp_program** out);
+/* One program factoring (solve != 0: solving) `count` matrices at once. */
+int pp_program_factor_batch(const pp_hmatrix* const* hs, int count, int workers, pp_program** out);
+int pp_program_solve_batch(const pp_hmatrix* const* hs, int count, const int32_t* nrhs, const int32_t* ld,
+                           int rhs_split, int workers, pp_program** out);
+void pp_program_destroy(pp_program* p);
+
+typedef struct pp_program_info {
+  int32_t n_insts, workers, n_slots, pad;
+  int64_t n_edges, flops, temp_elems, scratch_elems, total_words, comm_waits;
+  double dag_density, schedule_ms, dag_ms, encode_ms;
+} pp_program_info;
+int pp_program_get_info(const pp_program* p, pp_program_info* info);
+/* Temp-arena size (elements) of batch member i (slot 3*i+1). */
+int64_t pp_program_temp_elems(const pp_program* p, int member);
+int pp_program_worker_length(const pp_program* p, int worker); /* words */
+int pp_program_worker_words(const pp_program* p, int worker, int32_t* out, int cap);
+/* execute words of instruction `inst` (id slot filled); returns length */
+int pp_program_inst_words(const pp_program* p, int inst, int32_t* out, int cap);
+int64_t pp_program_inst_flops(const pp_program* p, int inst);
+/* predecessors (to-list) of inst; returns count (fills up to cap) */
+int pp_program_inst_preds(const pp_program* p, int inst, int32_t* preds, int32_t* masks, int cap);
+/* scheduler entries of one worker (like pp_schedule_worker_entries) */
+int pp_program_schedule_length(const pp_program* p, int worker);
+int pp_program_schedule_entries(const pp_program* p, int worker, int32_t* insts, int32_t* ex_w, int64_t* start_flops);
+
+/* ------------------------------------------------------------------ device runtime */
+
+typedef struct pp_ctx pp_ctx;
+/* Context on `device`; allocations through it are capped at budget_bytes
+ * (0 = no cap). */
+int pp_ctx_create(int device, uint64_t budget_bytes, pp_ctx** out);
+int pp_ctx_destroy(pp_ctx* ctx);
+int pp_ctx_cu_count(const pp_ctx* ctx);
+uint64_t pp_ctx_bytes_in_use(const pp_ctx* ctx);
+int pp_alloc(pp_ctx* ctx, uint64_t bytes, void** dptr);
+int pp_free(pp_ctx* ctx, void* dptr);
+int pp_copy_h2d(pp_ctx* ctx, void* dst, const void* src, uint64_t bytes);
+int pp_copy_d2h(pp_ctx* ctx, void* dst, const void* src, uint64_t bytes);
+
+/* Uploaded instruction stream (launchKernelWithArgs' arguments,
+ * src/launcher.cu:92-115: insts, ptrs, comm_space, block_tmps, rnd, clocks). */
+typedef struct pp_stream pp_stream;
+/* Generic upload: per-worker words, pointer table (device addresses),
+ * scratch bytes per worker.  `precision` selects the element type of the
+ * sketch matrix (generated from `seed`) and of the scratch. */
+int pp_stream_upload(pp_ctx* ctx, const int32_t* const* words, const int32_t* lengths, int workers,
+                     const void* const* ptr_table, int n_ptrs, uint64_t scratch_bytes_per_worker, int n_insts,
+                     int precision, pp_stream** out);
+/* Upload a program; ptr_table must hold n_slots device addresses.  Temp slots
+ * left NULL are allocated (and zero-filled before every launch) by the
+ * stream. */
+int pp_stream_upload_program(pp_ctx* ctx, const pp_program* prog, const void* const* ptr_table, int n_ptrs,
+                             int precision, pp_stream** out);
+/* Rebind pointer-table slot i. */
+int pp_stream_set_slot(pp_ctx* ctx, pp_stream* s, int slot, void* dptr);
+/* Launch the persistent VM (kernel_dynamic, src/kernel.cu:1375) on hip_stream
+ * (NULL = the context stream); threads = 64 * wavefronts per worker. */
+int pp_launch(pp_ctx* ctx, pp_stream* s, int wavefronts_per_inst, void* hip_stream);
+/* Wait for the launch; on a VM error reports the first unsignalled instruction
+ * (or the faulting one) in *first_unsignalled (may be NULL). */
+int pp_wait(pp_ctx* ctx, pp_stream* s, int timeout_ms, int32_t* first_unsignalled);
+/* Enable per-entry timestamps for later launches (1) or disable (0). */
+int pp_stream_enable_trace(pp_ctx* ctx, pp_stream* s, int enable);
+/* Per-worker entry timestamps (100 MHz wall clock) of the last traced
+ * launch, concatenated by worker, each worker holding 1 + its entry count;
+ * returns the number of values (instructions_scheduler::analyzeClocks). */
+int64_t pp_read_trace(pp_ctx* ctx, pp_stream* s, uint64_t* out, int64_t cap);
+int pp_stream_destroy(pp_ctx* ctx, pp_stream* s);
+/* Watchdog limit for waits in milliseconds (default 20000). */
+int pp_stream_set_watchdog(pp_ctx* ctx, pp_stream* s, int ms);
+/* Kernel time of the last launch in ms (HIP events around the VM kernel). */
+double pp_stream_last_ms(const pp_stream* s);
+/* Maximum workers that fit co-resident for a precision / wave count. */
+int pp_max_workers(pp_ctx* ctx, int precision, int wavefronts_per_inst);
+
+/* Run one execute entry (words, id slot ignored) on a single workgroup;
+ * device pointer table of n_ptrs slots.  Test harness for the device
+ * routines. */
+int pp_exec_single(pp_ctx* ctx, int precision, const int32_t* words, int n_words, const void* const* ptr_table,
+                   int n_ptrs, uint64_t scratch_bytes, uint64_t seed);
+/* Fill a device buffer with the sketch matrix used by accum (count values,
+ * uniform in [-1, 1) from the in-library generator). */
+int pp_fill_sketch(pp_ctx* ctx, void* dptr, int64_t count, int precision, uint64_t seed);
+/* MFMA lane-layout probe (mode 0 f64 16x16x4, 1 f32 16x16x4, 2 bf16 16x16x32):
+ * out[4*lane + r] = accumulator r of lane. a, b, out are host arrays. */
+int pp_mfma_probe(pp_ctx* ctx, int mode, const double* a, int na, const double* b, int nb, double* out);
+
+/* ------------------------------------------------------------------ solver (factor + solve + refinement) */
+
+typedef struct pp_solver pp_solver;
+typedef struct pp_solve_report {
+  int32_t iterations, converged, pad0, pad1;
+  double residual;      /* max over columns of ||b - A x|| / ||b|| (FP64) */
+  double factor_ms, solve_ms, total_ms;
+} pp_solve_report;
+
+/* Build a solver for H-matrix h whose FP64 arena is at host `storage`
+ * (pp_hmatrix_storage(h) doubles).  Keeps the FP64 matrix resident for the
+ * residual and a working copy in the precision's storage type for the
+ * factors. */
+int pp_solver_create(pp_ctx* ctx, const pp_hmatrix* h, const double* storage, int precision, int workers,
+                     int wavefronts_per_inst, int nrhs_max, int rhs_split, pp_solver** out);
+/* Same, with the FP64 arena already on the device (copied, not adopted). */
+int pp_solver_create_device(pp_ctx* ctx, const pp_hmatrix* h, const double* storage_dev, int precision, int workers,
+                            int wavefronts_per_inst, int nrhs_max, int rhs_split, pp_solver** out);
+/* Factor in place (restores the working copy from the FP64 matrix first). */
+int pp_solver_factor(pp_solver* s, void* hip_stream);
+/* x = A^{-1} b by the factors, refined in FP64 until the relative residual is
+ * <= tol or max_iter refinement sweeps ran (max_iter 0 = plain solve).  b and
+ * x are device n x nrhs row-major FP64 with pitch ld. */
+int pp_solver_solve(pp_solver* s, const double* b, double* x, int nrhs, int ld, double tol, int max_iter,
+                    void* hip_stream, pp_solve_report* rep);
+/* r = b - A x with the FP64 H-matrix (device buffers, pitch ld). */
+int pp_solver_residual(pp_solver* s, const double* b, const double* x, double* r, int nrhs, int ld, void* hip_stream);
+/* Device address of the working (factor) arena and its element size. */
+int pp_solver_factor_arena(pp_solver* s, void** dptr, int* elem_bytes);
+int pp_solver_destroy(pp_solver* s);
+/* Stream of the factorization (for tracing / timing). */
+pp_stream* pp_solver_factor_stream(pp_solver* s);
+double pp_solver_factor_flops(const pp_solver* s);
+double pp_solver_solve_flops(const pp_solver* s, int nrhs);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PASTEL_H */

@@ -1,0 +1,197 @@
+"""CPU oracle of the H-LU factor + solve path -- TEST INFRASTRUCTURE ONLY.
+
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may use
+this package, and only as the checker (or the timed CPU baseline); the
+product never imports it.
+
+  liboracle.so      plain-C FP64 restatement of the reference routines
+                    (oracle.c cites src/kernel.cu line by line), the
+                    instruction-stream executor and a dense no-pivot LU;
+  _ref/libref_rts.so the reference's own rts/ops.cpp + rts/dependency.cpp,
+                    compiled from /root/reference (built only where that
+                    tree exists; tests skip it elsewhere);
+  scheduler.py      pure-Python restatement of rts/scheduler.cpp.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "liboracle.so")
+REF_LIB = os.path.join(HERE, "_ref", "libref_rts.so")
+
+_lib = None
+_ref = None
+
+
+def build(ref=True):
+    """Compile liboracle.so (and the reference rts library when the reference
+    tree is present)."""
+    subprocess.run(["make", "-s", "-C", HERE, "all"], check=True)
+    if ref and os.path.isdir("/root/reference/rts"):
+        subprocess.run(["make", "-s", "-C", HERE, "ref"], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            build(ref=False)
+        L = C.CDLL(LIB)
+        dp, ip, i32 = C.POINTER(C.c_double), C.c_int, C.c_int32
+        L.or_getrf.argtypes = [dp, ip, ip, ip]
+        L.or_trsml.argtypes = [dp, dp, ip, ip, ip, ip, ip]
+        L.or_trsmr.argtypes = [dp, dp, ip, ip, ip, ip, ip]
+        L.or_trsmu.argtypes = [dp, dp, ip, ip, ip, ip]
+        L.or_gemm.argtypes = [C.c_double, C.c_double, dp, dp, dp, ip, ip, ip, ip, ip, ip, ip, ip]
+        L.or_givens_qr.argtypes = [dp, ip, ip, ip, dp, ip]
+        L.or_accum.argtypes = [dp, dp, dp, dp, ip, ip, ip, ip, ip, ip, ip, ip, dp]
+        L.or_exec.argtypes = [C.POINTER(i32), C.POINTER(dp), dp, dp]
+        L.or_exec.restype = ip
+        L.or_run_streams.argtypes = [C.POINTER(C.POINTER(i32)), ip, C.POINTER(dp), dp, C.c_int64, dp, ip, ip]
+        L.or_run_streams.restype = ip
+        L.or_dense_lu.argtypes = [dp, ip, ip]
+        L.or_dense_lu_solve.argtypes = [dp, ip, dp, ip, ip]
+        L.or_dense_residual.argtypes = [dp, ip, dp, dp, ip, ip]
+        L.or_dense_residual.restype = C.c_double
+        L.or_sketch.argtypes = [dp, C.c_int64, C.c_uint64]
+        _lib = L
+    return _lib
+
+
+def ref_available():
+    return os.path.exists(REF_LIB)
+
+
+def ref():
+    global _ref
+    if _ref is None:
+        R = C.CDLL(REF_LIB)
+        u = C.c_uint64
+        R.ref_check_overlap.argtypes = [u] * 8
+        R.ref_check_overlap.restype = C.c_int
+        R.ref_table_new.restype = C.c_void_p
+        R.ref_table_free.argtypes = [C.c_void_p]
+        R.ref_table_update.argtypes = [C.c_void_p, C.c_int64, C.c_int64, C.c_int]
+        R.ref_table_lookup.argtypes = [C.c_void_p, C.c_int64, C.c_int64]
+        R.ref_table_lookup.restype = C.c_int
+        for f in (R.ref_table_from, R.ref_table_to):
+            f.argtypes = [C.c_void_p, C.c_int64, C.POINTER(C.c_int64), C.c_int]
+            f.restype = C.c_int
+        for f in (R.ref_table_count_from, R.ref_table_count_to):
+            f.argtypes = [C.c_void_p, C.c_int64]
+            f.restype = C.c_int64
+        _ref = R
+    return _ref
+
+
+def _d(a):
+    assert a.dtype == np.float64 and a.flags.c_contiguous
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def getrf(m):
+    m = np.ascontiguousarray(m, np.float64).copy()
+    lib().or_getrf(_d(m), m.shape[1], m.shape[0], m.shape[1])
+    return m
+
+
+def trsml(b, l, b_t=False):
+    b = np.ascontiguousarray(b, np.float64).copy()
+    l = np.ascontiguousarray(l, np.float64)
+    ny, nx = (b.shape[1], b.shape[0]) if b_t else b.shape
+    lib().or_trsml(_d(b), _d(l), nx, ny, b.shape[1], l.shape[1], int(b_t))
+    return b
+
+
+def trsmr(b, u, b_t=False):
+    b = np.ascontiguousarray(b, np.float64).copy()
+    u = np.ascontiguousarray(u, np.float64)
+    ny, nx = (b.shape[1], b.shape[0]) if b_t else b.shape
+    lib().or_trsmr(_d(b), _d(u), nx, ny, b.shape[1], u.shape[1], int(b_t))
+    return b
+
+
+def trsmu(b, u):
+    b = np.ascontiguousarray(b, np.float64).copy()
+    u = np.ascontiguousarray(u, np.float64)
+    lib().or_trsmu(_d(b), _d(u), b.shape[1], b.shape[0], b.shape[1], u.shape[1])
+    return b
+
+
+def gemm(alpha, beta, m, a, b, a_t=False, b_t=False):
+    m = np.ascontiguousarray(m, np.float64).copy()
+    a = np.ascontiguousarray(a, np.float64)
+    b = np.ascontiguousarray(b, np.float64)
+    k = a.shape[0] if a_t else a.shape[1]
+    lib().or_gemm(alpha, beta, _d(m), _d(a), _d(b), m.shape[0], m.shape[1], k, m.shape[1], a.shape[1], b.shape[1],
+                  int(a_t), int(b_t))
+    return m
+
+
+def givens_qr(y):
+    y = np.ascontiguousarray(y, np.float64).copy()
+    ny, k = y.shape
+    q = np.zeros((ny, k))
+    lib().or_givens_qr(_d(y), k, ny, k, _d(q), k)
+    return q, np.triu(y[:k])
+
+
+def sketch(count, seed=0x5EED):
+    out = np.zeros(max(count, 1))
+    lib().or_sketch(_d(out), count, seed)
+    return out[:count]
+
+
+def accum(u1, v1, u2, v2, rnd=None):
+    u1 = np.ascontiguousarray(u1, np.float64).copy()
+    v1 = np.ascontiguousarray(v1, np.float64).copy()
+    u2 = np.ascontiguousarray(u2, np.float64)
+    v2 = np.ascontiguousarray(v2, np.float64)
+    ny, k1 = u1.shape
+    nx, k2 = v1.shape[0], u2.shape[1]
+    if rnd is None:
+        rnd = sketch(nx * k1)
+    rnd = np.ascontiguousarray(rnd, np.float64)
+    lib().or_accum(_d(u1), _d(v1), _d(u2), _d(v2), nx, ny, k1, k2, k1, k1, k2, k2, _d(rnd))
+    return u1, v1
+
+
+def run_streams(streams, slots, n_insts, scratch_elems, rnd=None, threads=1):
+    """Execute per-worker instruction streams (lists of int32 arrays) on host
+    arrays `slots` (float64, modified in place)."""
+    W = len(streams)
+    sarr = [np.ascontiguousarray(s, np.int32) for s in streams]
+    sp = (C.POINTER(C.c_int32) * W)(*[s.ctypes.data_as(C.POINTER(C.c_int32)) for s in sarr])
+    pp = (C.POINTER(C.c_double) * len(slots))(*[_d(s) if s is not None else None for s in slots])
+    stride = max(int(scratch_elems), 64)
+    scratch = np.zeros(stride * W)
+    if rnd is None:
+        rnd = sketch(1 << 16)
+    rnd = np.ascontiguousarray(rnd, np.float64)
+    rc = lib().or_run_streams(sp, W, pp, _d(scratch), stride, _d(rnd), n_insts, threads)
+    if rc != 0:
+        raise RuntimeError("oracle stream run failed (deadlock or bad entry)")
+
+
+def dense_lu(a, threads=8):
+    a = np.ascontiguousarray(a, np.float64).copy()
+    lib().or_dense_lu(_d(a), a.shape[0], threads)
+    return a
+
+
+def dense_lu_solve(lu, b):
+    x = np.ascontiguousarray(b, np.float64).copy()
+    if x.ndim == 1:
+        x = x[:, None].copy()
+    lib().or_dense_lu_solve(_d(np.ascontiguousarray(lu)), lu.shape[0], _d(x), x.shape[1], x.shape[1])
+    return x
+
+
+def dense_residual(a, x, b):
+    a = np.ascontiguousarray(a, np.float64)
+    x = np.ascontiguousarray(x, np.float64)
+    b = np.ascontiguousarray(b, np.float64)
+    return lib().or_dense_residual(_d(a), a.shape[0], _d(x), _d(b), x.shape[1], x.shape[1])

@@ -1,0 +1,445 @@
+/*
+ * CPU oracle -- TEST INFRASTRUCTURE ONLY (see oracle.h).
+ *
+ * Each routine restates the semantics of the reference routine it cites in
+ * straightforward loops; no blocking tricks, so a reader can check it against
+ * the reference line by line.
+ */
+#include "oracle.h"
+
+#include <math.h>
+#include <pthread.h>
+#include <sched.h>
+#include <stdatomic.h>
+#include <stdlib.h>
+#include <string.h>
+
+static inline double get_op(const double* X, int ld, int t, int i, int k) {
+  return t ? X[(int64_t)k * ld + i] : X[(int64_t)i * ld + k];
+}
+
+/* DenseGetrf, src/kernel.cu:104-147: for each pivot i, rows below divide by
+ * the pivot and subtract the scaled pivot row from their trailing part. */
+void or_getrf(double* M, int nx, int ny, int ld) {
+  const int mn = nx < ny ? nx : ny;
+  for (int i = 0; i < mn; ++i) {
+    const double piv = M[(int64_t)i * ld + i];
+    for (int r = i + 1; r < ny; ++r) {
+      double* row = &M[(int64_t)r * ld];
+      const double l = row[i] / piv;
+      row[i] = l;
+      for (int c = i + 1; c < nx; ++c) row[c] = fma(M[(int64_t)i * ld + c], -l, row[c]);
+    }
+  }
+}
+
+/* DenseTrsmL, src/kernel.cu:150-182 (forward substitution, unit diagonal). */
+void or_trsml(double* B, const double* L, int nx_b, int ny_b, int ld_b, int ld_l, int b_T) {
+  for (int i = 0; i < ny_b; ++i)
+    for (int r = i + 1; r < ny_b; ++r) {
+      const double l = L[(int64_t)r * ld_l + i];
+      for (int c = 0; c < nx_b; ++c) {
+        double* dst = b_T ? &B[(int64_t)c * ld_b + r] : &B[(int64_t)r * ld_b + c];
+        const double src = b_T ? B[(int64_t)c * ld_b + i] : B[(int64_t)i * ld_b + c];
+        *dst = fma(src, -l, *dst);
+      }
+    }
+}
+
+/* DenseTrsmR / DenseTrsmR_transposeB, src/kernel.cu:185-285: column i of X is
+ * B's column i divided by U[i][i] after subtracting earlier columns. */
+void or_trsmr(double* B, const double* U, int nx_b, int ny_b, int ld_b, int ld_u, int b_T) {
+  for (int i = 0; i < nx_b; ++i) {
+    const double d = U[(int64_t)i * ld_u + i];
+    for (int r = 0; r < ny_b; ++r) {
+      double* x = b_T ? &B[(int64_t)i * ld_b + r] : &B[(int64_t)r * ld_b + i];
+      *x = *x / d;
+    }
+    for (int j = i + 1; j < nx_b; ++j) {
+      const double u = U[(int64_t)i * ld_u + j];
+      for (int r = 0; r < ny_b; ++r) {
+        const double xi = b_T ? B[(int64_t)i * ld_b + r] : B[(int64_t)r * ld_b + i];
+        double* dst = b_T ? &B[(int64_t)j * ld_b + r] : &B[(int64_t)r * ld_b + j];
+        *dst = fma(xi, -u, *dst);
+      }
+    }
+  }
+}
+
+/* Backward substitution with the non-unit upper factor (no reference
+ * counterpart: the reference has no solve). */
+void or_trsmu(double* B, const double* U, int nx_b, int ny_b, int ld_b, int ld_u) {
+  for (int i = ny_b - 1; i >= 0; --i) {
+    const double d = U[(int64_t)i * ld_u + i];
+    for (int c = 0; c < nx_b; ++c) B[(int64_t)i * ld_b + c] /= d;
+    for (int r = 0; r < i; ++r) {
+      const double u = U[(int64_t)r * ld_u + i];
+      for (int c = 0; c < nx_b; ++c) B[(int64_t)r * ld_b + c] = fma(B[(int64_t)i * ld_b + c], -u, B[(int64_t)r * ld_b + c]);
+    }
+  }
+}
+
+/* blockDenseGemm, src/kernel.cu:415-664: M = beta M + alpha op(A) op(B). */
+void or_gemm(double alpha, double beta, double* M, const double* A, const double* B, int m, int n, int k, int ld_m,
+             int ld_a, int ld_b, int a_T, int b_T) {
+  for (int i = 0; i < m; ++i)
+    for (int j = 0; j < n; ++j) {
+      double s = 0.;
+      for (int kk = 0; kk < k; ++kk) s += get_op(A, ld_a, a_T, i, kk) * get_op(B, ld_b, !b_T, j, kk);
+      double* c = &M[(int64_t)i * ld_m + j];
+      *c = (beta == 0.) ? alpha * s : beta * *c + alpha * s;
+    }
+}
+
+/* Givens QR, src/kernel.cu:1160-1253: every column is cleared bottom-up by
+ * rotations of adjacent rows (the reference runs the same rotations in a
+ * wavefront order across columns; rotations on disjoint rows commute).  The
+ * explicit Q applies the transposed rotations to [I; 0] in reverse. */
+void or_givens_qr(double* Y, int ld, int ny, int k, double* Q, int ldq) {
+  const int nrot_max = (ny > 0 ? ny : 1) * (k > 0 ? k : 1);
+  double* cs = (double*)malloc(sizeof(double) * 2 * nrot_max);
+  int* rows = (int*)malloc(sizeof(int) * nrot_max);
+  int nrot = 0;
+  for (int c = 0; c < k && c < ny; ++c)
+    for (int r = ny - 2; r >= c; --r) {
+      const double a = Y[(int64_t)r * ld + c], b = Y[(int64_t)(r + 1) * ld + c];
+      double co, si;
+      if (b == 0.) {
+        co = 1.;
+        si = 0.;
+      } else {
+        const double h = hypot(a, b);
+        co = a / h;
+        si = b / h;
+      }
+      /* [a'; b'] = [co si; -si co] [a; b] zeroes b */
+      for (int j = c; j < k; ++j) {
+        const double x = Y[(int64_t)r * ld + j], y = Y[(int64_t)(r + 1) * ld + j];
+        Y[(int64_t)r * ld + j] = co * x + si * y;
+        Y[(int64_t)(r + 1) * ld + j] = -si * x + co * y;
+      }
+      cs[2 * nrot] = co;
+      cs[2 * nrot + 1] = si;
+      rows[nrot++] = r;
+    }
+  for (int i = 0; i < ny; ++i)
+    for (int j = 0; j < k; ++j) Q[(int64_t)i * ldq + j] = (i == j) ? 1. : 0.;
+  for (int t = nrot - 1; t >= 0; --t) {
+    const int r = rows[t];
+    const double co = cs[2 * t], si = cs[2 * t + 1];
+    for (int j = 0; j < k; ++j) {
+      const double x = Q[(int64_t)r * ldq + j], y = Q[(int64_t)(r + 1) * ldq + j];
+      Q[(int64_t)r * ldq + j] = co * x - si * y;
+      Q[(int64_t)(r + 1) * ldq + j] = si * x + co * y;
+    }
+  }
+  free(cs);
+  free(rows);
+}
+
+/* blockLowRankAccum, src/kernel.cu:1255-1280. */
+void or_accum(double* U1, double* V1, const double* U2, const double* V2, int nx, int ny, int k1, int k2, int ld_u1,
+              int ld_v1, int ld_u2, int ld_v2, const double* rnd) {
+  const int kmax = k1 > k2 ? k1 : k2;
+  double* S = (double*)malloc(sizeof(double) * k1 * kmax);
+  double* Y = (double*)malloc(sizeof(double) * (size_t)ny * k1);
+  double* Q = (double*)malloc(sizeof(double) * (size_t)ny * k1);
+  double* Vn = (double*)malloc(sizeof(double) * (size_t)nx * k1);
+  /* Y = (U1 V1^T + U2 V2^T) Omega */
+  or_gemm(1., 0., S, rnd, V1, k1, k1, nx, k1, k1, ld_v1, 1, 0);
+  or_gemm(1., 0., Y, U1, S, ny, k1, k1, k1, ld_u1, k1, 0, 1);
+  or_gemm(1., 0., S, rnd, V2, k1, k2, nx, k2, k1, ld_v2, 1, 0);
+  or_gemm(1., 1., Y, U2, S, ny, k1, k2, k1, ld_u2, k2, 0, 1);
+  or_givens_qr(Y, k1, ny, k1, Q, k1);
+  /* V1 <- V1 (Q^T U1)^T + V2 (Q^T U2)^T, U1 <- Q */
+  or_gemm(1., 0., S, Q, U1, k1, k1, ny, k1, k1, ld_u1, 1, 0);
+  or_gemm(1., 0., Vn, V1, S, nx, k1, k1, k1, ld_v1, k1, 0, 1);
+  or_gemm(1., 0., S, Q, U2, k1, k2, ny, k2, k1, ld_u2, 1, 0);
+  or_gemm(1., 1., Vn, V2, S, nx, k1, k2, k1, ld_v2, k2, 0, 1);
+  for (int i = 0; i < ny; ++i)
+    for (int j = 0; j < k1; ++j) U1[(int64_t)i * ld_u1 + j] = Q[(int64_t)i * k1 + j];
+  for (int i = 0; i < nx; ++i)
+    for (int j = 0; j < k1; ++j) V1[(int64_t)i * ld_v1 + j] = Vn[(int64_t)i * k1 + j];
+  free(S);
+  free(Y);
+  free(Q);
+  free(Vn);
+}
+
+/* Decoder of src/kernel.cu:1393-1501 with the word offsets of the
+ * instruction layout (opcode, id, operation, operands...). */
+int or_exec(const int32_t* w, double* const* P, double* tmp, const double* rnd) {
+  switch (w[2]) {
+    case 0: /* nop */
+      return 3;
+    case 1: /* getrf */
+      or_getrf(P[w[3]] + w[4], w[5], w[6], w[7]);
+      return 8;
+    case 2: /* trsml */
+      or_trsml(P[w[3]] + w[5], P[w[4]] + w[6], w[7], w[8], w[10], w[11], w[12]);
+      return 13;
+    case 3: /* trsmr */
+      or_trsmr(P[w[3]] + w[5], P[w[4]] + w[6], w[7], w[8], w[10], w[11], w[12]);
+      return 13;
+    case 9: /* trsmu */
+      or_trsmu(P[w[3]] + w[5], P[w[4]] + w[6], w[7], w[8], w[10], w[11]);
+      return 13;
+    case 4: /* gemm: M -= A B */
+    case 5: /* gemm_plus: M += A B */
+      or_gemm(w[2] == 4 ? -1. : 1., 1., P[w[3]] + w[6], P[w[4]] + w[7], P[w[5]] + w[8], w[9], w[10], w[11], w[12],
+              w[13], w[14], w[15], w[16]);
+      return 17;
+    case 6: { /* gemm_3x: M -= A B C */
+      double *M = P[w[3]] + w[7], *A = P[w[4]] + w[8], *B = P[w[5]] + w[9], *C = P[w[6]] + w[10];
+      const int m = w[11], n = w[12], k = w[13], l = w[14];
+      if (w[22]) {
+        or_gemm(1., 0., tmp, A, B, m, l, k, l, w[16], w[17], w[19], w[20]);
+        or_gemm(-1., 1., M, tmp, C, m, n, l, w[15], l, w[18], 0, w[21]);
+      } else {
+        or_gemm(1., 0., tmp, B, C, k, n, l, n, w[17], w[18], w[20], w[21]);
+        or_gemm(-1., 1., M, A, tmp, m, n, k, w[15], w[16], n, w[19], 0);
+      }
+      return 23;
+    }
+    case 7: { /* gemm_4x: M -= A B C D, association by control (src/kernel.cu:914-953) */
+      double *M = P[w[3]] + w[8], *A = P[w[4]] + w[9], *B = P[w[5]] + w[10], *C = P[w[6]] + w[11], *D = P[w[7]] + w[12];
+      const int m = w[13], n = w[14], k = w[15], l = w[16], o = w[17];
+      const int ldm = w[18], lda = w[19], ldb = w[20], ldc = w[21], ldd = w[22];
+      const int aT = w[23], bT = w[24], cT = w[25], dT = w[26];
+      double *t1 = tmp, *t2 = tmp + w[28];
+      switch (w[27]) {
+        case 0:
+          or_gemm(1., 0., t1, A, B, m, l, k, l, lda, ldb, aT, bT);
+          or_gemm(1., 0., t2, t1, C, m, o, l, o, l, ldc, 0, cT);
+          or_gemm(-1., 1., M, t2, D, m, n, o, ldm, o, ldd, 0, dT);
+          break;
+        case 1:
+          or_gemm(1., 0., t1, B, C, k, o, l, o, ldb, ldc, bT, cT);
+          or_gemm(1., 0., t2, A, t1, m, o, k, o, lda, o, aT, 0);
+          or_gemm(-1., 1., M, t2, D, m, n, o, ldm, o, ldd, 0, dT);
+          break;
+        case 2:
+          or_gemm(1., 0., t1, B, C, k, o, l, o, ldb, ldc, bT, cT);
+          or_gemm(1., 0., t2, t1, D, k, n, o, n, o, ldd, 0, dT);
+          or_gemm(-1., 1., M, A, t2, m, n, k, ldm, lda, n, aT, 0);
+          break;
+        case 3:
+          or_gemm(1., 0., t1, C, D, l, n, o, n, ldc, ldd, cT, dT);
+          or_gemm(1., 0., t2, B, t1, k, n, l, n, ldb, n, bT, 0);
+          or_gemm(-1., 1., M, A, t2, m, n, k, ldm, lda, n, aT, 0);
+          break;
+        case 4:
+          or_gemm(1., 0., t1, A, B, m, l, k, l, lda, ldb, aT, bT);
+          or_gemm(1., 0., t2, C, D, l, n, o, n, ldc, ldd, cT, dT);
+          or_gemm(-1., 1., M, t1, t2, m, n, l, ldm, l, n, 0, 0);
+          break;
+        default:
+          return -1;
+      }
+      return 29;
+    }
+    case 8: /* accum */
+      or_accum(P[w[3]] + w[7], P[w[4]] + w[8], P[w[5]] + w[9], P[w[6]] + w[10], w[11], w[12], w[13], w[14], w[15],
+               w[16], w[17], w[18], rnd);
+      return 21;
+    default:
+      return -1;
+  }
+}
+
+/* ------------------------------------------------------------------ stream runner */
+
+typedef struct {
+  const int32_t* const* streams;
+  int workers, nthreads, tid;
+  double* const* ptrs;
+  double* scratch;
+  int64_t stride;
+  const double* rnd;
+  atomic_int* flags;
+  atomic_int* failed;
+} runner_t;
+
+/* Advance worker w from *pc until it blocks on an unmet wait (returns 0),
+ * finishes (returns 1), or fails (-1).  Progress flag set when anything ran. */
+static int step_worker(runner_t* r, int w, int64_t* pc, int* progressed) {
+  const int32_t* s = r->streams[w];
+  for (;;) {
+    const int32_t op = s[*pc];
+    if (op == 0) return 1;
+    if (op == 2) {
+      if (atomic_load_explicit(&r->flags[s[*pc + 1]], memory_order_acquire) == 0) return 0;
+      *pc += 3;
+      *progressed = 1;
+      continue;
+    }
+    if (op != 1) return -1;
+    const int len = or_exec(&s[*pc], r->ptrs, r->scratch + (int64_t)w * r->stride, r->rnd);
+    if (len < 0) return -1;
+    atomic_store_explicit(&r->flags[s[*pc + 1]], 1, memory_order_release);
+    *pc += len;
+    *progressed = 1;
+  }
+}
+
+static void* thread_main(void* arg) {
+  runner_t* r = (runner_t*)arg;
+  int nmine = 0;
+  for (int w = r->tid; w < r->workers; w += r->nthreads) ++nmine;
+  int64_t* pcs = (int64_t*)calloc(nmine > 0 ? nmine : 1, sizeof(int64_t));
+  int* done = (int*)calloc(nmine > 0 ? nmine : 1, sizeof(int));
+  int left = nmine;
+  long idle = 0;
+  while (left > 0 && !atomic_load(r->failed)) {
+    int progressed = 0;
+    for (int q = 0; q < nmine; ++q) {
+      if (done[q]) continue;
+      const int st = step_worker(r, r->tid + q * r->nthreads, &pcs[q], &progressed);
+      if (st < 0) {
+        atomic_store(r->failed, 1);
+        break;
+      }
+      if (st == 1) {
+        done[q] = 1;
+        --left;
+      }
+    }
+    if (!progressed) {
+      if (r->nthreads == 1 || ++idle > 200000000L) {
+        atomic_store(r->failed, 1); /* single thread without progress: deadlock */
+        break;
+      }
+      sched_yield();
+    } else {
+      idle = 0;
+    }
+  }
+  free(pcs);
+  free(done);
+  return NULL;
+}
+
+int or_run_streams(const int32_t* const* streams, int workers, double* const* ptrs, double* scratch,
+                   int64_t scratch_stride, const double* rnd, int n_insts, int threads) {
+  if (threads < 1) threads = 1;
+  if (threads > workers) threads = workers;
+  atomic_int* flags = (atomic_int*)calloc(n_insts > 0 ? n_insts : 1, sizeof(atomic_int));
+  atomic_int failed;
+  atomic_init(&failed, 0);
+  runner_t* rs = (runner_t*)calloc(threads, sizeof(runner_t));
+  pthread_t* th = (pthread_t*)calloc(threads, sizeof(pthread_t));
+  for (int t = 0; t < threads; ++t) {
+    rs[t] = (runner_t){streams, workers, threads, t, ptrs, scratch, scratch_stride, rnd, flags, &failed};
+    if (threads == 1) thread_main(&rs[t]);
+    else pthread_create(&th[t], NULL, thread_main, &rs[t]);
+  }
+  if (threads > 1)
+    for (int t = 0; t < threads; ++t) pthread_join(th[t], NULL);
+  const int bad = atomic_load(&failed);
+  free(flags);
+  free(rs);
+  free(th);
+  return bad ? -1 : 0;
+}
+
+/* ------------------------------------------------------------------ dense oracle */
+
+typedef struct {
+  double* A;
+  int n, p, pb, t0, t1;
+} lu_task_t;
+
+static void* lu_trailing(void* arg) {
+  lu_task_t* t = (lu_task_t*)arg;
+  double* A = t->A;
+  const int n = t->n, p = t->p, pe = t->p + t->pb;
+  for (int r = t->t0; r < t->t1; ++r) {
+    double* row = &A[(int64_t)r * n];
+    for (int k = p; k < pe; ++k) {
+      const double l = row[k];
+      const double* prow = &A[(int64_t)k * n];
+      for (int c = pe; c < n; ++c) row[c] = fma(prow[c], -l, row[c]);
+    }
+  }
+  return NULL;
+}
+
+void or_dense_lu(double* A, int n, int threads) {
+  const int pb = 64;
+  if (threads < 1) threads = 1;
+  pthread_t th[256];
+  lu_task_t tk[256];
+  if (threads > 256) threads = 256;
+  for (int p = 0; p < n; p += pb) {
+    const int w = (n - p) < pb ? (n - p) : pb;
+    /* panel: unblocked LU of A[p:n, p:p+w] plus the U rows A[p:p+w, p+w:n] */
+    for (int i = p; i < p + w; ++i) {
+      const double piv = A[(int64_t)i * n + i];
+      for (int r = i + 1; r < n; ++r) {
+        double* row = &A[(int64_t)r * n];
+        const double l = row[i] / piv;
+        row[i] = l;
+        const int cend = r < p + w ? n : p + w; /* rows inside the panel update everything */
+        for (int c = i + 1; c < cend; ++c) row[c] = fma(A[(int64_t)i * n + c], -l, row[c]);
+      }
+    }
+    /* trailing update of rows p+w.. with the panel */
+    const int rows = n - p - w;
+    if (rows <= 0) continue;
+    const int nt = rows < threads ? rows : threads;
+    for (int t = 0; t < nt; ++t) {
+      tk[t] = (lu_task_t){A, n, p, w, p + w + (int)((int64_t)rows * t / nt), p + w + (int)((int64_t)rows * (t + 1) / nt)};
+      if (nt == 1) lu_trailing(&tk[t]);
+      else pthread_create(&th[t], NULL, lu_trailing, &tk[t]);
+    }
+    if (nt > 1)
+      for (int t = 0; t < nt; ++t) pthread_join(th[t], NULL);
+  }
+}
+
+void or_dense_lu_solve(const double* LU, int n, double* X, int nrhs, int ldx) {
+  for (int i = 0; i < n; ++i)
+    for (int k = 0; k < i; ++k) {
+      const double l = LU[(int64_t)i * n + k];
+      for (int c = 0; c < nrhs; ++c) X[(int64_t)i * ldx + c] -= l * X[(int64_t)k * ldx + c];
+    }
+  for (int i = n - 1; i >= 0; --i) {
+    for (int k = i + 1; k < n; ++k) {
+      const double u = LU[(int64_t)i * n + k];
+      for (int c = 0; c < nrhs; ++c) X[(int64_t)i * ldx + c] -= u * X[(int64_t)k * ldx + c];
+    }
+    const double d = LU[(int64_t)i * n + i];
+    for (int c = 0; c < nrhs; ++c) X[(int64_t)i * ldx + c] /= d;
+  }
+}
+
+double or_dense_residual(const double* A, int n, const double* X, const double* B, int nrhs, int ld) {
+  double worst = 0.;
+  double* rn = (double*)calloc(nrhs, sizeof(double));
+  double* bn = (double*)calloc(nrhs, sizeof(double));
+  for (int i = 0; i < n; ++i)
+    for (int c = 0; c < nrhs; ++c) {
+      double s = B[(int64_t)i * ld + c];
+      for (int k = 0; k < n; ++k) s -= A[(int64_t)i * n + k] * X[(int64_t)k * ld + c];
+      rn[c] += s * s;
+      bn[c] += B[(int64_t)i * ld + c] * B[(int64_t)i * ld + c];
+    }
+  for (int c = 0; c < nrhs; ++c) {
+    const double r = bn[c] > 0 ? sqrt(rn[c] / bn[c]) : sqrt(rn[c]);
+    if (r > worst) worst = r;
+  }
+  free(rn);
+  free(bn);
+  return worst;
+}
+
+void or_sketch(double* out, int64_t count, uint64_t seed) {
+  uint64_t st = seed;
+  for (int64_t i = 0; i < count; ++i) {
+    uint64_t z = (st += 0x9E3779B97F4A7C15ULL);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+    z ^= z >> 31;
+    out[i] = 2.0 * ((double)(z >> 11) * 0x1.0p-53) - 1.0;
+  }
+}

@@ -1,0 +1,341 @@
+// C ABI, host half: rts queries, H-matrix structures and programs.
+#include <chrono>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <new>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/pastel.h"
+#include "capi_common.h"
+#include "h_ops/h_ops.h"
+#include "instructions/instructions_manager.h"
+#include "matrix/Hierarchical.h"
+#include "rts/h_ops_dag.h"
+#include "rts/scheduler.h"
+
+namespace {
+thread_local std::string g_last_error;
+}
+
+namespace ppc {
+
+int set_error(int code, const std::string& msg) {
+  g_last_error = msg;
+  return code;
+}
+
+int from_exception() {
+  try {
+    throw;
+  } catch (const std::bad_alloc&) {
+    return set_error(PP_ERR_NOMEM, "host allocation failed");
+  } catch (const std::invalid_argument& e) {
+    return set_error(PP_ERR_INVALID, e.what());
+  } catch (const std::out_of_range& e) {
+    return set_error(PP_ERR_INVALID, e.what());
+  } catch (const std::exception& e) {
+    return set_error(PP_ERR_INTERNAL, e.what());
+  } catch (...) {
+    return set_error(PP_ERR_INTERNAL, "unknown exception");
+  }
+}
+
+}  // namespace ppc
+
+using ppc::from_exception;
+using ppc::set_error;
+
+struct pp_schedule {
+  std::unique_ptr<h_ops_dag> dag;
+  std::unique_ptr<instructions_scheduler> sched;
+};
+
+extern "C" {
+
+const char* pp_last_error(void) { return g_last_error.c_str(); }
+const char* pp_version(void) { return "pastel-mi355x 0.1"; }
+int64_t pp_min_inst_flops(void) { return _MIN_INST_FLOPS; }
+
+int pp_rts_check_overlap(uint64_t ptr_a, uint64_t pitch_a, uint64_t width_a, uint64_t height_a, uint64_t ptr_b,
+                         uint64_t pitch_b, uint64_t width_b, uint64_t height_b) {
+  try {
+    const DataMap2D a(reinterpret_cast<void*>(ptr_a), pitch_a, width_a, height_a);
+    const DataMap2D b(reinterpret_cast<void*>(ptr_b), pitch_b, width_b, height_b);
+    return a.checkOverlap(b) ? 1 : 0;
+  } catch (...) {
+    return from_exception();
+  }
+}
+
+int pp_rts_check_dependency(int n_from, const uint64_t* maps_from, const int32_t* ro_from, int n_to,
+                            const uint64_t* maps_to, const int32_t* ro_to) {
+  try {
+    if (n_from < 0 || n_to < 0) return set_error(PP_ERR_INVALID, "negative operand count");
+    Operation from, to;
+    for (int i = 0; i < n_from; ++i)
+      from.addData(ro_from[i] != 0, DataMap2D(reinterpret_cast<void*>(maps_from[4 * i]), maps_from[4 * i + 1],
+                                              maps_from[4 * i + 2], maps_from[4 * i + 3]));
+    for (int i = 0; i < n_to; ++i)
+      to.addData(ro_to[i] != 0, DataMap2D(reinterpret_cast<void*>(maps_to[4 * i]), maps_to[4 * i + 1],
+                                          maps_to[4 * i + 2], maps_to[4 * i + 3]));
+    return (int)to.checkDependencyFrom(from);
+  } catch (...) {
+    return from_exception();
+  }
+}
+
+int pp_schedule_create(int n, const int64_t* flops, int n_edges, const int32_t* edges, const int32_t* masks,
+                       int workers, pp_schedule** out) {
+  try {
+    if (out == nullptr || n < 0 || n_edges < 0 || workers < 1) return set_error(PP_ERR_INVALID, "bad schedule args");
+    std::vector<long long> f(flops, flops + n);
+    auto s = std::make_unique<pp_schedule>();
+    s->dag = std::make_unique<h_ops_dag>(n, f.data(), n_edges, edges, masks);
+    s->sched = std::make_unique<instructions_scheduler>(s->dag.get(), workers);
+    *out = s.release();
+    return PP_OK;
+  } catch (...) {
+    return from_exception();
+  }
+}
+
+int pp_schedule_worker_length(const pp_schedule* s, int worker) {
+  if (s == nullptr || worker < 0 || worker >= s->sched->getWorkers()) return set_error(PP_ERR_INVALID, "bad worker");
+  return s->sched->getLength(worker);
+}
+
+static int fill_entries(const instructions_scheduler* sc, int worker, int32_t* insts, int32_t* ex_w,
+                        int64_t* start_flops) {
+  if (worker < 0 || worker >= sc->getWorkers()) return set_error(PP_ERR_INVALID, "bad worker");
+  int i = 0;
+  for (const instructions_queue* q = sc->getSchedule(worker); q != nullptr; q = q->getNext(), ++i) {
+    if (insts) insts[i] = q->getInst();
+    if (ex_w) ex_w[i] = q->getExW() ? 1 : 0;
+    if (start_flops) start_flops[i] = q->getStartingFlops();
+  }
+  return i;
+}
+
+int pp_schedule_worker_entries(const pp_schedule* s, int worker, int32_t* insts, int32_t* ex_w, int64_t* start_flops) {
+  if (s == nullptr) return set_error(PP_ERR_INVALID, "null schedule");
+  return fill_entries(s->sched.get(), worker, insts, ex_w, start_flops);
+}
+
+int pp_schedule_comm_waits(const pp_schedule* s) { return s ? s->sched->getCommWaitCounts() : PP_ERR_INVALID; }
+
+void pp_schedule_destroy(pp_schedule* s) { delete s; }
+
+// ------------------------------------------------------------------ H-matrix
+
+static int make_hmatrix(Hierarchical* h, pp_hmatrix** out) {
+  auto m = std::make_unique<pp_hmatrix>();
+  m->h.reset(h);
+  m->h->leaves(m->leaves);
+  *out = m.release();
+  return PP_OK;
+}
+
+int pp_hmatrix_create_hodlr(int n, int levels, int rank, pp_hmatrix** out) {
+  try {
+    if (out == nullptr) return set_error(PP_ERR_INVALID, "null out");
+    return make_hmatrix(Hierarchical::hodlr(n, levels, rank, 0), out);
+  } catch (...) {
+    return from_exception();
+  }
+}
+
+int pp_hmatrix_create_dense(int n, int leaf, pp_hmatrix** out) {
+  try {
+    if (out == nullptr) return set_error(PP_ERR_INVALID, "null out");
+    return make_hmatrix(Hierarchical::denseGrid(n, leaf, 0), out);
+  } catch (...) {
+    return from_exception();
+  }
+}
+
+void pp_hmatrix_destroy(pp_hmatrix* h) { delete h; }
+int pp_hmatrix_size(const pp_hmatrix* h) { return h ? h->h->getNx_abs() : PP_ERR_INVALID; }
+int64_t pp_hmatrix_storage(const pp_hmatrix* h) { return h ? h->h->getStorage() : PP_ERR_INVALID; }
+int pp_hmatrix_num_leaves(const pp_hmatrix* h) { return h ? (int)h->leaves.size() : PP_ERR_INVALID; }
+
+int pp_hmatrix_leaf(const pp_hmatrix* h, int i, int64_t info[8]) {
+  if (h == nullptr || i < 0 || i >= (int)h->leaves.size()) return set_error(PP_ERR_INVALID, "bad leaf index");
+  const h_block* b = h->leaves[i];
+  info[0] = (int64_t)b->type;
+  info[1] = b->row0;
+  info[2] = b->col0;
+  info[3] = b->ny;
+  info[4] = b->nx;
+  info[5] = b->rank;
+  info[6] = b->type == element_t::dense ? b->off : b->off_u;
+  info[7] = b->off_v;
+  return PP_OK;
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------------ programs
+
+pp_program::~pp_program() = default;
+
+namespace {
+
+double ms_since(std::chrono::steady_clock::time_point t0) {
+  return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+}
+
+// tree -> DAG -> schedule -> words
+void finish_program(pp_program* p, int workers) {
+  auto t0 = std::chrono::steady_clock::now();
+  p->dag = std::make_unique<h_ops_dag>(&p->tree, 0, 0);
+  p->dag_ms = ms_since(t0);
+  t0 = std::chrono::steady_clock::now();
+  p->sched = std::make_unique<instructions_scheduler>(p->dag.get(), workers);
+  p->schedule_ms = ms_since(t0);
+  t0 = std::chrono::steady_clock::now();
+  p->mgr = std::make_unique<instructions_manager>(workers, p->dag.get(), p->sched.get(), &p->tree, 0);
+  p->encode_ms = ms_since(t0);
+  p->workers = workers;
+  p->scratch = p->tree.maxScratch();
+}
+
+}  // namespace
+
+extern "C" {
+
+int pp_program_factor_batch(const pp_hmatrix* const* hs, int count, int workers, pp_program** out) {
+  try {
+    if (hs == nullptr || count < 1 || workers < 1 || out == nullptr) return set_error(PP_ERR_INVALID, "bad args");
+    auto p = std::make_unique<pp_program>();
+    p->n_slots = 3 * count;
+    for (int i = 0; i < count; ++i) {
+      if (hs[i] == nullptr) return set_error(PP_ERR_INVALID, "null matrix in batch");
+      dev_temp tmp(3 * i + 1);
+      std::unique_ptr<const h_index> root(hs[i]->h->getRootIndex());
+      std::unique_ptr<h_ops_tree> t(hs[i]->h->generateOps_GETRF(root.get(), &tmp, 3 * i));
+      p->tree.append(*t);
+      p->temp_elems.push_back(tmp.size());
+    }
+    finish_program(p.get(), workers);
+    *out = p.release();
+    return PP_OK;
+  } catch (...) {
+    return from_exception();
+  }
+}
+
+int pp_program_factor(const pp_hmatrix* h, int workers, pp_program** out) {
+  return pp_program_factor_batch(&h, 1, workers, out);
+}
+
+int pp_program_solve_batch(const pp_hmatrix* const* hs, int count, const int32_t* nrhs, const int32_t* ld,
+                           int rhs_split, int workers, pp_program** out) {
+  try {
+    if (hs == nullptr || count < 1 || workers < 1 || out == nullptr || nrhs == nullptr || ld == nullptr)
+      return set_error(PP_ERR_INVALID, "bad args");
+    auto p = std::make_unique<pp_program>();
+    p->n_slots = 3 * count;
+    for (int i = 0; i < count; ++i) {
+      if (hs[i] == nullptr) return set_error(PP_ERR_INVALID, "null matrix in batch");
+      std::unique_ptr<const h_index> root(hs[i]->h->getRootIndex());
+      std::unique_ptr<h_ops_tree> t(
+          hs[i]->h->generateOps_Solve(root.get(), 3 * i + 2, 0, nrhs[i], ld[i], rhs_split, 3 * i));
+      p->tree.append(*t);
+      p->temp_elems.push_back(0);
+    }
+    finish_program(p.get(), workers);
+    *out = p.release();
+    return PP_OK;
+  } catch (...) {
+    return from_exception();
+  }
+}
+
+int pp_program_solve(const pp_hmatrix* h, int nrhs, int ld, int rhs_split, int workers, pp_program** out) {
+  const int32_t n = nrhs, l = ld;
+  return pp_program_solve_batch(&h, 1, &n, &l, rhs_split, workers, out);
+}
+
+void pp_program_destroy(pp_program* p) { delete p; }
+
+int pp_program_get_info(const pp_program* p, pp_program_info* info) {
+  if (p == nullptr || info == nullptr) return set_error(PP_ERR_INVALID, "null");
+  std::memset(info, 0, sizeof(*info));
+  info->n_insts = p->dag->getLength();
+  info->workers = p->workers;
+  info->n_slots = p->n_slots;
+  info->n_edges = p->dag->getEdgeCount();
+  info->flops = p->dag->getFlops();
+  int64_t t = 0;
+  for (int64_t e : p->temp_elems) t += e;
+  info->temp_elems = t;
+  info->scratch_elems = p->scratch;
+  info->total_words = p->mgr->totalWords();
+  info->comm_waits = p->sched->getCommWaitCounts();
+  info->dag_density = p->dag->dag_density();
+  info->schedule_ms = p->schedule_ms;
+  info->dag_ms = p->dag_ms;
+  info->encode_ms = p->encode_ms;
+  return PP_OK;
+}
+
+int64_t pp_program_temp_elems(const pp_program* p, int member) {
+  if (p == nullptr || member < 0 || member >= (int)p->temp_elems.size()) return set_error(PP_ERR_INVALID, "bad member");
+  return p->temp_elems[member];
+}
+
+int pp_program_worker_length(const pp_program* p, int worker) {
+  if (p == nullptr || worker < 0 || worker >= p->workers) return set_error(PP_ERR_INVALID, "bad worker");
+  return (int)p->mgr->getStream(worker).size();
+}
+
+int pp_program_worker_words(const pp_program* p, int worker, int32_t* out, int cap) {
+  if (p == nullptr || worker < 0 || worker >= p->workers) return set_error(PP_ERR_INVALID, "bad worker");
+  const auto& s = p->mgr->getStream(worker);
+  const int n = (int)s.size();
+  if (out != nullptr) std::memcpy(out, s.data(), sizeof(int32_t) * (size_t)std::min(n, cap));
+  return n;
+}
+
+int pp_program_inst_words(const pp_program* p, int inst, int32_t* out, int cap) {
+  if (p == nullptr || inst < 0 || inst >= p->tree.length()) return set_error(PP_ERR_INVALID, "bad instruction");
+  const auto& w = p->tree.getOp(inst).words;
+  const int n = (int)w.size();
+  if (out != nullptr) {
+    std::memcpy(out, w.data(), sizeof(int32_t) * (size_t)std::min(n, cap));
+    if (cap > 1) out[1] = inst;
+  }
+  return n;
+}
+
+int64_t pp_program_inst_flops(const pp_program* p, int inst) {
+  if (p == nullptr || inst < 0 || inst >= p->tree.length()) return set_error(PP_ERR_INVALID, "bad instruction");
+  return p->tree.getOp(inst).flops;
+}
+
+int pp_program_inst_preds(const pp_program* p, int inst, int32_t* preds, int32_t* masks, int cap) {
+  if (p == nullptr || inst < 0 || inst >= p->dag->getLength()) return set_error(PP_ERR_INVALID, "bad instruction");
+  int n = 0;
+  for (const dependency_linked_list* d = p->dag->getDepList_To(inst); d != nullptr; d = d->getNext(), ++n) {
+    if (n < cap) {
+      if (preds) preds[n] = d->getInst();
+      if (masks) masks[n] = (int)d->getDep();
+    }
+  }
+  return n;
+}
+
+int pp_program_schedule_length(const pp_program* p, int worker) {
+  if (p == nullptr || worker < 0 || worker >= p->workers) return set_error(PP_ERR_INVALID, "bad worker");
+  return p->sched->getLength(worker);
+}
+
+int pp_program_schedule_entries(const pp_program* p, int worker, int32_t* insts, int32_t* ex_w, int64_t* start_flops) {
+  if (p == nullptr) return set_error(PP_ERR_INVALID, "null program");
+  return fill_entries(p->sched.get(), worker, insts, ex_w, start_flops);
+}
+
+}  // extern "C"

@@ -1,0 +1,598 @@
+// Leaf-op emitters and the H-LU / H-solve op generators.
+//
+// The reference's task-tree generator (h_ops/*, src/launcher.cu:138-139) is
+// absent from its snapshot; docs/README.md:23-31 defines the h-ops GETRF,
+// TRSM, GEMM (A = A - B*C) and ACCM (A = A + B).  This file fixes the
+// decomposition used here (right-looking block LU without pivoting):
+//
+//   getrf(H):  for i: getrf(H_ii); trsml(H_ii, H_ij) for j > i;
+//              trsmr(H_ii, H_ji) for j > i; gemm(H_jk, H_ji, H_ik) for j,k > i
+//   trsml(L, B), trsmr(U, B): recurse over the hierarchical factor, applying
+//              a low-rank B = U_B V_B^T to U_B (left) or V_B (right)
+//   gemm(M, A, B) with A, B low rank and M hierarchical or low rank:
+//              T = V_A^T U_B (once), W = -V_B T^T (once), then M += U_A W^T
+//              distributed over M's leaves: dense leaves by gemm_plus,
+//              low-rank leaves by accum.
+//   dense M with low-rank operands uses gemm_3x / gemm_4x directly.
+//
+// Ops are emitted in program order; the DAG is derived from their operand
+// maps alone.
+#include "h_ops.h"
+
+#include <algorithm>
+#include <stdexcept>
+#include <string>
+
+#include "../matrix/Hierarchical.h"
+
+namespace {
+
+int64_t align32(int64_t x) { return (x + 31) / 32 * 32; }
+
+void require(bool ok, const char* what) {
+  if (!ok) throw std::invalid_argument(std::string("h_ops: ") + what);
+}
+
+h_op make_op(operation_t op, std::initializer_list<int> args) {
+  h_op o;
+  o.op = op;
+  o.words.reserve(3 + args.size());
+  o.words.push_back(execute);
+  o.words.push_back(0);
+  o.words.push_back(op);
+  for (int a : args) o.words.push_back(a);
+  return o;
+}
+
+int off32(int64_t off) {
+  if (off < 0 || off > 0x7fffffffLL) throw std::out_of_range("h_ops: element offset exceeds int32");
+  return (int)off;
+}
+
+}  // namespace
+
+int operationLength(int op) {
+  switch (op) {
+    case nop: return nop_l;
+    case getrf: return getrf_l;
+    case trsml: return trsml_l;
+    case trsmr: return trsmr_l;
+    case gemm: return gemm_l;
+    case gemm_plus: return gemm_plus_l;
+    case gemm_3x: return gemm_3x_l;
+    case gemm_4x: return gemm_4x_l;
+    case accum: return accum_l;
+    case trsmu: return trsmu_l;
+    default: return -1;
+  }
+}
+
+// ------------------------------------------------------------------ FLOP model
+
+long long flops_getrf(long long nx, long long ny) {
+  long long f = 0;
+  const long long mn = std::min(nx, ny);
+  for (long long i = 0; i < mn; ++i) f += (ny - i - 1) + 2 * (ny - i - 1) * (nx - i - 1);
+  return f;
+}
+
+long long flops_trsml(long long nx_b, long long ny_b) { return nx_b * ny_b * (ny_b - 1); }
+long long flops_trsmr(long long nx_b, long long ny_b) { return ny_b * nx_b * nx_b; }
+long long flops_gemm(long long m, long long n, long long k) { return 2 * m * n * k; }
+
+long long getFlops_GETRF(long long nx, long long ny) {
+  const long long n = std::min(nx, ny);
+  return n * (n - 1) / 2 + n * (n - 1) * (2 * n - 1) / 3;
+}
+
+void h_ops_tree::append(const h_ops_tree& other) {
+  for (const h_op& o : other.ops) ops.push_back(o);
+}
+
+long long h_ops_tree::getFlops() const {
+  long long f = 0;
+  for (const h_op& o : ops) f += o.flops;
+  return f;
+}
+
+int64_t h_ops_tree::maxScratch() const {
+  int64_t s = 0;
+  for (const h_op& o : ops) s = std::max(s, o.scratch);
+  return s;
+}
+
+dview dev_temp::request(int ny, int nx) {
+  dview v;
+  v.ptr = ptr_id;
+  v.off = used;
+  v.ny = ny;
+  v.nx = nx;
+  v.ld = nx;
+  used = align32(used + (int64_t)ny * nx);
+  return v;
+}
+
+void accum_scratch(int nx, int ny, int k1, int k2, int64_t* offset1, int64_t* offset2, int64_t* total) {
+  const int kmax = std::max(k1, k2);
+  *offset1 = align32((int64_t)ny * k1);
+  *offset2 = align32(*offset1 + (int64_t)nx * k1);
+  *total = align32(*offset2 + (int64_t)ny * k1) + align32(4LL * k1 * kmax + 2LL * kmax);
+}
+
+// ------------------------------------------------------------------ emitters
+
+void emit_getrf(h_ops_tree* tree, const dview& m) {
+  h_op o = make_op(getrf, {m.ptr, off32(m.off), m.nx, m.ny, m.ld});
+  o.data.addData(false, m.map());
+  o.flops = flops_getrf(m.nx, m.ny);
+  tree->push(std::move(o));
+}
+
+void emit_trsml(h_ops_tree* tree, const mview& b, const dview& l) {
+  const int ny_b = b.rows(), nx_b = b.cols();
+  require(l.ny == ny_b && l.nx == ny_b, "trsml: L must be square with B's row count");
+  h_op o = make_op(trsml, {b.v.ptr, l.ptr, off32(b.v.off), off32(l.off), nx_b, ny_b, l.nx, b.v.ld, l.ld, b.t ? 1 : 0});
+  o.data.addData(false, b.v.map());
+  o.data.addData(true, l.map());
+  o.flops = flops_trsml(nx_b, ny_b);
+  tree->push(std::move(o));
+}
+
+void emit_trsmr(h_ops_tree* tree, const mview& b, const dview& u) {
+  const int ny_b = b.rows(), nx_b = b.cols();
+  require(u.ny == nx_b && u.nx == nx_b, "trsmr: U must be square with B's column count");
+  h_op o = make_op(trsmr, {b.v.ptr, u.ptr, off32(b.v.off), off32(u.off), nx_b, ny_b, u.ny, b.v.ld, u.ld, b.t ? 1 : 0});
+  o.data.addData(false, b.v.map());
+  o.data.addData(true, u.map());
+  o.flops = flops_trsmr(nx_b, ny_b);
+  tree->push(std::move(o));
+}
+
+void emit_trsmu(h_ops_tree* tree, const dview& b, const dview& u) {
+  require(u.ny == b.ny && u.nx == b.ny, "trsmu: U must be square with B's row count");
+  h_op o = make_op(trsmu, {b.ptr, u.ptr, off32(b.off), off32(u.off), b.nx, b.ny, u.ny, b.ld, u.ld, 0});
+  o.data.addData(false, b.map());
+  o.data.addData(true, u.map());
+  o.flops = (long long)b.nx * b.ny * b.ny;
+  tree->push(std::move(o));
+}
+
+void emit_gemm(h_ops_tree* tree, mview m, mview a, mview b, bool plus) {
+  if (m.t) {  // M^T -/+= A B  <=>  M -/+= B^T A^T
+    mview a2 = b.T(), b2 = a.T();
+    m = m.T();
+    a = a2;
+    b = b2;
+  }
+  const int M = m.rows(), N = m.cols(), K = a.cols();
+  require(a.rows() == M && b.rows() == K && b.cols() == N, "gemm: shape mismatch");
+  h_op o = make_op(plus ? gemm_plus : gemm,
+                   {m.v.ptr, a.v.ptr, b.v.ptr, off32(m.v.off), off32(a.v.off), off32(b.v.off), M, N, K, m.v.ld, a.v.ld,
+                    b.v.ld, a.t ? 1 : 0, b.t ? 1 : 0});
+  o.data.addData(false, m.v.map());
+  o.data.addData(true, a.v.map());
+  o.data.addData(true, b.v.map());
+  o.flops = flops_gemm(M, N, K);
+  tree->push(std::move(o));
+}
+
+void emit_gemm3(h_ops_tree* tree, mview m, mview a, mview b, mview c) {
+  if (m.t) {
+    mview a2 = c.T(), b2 = b.T(), c2 = a.T();
+    m = m.T();
+    a = a2;
+    b = b2;
+    c = c2;
+  }
+  const long long M = m.rows(), N = m.cols(), K = a.cols(), L = b.cols();
+  require(a.rows() == M && b.rows() == K && c.rows() == L && c.cols() == N, "gemm_3x: shape mismatch");
+  const long long f_ab_c = 2 * M * K * L + 2 * M * L * N;  // (A B) C, control 1
+  const long long f_a_bc = 2 * K * N * L + 2 * M * N * K;  // A (B C), control 0
+  const int control = f_ab_c <= f_a_bc ? 1 : 0;
+  h_op o = make_op(gemm_3x, {m.v.ptr, a.v.ptr, b.v.ptr, c.v.ptr, off32(m.v.off), off32(a.v.off), off32(b.v.off),
+                             off32(c.v.off), (int)M, (int)N, (int)K, (int)L, m.v.ld, a.v.ld, b.v.ld, c.v.ld,
+                             a.t ? 1 : 0, b.t ? 1 : 0, c.t ? 1 : 0, control});
+  o.data.addData(false, m.v.map());
+  o.data.addData(true, a.v.map());
+  o.data.addData(true, b.v.map());
+  o.data.addData(true, c.v.map());
+  o.flops = std::min(f_ab_c, f_a_bc);
+  o.scratch = control ? M * L : K * N;
+  tree->push(std::move(o));
+}
+
+void emit_gemm4(h_ops_tree* tree, mview m, mview a, mview b, mview c, mview d) {
+  if (m.t) {
+    mview a2 = d.T(), b2 = c.T(), c2 = b.T(), d2 = a.T();
+    m = m.T();
+    a = a2;
+    b = b2;
+    c = c2;
+    d = d2;
+  }
+  const long long M = m.rows(), N = m.cols(), K = a.cols(), L = b.cols(), O = c.cols();
+  require(a.rows() == M && b.rows() == K && c.rows() == L && d.rows() == O && d.cols() == N, "gemm_4x: shape mismatch");
+  // association orders of src/kernel.cu:914-953: FLOPs, first temp, second temp
+  const long long f[5] = {2 * (M * K * L + M * L * O + M * O * N), 2 * (K * L * O + M * K * O + M * O * N),
+                          2 * (K * L * O + K * O * N + M * K * N), 2 * (L * O * N + K * L * N + M * K * N),
+                          2 * (M * K * L + L * O * N + M * L * N)};
+  const long long t1[5] = {M * L, K * O, K * O, L * N, M * L};
+  const long long t2[5] = {M * O, M * O, K * N, K * N, L * N};
+  int control = 0;
+  for (int i = 1; i < 5; ++i)
+    if (f[i] < f[control]) control = i;
+  const int64_t offset = align32(t1[control]);
+  h_op o = make_op(gemm_4x, {m.v.ptr, a.v.ptr, b.v.ptr, c.v.ptr, d.v.ptr, off32(m.v.off), off32(a.v.off),
+                             off32(b.v.off), off32(c.v.off), off32(d.v.off), (int)M, (int)N, (int)K, (int)L, (int)O,
+                             m.v.ld, a.v.ld, b.v.ld, c.v.ld, d.v.ld, a.t ? 1 : 0, b.t ? 1 : 0, c.t ? 1 : 0,
+                             d.t ? 1 : 0, control, off32(offset)});
+  o.data.addData(false, m.v.map());
+  o.data.addData(true, a.v.map());
+  o.data.addData(true, b.v.map());
+  o.data.addData(true, c.v.map());
+  o.data.addData(true, d.v.map());
+  o.flops = f[control];
+  o.scratch = offset + t2[control];
+  tree->push(std::move(o));
+}
+
+void emit_accum(h_ops_tree* tree, const dview& u1, const dview& v1, const dview& u2, const dview& v2) {
+  const int ny = u1.ny, nx = v1.ny, k1 = u1.nx, k2 = u2.nx;
+  require(v1.nx == k1 && u2.ny == ny && v2.ny == nx && v2.nx == k2, "accum: shape mismatch");
+  int64_t o1, o2, total;
+  accum_scratch(nx, ny, k1, k2, &o1, &o2, &total);
+  h_op o = make_op(accum, {u1.ptr, v1.ptr, u2.ptr, v2.ptr, off32(u1.off), off32(v1.off), off32(u2.off),
+                           off32(v2.off), nx, ny, k1, k2, u1.ld, v1.ld, u2.ld, v2.ld, off32(o1), off32(o2)});
+  o.data.addData(false, u1.map());
+  o.data.addData(false, v1.map());
+  o.data.addData(true, u2.map());
+  o.data.addData(true, v2.map());
+  const long long NX = nx, NY = ny, K1 = k1, K2 = k2;
+  o.flops = 2 * K1 * K1 * NX + 2 * NY * K1 * K1 + 2 * K1 * K2 * NX + 2 * NY * K1 * K2  // sketch
+            + 4 * NY * K1 * K1                                                        // QR + explicit Q
+            + 2 * K1 * K1 * NY + 2 * NX * K1 * K1 + 2 * K1 * K2 * NY + 2 * NX * K1 * K2;  // projection
+  o.scratch = total;
+  tree->push(std::move(o));
+}
+
+// ------------------------------------------------------------------ generator
+
+namespace {
+
+struct operand {
+  element_t type = element_t::empty;
+  int rows = 0, cols = 0;
+  mview d;        // dense
+  mview u, v;     // low rank: logical = u v^T
+  const h_block* h = nullptr;
+};
+
+operand of_block(const h_block* b, int ptr) {
+  operand o;
+  o.type = b->type;
+  o.rows = b->ny;
+  o.cols = b->nx;
+  if (b->type == element_t::dense) {
+    o.d = mview{dview{ptr, b->off, b->ny, b->nx, b->nx}, false};
+  } else if (b->type == element_t::low_rank) {
+    o.u = mview{dview{ptr, b->off_u, b->ny, b->rank, b->rank}, false};
+    o.v = mview{dview{ptr, b->off_v, b->nx, b->rank, b->rank}, false};
+  } else if (b->type == element_t::hierarchical) {
+    o.h = b;
+  } else {
+    throw std::invalid_argument("h_ops: empty block");
+  }
+  return o;
+}
+
+operand of_dense(const mview& m) {
+  operand o;
+  o.type = element_t::dense;
+  o.rows = m.rows();
+  o.cols = m.cols();
+  o.d = m;
+  return o;
+}
+
+// Row / column cut points of a hierarchical operand.
+std::vector<int> prefix(const std::vector<int>& sizes) {
+  std::vector<int> p(1, 0);
+  for (int s : sizes) p.push_back(p.back() + s);
+  return p;
+}
+
+class op_builder {
+public:
+  h_ops_tree* tree;
+  dev_temp* tmp;
+  int ptr;  // matrix slot
+
+  op_builder(h_ops_tree* t, dev_temp* d, int p) : tree(t), tmp(d), ptr(p) {}
+
+  operand child(const operand& o, int i, int j) const { return of_block(o.h->child(i, j), ptr); }
+
+  // sub-block of a dense or low-rank operand
+  static operand cut(const operand& o, int r0, int c0, int h, int w) {
+    operand s;
+    s.type = o.type;
+    s.rows = h;
+    s.cols = w;
+    if (o.type == element_t::dense) {
+      s.d = o.d.sub(r0, c0, h, w);
+    } else if (o.type == element_t::low_rank) {
+      s.u = o.u.sub(r0, 0, h, o.u.cols());
+      s.v = o.v.sub(c0, 0, w, o.v.cols());
+    } else {
+      throw std::invalid_argument("h_ops: cannot cut a hierarchical operand");
+    }
+    return s;
+  }
+
+  // ---- LU
+  void getrf(const operand& a) {
+    if (a.type == element_t::dense) {
+      require(!a.d.t, "getrf: transposed block");
+      emit_getrf(tree, a.d.v);
+      return;
+    }
+    require(a.type == element_t::hierarchical, "getrf: diagonal block must be dense or hierarchical");
+    const h_block* h = a.h;
+    require(h->py == h->px && h->row_sizes == h->col_sizes, "getrf: diagonal block must have a square partition");
+    const int p = h->py;
+    for (int i = 0; i < p; ++i) {
+      getrf(child(a, i, i));
+      for (int j = i + 1; j < p; ++j) trsml(child(a, i, i), child(a, i, j));
+      for (int j = i + 1; j < p; ++j) trsmr(child(a, i, i), child(a, j, i));
+      for (int j = i + 1; j < p; ++j)
+        for (int k = i + 1; k < p; ++k) gemm(child(a, j, k), child(a, j, i), child(a, i, k));
+    }
+  }
+
+  // B <- L^{-1} B with the unit-lower part of L
+  void trsml(const operand& l, const operand& b) {
+    if (b.type == element_t::low_rank) {
+      trsml(l, of_dense(b.u));
+      return;
+    }
+    if (l.type == element_t::dense) {
+      if (b.type == element_t::dense) {
+        require(!l.d.t, "trsml: transposed L");
+        emit_trsml(tree, b.d, l.d.v);
+        return;
+      }
+      // dense L against hierarchical B: cut L along B's rows
+      const h_block* hb = b.h;
+      const std::vector<int> rp = prefix(hb->row_sizes);
+      for (int i = 0; i < hb->py; ++i) {
+        const operand lii = cut(l, rp[i], rp[i], hb->row_sizes[i], hb->row_sizes[i]);
+        for (int j = 0; j < hb->px; ++j) trsml(lii, child(b, i, j));
+        for (int k = i + 1; k < hb->py; ++k) {
+          const operand lki = cut(l, rp[k], rp[i], hb->row_sizes[k], hb->row_sizes[i]);
+          for (int j = 0; j < hb->px; ++j) gemm(child(b, k, j), lki, child(b, i, j));
+        }
+      }
+      return;
+    }
+    const h_block* hl = l.h;
+    const std::vector<int> rp = prefix(hl->row_sizes);
+    if (b.type == element_t::dense) {
+      for (int i = 0; i < hl->py; ++i) {
+        const operand bi = cut(b, rp[i], 0, hl->row_sizes[i], b.cols);
+        trsml(child(l, i, i), bi);
+        for (int k = i + 1; k < hl->py; ++k)
+          gemm(cut(b, rp[k], 0, hl->row_sizes[k], b.cols), child(l, k, i), bi);
+      }
+      return;
+    }
+    const h_block* hb = b.h;
+    require(hb->row_sizes == hl->row_sizes, "trsml: partition mismatch");
+    for (int i = 0; i < hl->py; ++i) {
+      for (int j = 0; j < hb->px; ++j) trsml(child(l, i, i), child(b, i, j));
+      for (int k = i + 1; k < hl->py; ++k)
+        for (int j = 0; j < hb->px; ++j) gemm(child(b, k, j), child(l, k, i), child(b, i, j));
+    }
+  }
+
+  // B <- B U^{-1} with the upper part of U
+  void trsmr(const operand& u, const operand& b) {
+    if (b.type == element_t::low_rank) {
+      trsmr(u, of_dense(b.v.T()));  // V^T <- V^T U^{-1}
+      return;
+    }
+    if (u.type == element_t::dense) {
+      if (b.type == element_t::dense) {
+        require(!u.d.t, "trsmr: transposed U");
+        emit_trsmr(tree, b.d, u.d.v);
+        return;
+      }
+      const h_block* hb = b.h;
+      const std::vector<int> cp = prefix(hb->col_sizes);
+      for (int j = 0; j < hb->px; ++j) {
+        const operand ujj = cut(u, cp[j], cp[j], hb->col_sizes[j], hb->col_sizes[j]);
+        for (int i = 0; i < hb->py; ++i) trsmr(ujj, child(b, i, j));
+        for (int k = j + 1; k < hb->px; ++k) {
+          const operand ujk = cut(u, cp[j], cp[k], hb->col_sizes[j], hb->col_sizes[k]);
+          for (int i = 0; i < hb->py; ++i) gemm(child(b, i, k), child(b, i, j), ujk);
+        }
+      }
+      return;
+    }
+    const h_block* hu = u.h;
+    const std::vector<int> cp = prefix(hu->col_sizes);
+    if (b.type == element_t::dense) {
+      for (int j = 0; j < hu->px; ++j) {
+        const operand bj = cut(b, 0, cp[j], b.rows, hu->col_sizes[j]);
+        trsmr(child(u, j, j), bj);
+        for (int k = j + 1; k < hu->px; ++k)
+          gemm(cut(b, 0, cp[k], b.rows, hu->col_sizes[k]), bj, child(u, j, k));
+      }
+      return;
+    }
+    const h_block* hb = b.h;
+    require(hb->col_sizes == hu->col_sizes, "trsmr: partition mismatch");
+    for (int j = 0; j < hu->px; ++j) {
+      for (int i = 0; i < hb->py; ++i) trsmr(child(u, j, j), child(b, i, j));
+      for (int k = j + 1; k < hu->px; ++k)
+        for (int i = 0; i < hb->py; ++i) gemm(child(b, i, k), child(b, i, j), child(u, j, k));
+    }
+  }
+
+  // M += U W^T, U rows x r, W cols x r
+  void add_lr(const operand& m, const mview& u, const mview& w) {
+    if (m.type == element_t::dense) {
+      emit_gemm(tree, m.d, u, w.T(), true);
+    } else if (m.type == element_t::low_rank) {
+      require(!m.u.t && !m.v.t && !u.t && !w.t, "accum: transposed factor");
+      emit_accum(tree, m.u.v, m.v.v, u.v, w.v);
+    } else {
+      const h_block* h = m.h;
+      const std::vector<int> rp = prefix(h->row_sizes), cp = prefix(h->col_sizes);
+      for (int i = 0; i < h->py; ++i)
+        for (int j = 0; j < h->px; ++j)
+          add_lr(child(m, i, j), u.sub(rp[i], 0, h->row_sizes[i], u.cols()),
+                 w.sub(cp[j], 0, h->col_sizes[j], w.cols()));
+    }
+  }
+
+  // M -= A B
+  void gemm(const operand& m, const operand& a, const operand& b) {
+    require(m.rows == a.rows && m.cols == b.cols && a.cols == b.rows, "gemm: operand shapes");
+    const bool a_lr = a.type == element_t::low_rank, b_lr = b.type == element_t::low_rank;
+    const bool a_d = a.type == element_t::dense, b_d = b.type == element_t::dense;
+
+    if (m.type == element_t::dense) {
+      if (a_d && b_d) return emit_gemm(tree, m.d, a.d, b.d, false);
+      if (a_lr && b_d) return emit_gemm3(tree, m.d, a.u, a.v.T(), b.d);
+      if (a_d && b_lr) return emit_gemm3(tree, m.d, a.d, b.u, b.v.T());
+      if (a_lr && b_lr) return emit_gemm4(tree, m.d, a.u, a.v.T(), b.u, b.v.T());
+    }
+
+    if (m.type != element_t::dense && (a_lr || b_lr) && (a_lr || a_d) && (b_lr || b_d)) {
+      // low-rank product P = A B = -X Y^T
+      mview x, y;
+      if (a_lr && b_lr) {
+        const dview t = tmp->request(a.u.cols(), b.u.cols());
+        emit_gemm(tree, mview{t, false}, a.v.T(), b.u, true);           // T = V_A^T U_B
+        const dview w = tmp->request(b.cols, a.u.cols());
+        emit_gemm(tree, mview{w, false}, b.v, mview{t, true}, false);   // W = -V_B T^T
+        x = a.u;
+        y = mview{w, false};
+      } else if (a_d) {
+        const dview w = tmp->request(a.rows, b.u.cols());
+        emit_gemm(tree, mview{w, false}, a.d, b.u, false);               // W = -A U_B
+        x = mview{w, false};
+        y = b.v;
+      } else {
+        const dview w = tmp->request(b.cols, a.u.cols());
+        emit_gemm(tree, mview{w, false}, b.d.T(), a.v, false);           // W = -B^T V_A
+        x = a.u;
+        y = mview{w, false};
+      }
+      add_lr(m, x, y);  // M -= P  <=>  M += X Y^T
+      return;
+    }
+
+    // structural recursion: cut operands along the hierarchical partitions
+    std::vector<int> rows, cols, inner;
+    if (m.type == element_t::hierarchical) {
+      rows = m.h->row_sizes;
+      cols = m.h->col_sizes;
+    } else {
+      rows = a.type == element_t::hierarchical ? a.h->row_sizes : std::vector<int>{m.rows};
+      cols = b.type == element_t::hierarchical ? b.h->col_sizes : std::vector<int>{m.cols};
+    }
+    if (a.type == element_t::hierarchical) inner = a.h->col_sizes;
+    else if (b.type == element_t::hierarchical) inner = b.h->row_sizes;
+    else inner = {a.cols};
+    require(m.type != element_t::low_rank, "gemm: unsupported low-rank target with hierarchical operands");
+
+    const std::vector<int> rp = prefix(rows), cp = prefix(cols), kp = prefix(inner);
+    auto piece = [&](const operand& o, const std::vector<int>& rs, const std::vector<int>& cs, int i, int j,
+                     const std::vector<int>& rpp, const std::vector<int>& cpp) -> operand {
+      if (o.type == element_t::hierarchical) {
+        require(o.h->row_sizes == rs && o.h->col_sizes == cs, "gemm: partition mismatch");
+        return child(o, i, j);
+      }
+      return cut(o, rpp[i], cpp[j], rs[i], cs[j]);
+    };
+    for (size_t i = 0; i < rows.size(); ++i)
+      for (size_t j = 0; j < cols.size(); ++j)
+        for (size_t k = 0; k < inner.size(); ++k)
+          gemm(piece(m, rows, cols, (int)i, (int)j, rp, cp), piece(a, rows, inner, (int)i, (int)k, rp, kp),
+               piece(b, inner, cols, (int)k, (int)j, kp, cp));
+  }
+
+  // ---- solve: b <- L^{-1} b (unit lower), then b <- U^{-1} b
+  void forward(const operand& l, const mview& b) {
+    if (l.type == element_t::dense) {
+      emit_trsml(tree, b, l.d.v);
+      return;
+    }
+    const h_block* h = l.h;
+    const std::vector<int> rp = prefix(h->row_sizes);
+    for (int i = 0; i < h->py; ++i) {
+      const mview bi = b.sub(rp[i], 0, h->row_sizes[i], b.cols());
+      forward(child(l, i, i), bi);
+      for (int k = i + 1; k < h->py; ++k)
+        gemm(of_dense(b.sub(rp[k], 0, h->row_sizes[k], b.cols())), child(l, k, i), of_dense(bi));
+    }
+  }
+
+  void backward(const operand& u, const mview& b) {
+    if (u.type == element_t::dense) {
+      require(!b.t, "trsmu: transposed rhs");
+      emit_trsmu(tree, b.v, u.d.v);
+      return;
+    }
+    const h_block* h = u.h;
+    const std::vector<int> rp = prefix(h->row_sizes);
+    for (int i = h->py - 1; i >= 0; --i) {
+      const mview bi = b.sub(rp[i], 0, h->row_sizes[i], b.cols());
+      backward(child(u, i, i), bi);
+      for (int k = i - 1; k >= 0; --k)
+        gemm(of_dense(b.sub(rp[k], 0, h->row_sizes[k], b.cols())), child(u, k, i), of_dense(bi));
+    }
+  }
+};
+
+}  // namespace
+
+h_ops_tree* Hierarchical::generateOps_GETRF(const h_index* self, dev_temp* tmp, int slot) const {
+  if (self == nullptr || tmp == nullptr) throw std::invalid_argument("generateOps_GETRF: null argument");
+  const int ms = slot >= 0 ? slot : ptr_id;
+  h_ops_tree* tree = new h_ops_tree();
+  try {
+    op_builder bld(tree, tmp, ms);
+    bld.getrf(of_block(self->node, ms));
+  } catch (...) {
+    delete tree;
+    throw;
+  }
+  return tree;
+}
+
+h_ops_tree* Hierarchical::generateOps_Solve(const h_index* self, int rhs_ptr_id, int64_t rhs_off, int nrhs, int rhs_ld,
+                                            int rhs_split, int slot) const {
+  const int ms = slot >= 0 ? slot : ptr_id;
+  if (self == nullptr) throw std::invalid_argument("generateOps_Solve: null index");
+  if (nrhs <= 0 || rhs_ld < nrhs || rhs_split <= 0) throw std::invalid_argument("generateOps_Solve: bad rhs shape");
+  h_ops_tree* tree = new h_ops_tree();
+  try {
+    dev_temp unused(-1);
+    op_builder bld(tree, &unused, ms);
+    const operand root = of_block(self->node, ms);
+    const int groups = std::min(rhs_split, nrhs);
+    int c0 = 0;
+    for (int g = 0; g < groups; ++g) {
+      const int w = nrhs / groups + (g < nrhs % groups ? 1 : 0);
+      const mview b{dview{rhs_ptr_id, rhs_off + c0, self->node->ny, w, rhs_ld}, false};
+      bld.forward(root, b);
+      bld.backward(root, b);
+      c0 += w;
+    }
+  } catch (...) {
+    delete tree;
+    throw;
+  }
+  return tree;
+}

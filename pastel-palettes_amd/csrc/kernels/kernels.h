@@ -1,0 +1,49 @@
+// Host-side declarations of the device kernels' launch wrappers.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "vm.h"
+
+namespace pp {
+
+// One leaf of an H-matrix for the matrix-vector product.
+struct MatvecBlock {
+  int32_t type;  // 1 dense, 2 low rank
+  int32_t row0, col0, ny, nx, rank;
+  int64_t off, off_u, off_v;  // element offsets in the FP64 matrix arena
+  int64_t t_off;              // low rank: offset of its rank x nrhs V^T x block
+};
+
+// A row range of the product and the leaves covering it.
+struct MatvecTile {
+  int32_t row0, rows, first, count;  // contributions [first, first + count)
+};
+
+struct MatvecPlanDev {
+  const MatvecBlock* blocks;
+  const MatvecTile* tiles;
+  const int* contrib;  // pairs (block index, unused)
+  const int* lr_list;  // indices of low-rank blocks
+  int n_tiles, n_lowrank;
+};
+
+hipError_t launch_vm(const VmArgs& a, int precision, int workers, int threads, uint32_t* epoch_dev, int32_t* status,
+                     hipStream_t s);
+int vm_blocks_per_cu(int precision, int threads);
+
+hipError_t launch_fill_zero(void* p, int64_t n, int elem_bytes, hipStream_t s);
+hipError_t launch_convert(void* dst, int dst_bytes, const void* src, int src_bytes, int64_t n, hipStream_t s);
+hipError_t launch_copy2d(void* dst, int dst_bytes, int ldd, const void* src, int src_bytes, int lds_, int n, int nrhs,
+                         hipStream_t s);
+hipError_t launch_add_correction(double* x, int ldx, const void* d, int d_bytes, int ldd, int n, int nrhs,
+                                 hipStream_t s);
+hipError_t launch_hmatvec(const MatvecPlanDev& p, const double* A, const double* x, int ldx, const double* y, int ldy,
+                          double* out, int ldo, int nrhs, double* t, hipStream_t s);
+hipError_t launch_colnorm2(const double* r, int ldr, int n, int nrhs, double* out, hipStream_t s);
+hipError_t launch_mfma_probe(int mode, const double* a, const double* b, double* out, hipStream_t s);
+hipError_t launch_single_op(int precision, const int32_t* words, void* const* ptrs, void* scratch, const void* rnd,
+                            int32_t* status, hipStream_t s);
+
+}  // namespace pp

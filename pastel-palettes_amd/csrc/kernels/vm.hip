@@ -1,0 +1,495 @@
+// Persistent VM kernel, its launch wrappers and the small device kernels of
+// the solve / refinement path (fills, conversions, H-matrix-vector product,
+// norms).
+#include <hip/hip_runtime.h>
+
+#include "../h_ops/h_ops.h"
+#include "dev_lu.h"
+#include "kernels.h"
+#include "vm.h"
+
+namespace pp {
+
+__device__ __forceinline__ int32_t word(const int32_t* pc, int i) { return __builtin_amdgcn_readfirstlane(pc[i]); }
+
+template <typename T>
+__device__ __forceinline__ T* slot(void* const* ptrs, const int32_t* pc, int i) {
+  return reinterpret_cast<T*>(ptrs[word(pc, i)]);
+}
+
+__device__ void record_error(int32_t* status, int32_t code, int32_t worker, int32_t inst) {
+  if (atomicCAS(&status[0], VM_OK, code) == VM_OK) {
+    status[1] = worker;
+    status[2] = inst;
+  }
+  __hip_atomic_store(&status[3], 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Executes one execute entry; returns its length in words or -1.
+template <typename T, int FMODE, int GMODE>
+__device__ int exec_entry(const int32_t* pc, const VmArgs& a, T* tmp, void* lds) {
+  void* const* P = a.ptrs;
+  switch (word(pc, 2)) {
+    case nop:
+      return nop_l;
+    case getrf: {
+      T* M = slot<T>(P, pc, 3) + word(pc, 4);
+      getrf_block<T, FMODE>(M, word(pc, 5), word(pc, 6), word(pc, 7), lds);
+      return getrf_l;
+    }
+    case trsml: {
+      T* B = slot<T>(P, pc, 3) + word(pc, 5);
+      const T* L = slot<T>(P, pc, 4) + word(pc, 6);
+      trsml_block<T, FMODE>(B, L, word(pc, 7), word(pc, 8), word(pc, 10), word(pc, 11), word(pc, 12) != 0, lds);
+      return trsml_l;
+    }
+    case trsmr: {
+      T* B = slot<T>(P, pc, 3) + word(pc, 5);
+      const T* U = slot<T>(P, pc, 4) + word(pc, 6);
+      trsmr_block<T, FMODE>(B, U, word(pc, 7), word(pc, 8), word(pc, 10), word(pc, 11), word(pc, 12) != 0, lds);
+      return trsmr_l;
+    }
+    case trsmu: {
+      T* B = slot<T>(P, pc, 3) + word(pc, 5);
+      const T* U = slot<T>(P, pc, 4) + word(pc, 6);
+      trsmu_block<T, FMODE>(B, U, word(pc, 7), word(pc, 8), word(pc, 10), word(pc, 11), lds);
+      return trsmu_l;
+    }
+    case gemm:
+    case gemm_plus: {
+      T* M = slot<T>(P, pc, 3) + word(pc, 6);
+      const T* A = slot<T>(P, pc, 4) + word(pc, 7);
+      const T* B = slot<T>(P, pc, 5) + word(pc, 8);
+      const T alpha = word(pc, 2) == gemm ? T(-1) : T(1);
+      gemm_block<T, GMODE>(word(pc, 9), word(pc, 10), word(pc, 11), alpha, T(1), M, word(pc, 12), A, word(pc, 13),
+                           word(pc, 15) != 0, B, word(pc, 14), word(pc, 16) != 0, lds);
+      return gemm_l;
+    }
+    case gemm_3x: {
+      T* M = slot<T>(P, pc, 3) + word(pc, 7);
+      const T* A = slot<T>(P, pc, 4) + word(pc, 8);
+      const T* B = slot<T>(P, pc, 5) + word(pc, 9);
+      const T* C = slot<T>(P, pc, 6) + word(pc, 10);
+      const int m = word(pc, 11), n = word(pc, 12), k = word(pc, 13), l = word(pc, 14);
+      const int ldm = word(pc, 15), lda = word(pc, 16), ldb = word(pc, 17), ldc = word(pc, 18);
+      const bool aT = word(pc, 19) != 0, bT = word(pc, 20) != 0, cT = word(pc, 21) != 0;
+      if (word(pc, 22)) {  // (A B) C
+        gemm_block<T, GMODE>(m, l, k, T(1), T(0), tmp, l, A, lda, aT, B, ldb, bT, lds);
+        gemm_block<T, GMODE>(m, n, l, T(-1), T(1), M, ldm, tmp, l, false, C, ldc, cT, lds);
+      } else {  // A (B C)
+        gemm_block<T, GMODE>(k, n, l, T(1), T(0), tmp, n, B, ldb, bT, C, ldc, cT, lds);
+        gemm_block<T, GMODE>(m, n, k, T(-1), T(1), M, ldm, A, lda, aT, tmp, n, false, lds);
+      }
+      return gemm_3x_l;
+    }
+    case gemm_4x: {
+      T* M = slot<T>(P, pc, 3) + word(pc, 8);
+      const T* A = slot<T>(P, pc, 4) + word(pc, 9);
+      const T* B = slot<T>(P, pc, 5) + word(pc, 10);
+      const T* C = slot<T>(P, pc, 6) + word(pc, 11);
+      const T* D = slot<T>(P, pc, 7) + word(pc, 12);
+      const int m = word(pc, 13), n = word(pc, 14), k = word(pc, 15), l = word(pc, 16), o = word(pc, 17);
+      const int ldm = word(pc, 18), lda = word(pc, 19), ldb = word(pc, 20), ldc = word(pc, 21), ldd = word(pc, 22);
+      const bool aT = word(pc, 23) != 0, bT = word(pc, 24) != 0, cT = word(pc, 25) != 0, dT = word(pc, 26) != 0;
+      T* t1 = tmp;
+      T* t2 = tmp + word(pc, 28);
+      switch (word(pc, 27)) {
+        case 0:  // ((A B) C) D
+          gemm_block<T, GMODE>(m, l, k, T(1), T(0), t1, l, A, lda, aT, B, ldb, bT, lds);
+          gemm_block<T, GMODE>(m, o, l, T(1), T(0), t2, o, t1, l, false, C, ldc, cT, lds);
+          gemm_block<T, GMODE>(m, n, o, T(-1), T(1), M, ldm, t2, o, false, D, ldd, dT, lds);
+          break;
+        case 1:  // (A (B C)) D
+          gemm_block<T, GMODE>(k, o, l, T(1), T(0), t1, o, B, ldb, bT, C, ldc, cT, lds);
+          gemm_block<T, GMODE>(m, o, k, T(1), T(0), t2, o, A, lda, aT, t1, o, false, lds);
+          gemm_block<T, GMODE>(m, n, o, T(-1), T(1), M, ldm, t2, o, false, D, ldd, dT, lds);
+          break;
+        case 2:  // A ((B C) D)
+          gemm_block<T, GMODE>(k, o, l, T(1), T(0), t1, o, B, ldb, bT, C, ldc, cT, lds);
+          gemm_block<T, GMODE>(k, n, o, T(1), T(0), t2, n, t1, o, false, D, ldd, dT, lds);
+          gemm_block<T, GMODE>(m, n, k, T(-1), T(1), M, ldm, A, lda, aT, t2, n, false, lds);
+          break;
+        case 3:  // A (B (C D))
+          gemm_block<T, GMODE>(l, n, o, T(1), T(0), t1, n, C, ldc, cT, D, ldd, dT, lds);
+          gemm_block<T, GMODE>(k, n, l, T(1), T(0), t2, n, B, ldb, bT, t1, n, false, lds);
+          gemm_block<T, GMODE>(m, n, k, T(-1), T(1), M, ldm, A, lda, aT, t2, n, false, lds);
+          break;
+        case 4:  // (A B) (C D)
+          gemm_block<T, GMODE>(m, l, k, T(1), T(0), t1, l, A, lda, aT, B, ldb, bT, lds);
+          gemm_block<T, GMODE>(l, n, o, T(1), T(0), t2, n, C, ldc, cT, D, ldd, dT, lds);
+          gemm_block<T, GMODE>(m, n, l, T(-1), T(1), M, ldm, t1, l, false, t2, n, false, lds);
+          break;
+        default:
+          return -1;
+      }
+      return gemm_4x_l;
+    }
+    case accum: {
+      T* U1 = slot<T>(P, pc, 3) + word(pc, 7);
+      T* V1 = slot<T>(P, pc, 4) + word(pc, 8);
+      const T* U2 = slot<T>(P, pc, 5) + word(pc, 9);
+      const T* V2 = slot<T>(P, pc, 6) + word(pc, 10);
+      accum_block<T, GMODE>(U1, V1, U2, V2, word(pc, 11), word(pc, 12), word(pc, 13), word(pc, 14), word(pc, 15),
+                            word(pc, 16), word(pc, 17), word(pc, 18), word(pc, 19), word(pc, 20), tmp,
+                            reinterpret_cast<const T*>(a.rnd), lds);
+      return accum_l;
+    }
+    default:
+      return -1;
+  }
+}
+
+template <typename T, int FMODE, int GMODE, int NT>
+__global__ __launch_bounds__(NT) void vm_kernel(VmArgs a) {
+  __shared__ __attribute__((aligned(16))) char lds[kLdsBytes];
+  __shared__ int32_t ctl[2];
+  const int w = blockIdx.x;
+  const int32_t* pc = a.words + a.stream_off[w];
+  T* tmp = reinterpret_cast<T*>(a.scratch) + (int64_t)w * a.scratch_stride;
+  uint64_t* trace = a.trace ? a.trace + a.trace_off[w] : nullptr;
+  const uint32_t epoch = __builtin_amdgcn_readfirstlane(*a.epoch);
+  int entry = 0;
+  for (;; ++entry) {
+    const int32_t op = word(pc, 0);
+    if (trace != nullptr && threadIdx.x == 0) trace[entry] = wall_clock64();
+    if (op == finish) break;
+    if (op == signal_wait) {
+      const int32_t id = word(pc, 1);
+      const int hint = word(pc, 2);
+      if (threadIdx.x == 0) {
+        int ok = 1;
+        const uint64_t t0 = wall_clock64();
+        int64_t polls = 0;
+        while ((int32_t)(__hip_atomic_load(&a.flags[id], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) - epoch) < 0) {
+          if ((++polls & 31) == 0) {
+            if (__hip_atomic_load(&a.status[3], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) {
+              ok = 0;
+              break;
+            }
+            if ((int64_t)(wall_clock64() - t0) > a.spin_limit) {
+              record_error(a.status, VM_TIMEOUT, w, id);
+              ok = 0;
+              break;
+            }
+          }
+          if (hint > 8) __builtin_amdgcn_s_sleep(8);
+          else __builtin_amdgcn_s_sleep(1);
+        }
+        ctl[entry & 1] = ok;
+      }
+      __syncthreads();
+      if (!ctl[entry & 1]) break;
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+      pc += wait_l;
+      continue;
+    }
+    if (op != execute) {
+      if (threadIdx.x == 0) record_error(a.status, VM_BAD_OPCODE, w, op);
+      break;
+    }
+    const int32_t id = word(pc, 1);
+    const int len = exec_entry<T, FMODE, GMODE>(pc, a, tmp, lds);
+    if (len < 0) {
+      if (threadIdx.x == 0) record_error(a.status, VM_BAD_OPERATION, w, id);
+      break;
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    __syncthreads();
+    if (threadIdx.x == 0) __hip_atomic_store(&a.flags[id], epoch, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    pc += len;
+  }
+}
+
+// Advances the launch epoch and clears the status words (a kernel rather
+// than a memset so that the sequence can be captured in a hipGraph).
+__global__ void bump_epoch_kernel(uint32_t* e, int32_t* status) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *e = *e + 1u;
+  if (threadIdx.x < 8) status[threadIdx.x] = 0;
+}
+
+// ---------------------------------------------------------------- launch wrappers
+
+template <typename T, int FMODE, int GMODE, int NT>
+static hipError_t launch_vm_t(const VmArgs& a, int workers, hipStream_t s) {
+  hipLaunchKernelGGL((vm_kernel<T, FMODE, GMODE, NT>), dim3(workers), dim3(NT), 0, s, a);
+  return hipGetLastError();
+}
+
+template <typename T, int FMODE, int GMODE>
+static hipError_t launch_vm_nt(const VmArgs& a, int workers, int nt, hipStream_t s) {
+  switch (nt) {
+    case 256: return launch_vm_t<T, FMODE, GMODE, 256>(a, workers, s);
+    case 512: return launch_vm_t<T, FMODE, GMODE, 512>(a, workers, s);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+hipError_t launch_vm(const VmArgs& a, int precision, int workers, int threads, uint32_t* epoch_dev, int32_t* status,
+                     hipStream_t s) {
+  hipLaunchKernelGGL(bump_epoch_kernel, dim3(1), dim3(64), 0, s, epoch_dev, status);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  switch (precision) {
+    case 0: return launch_vm_nt<double, MODE_FP64, MODE_FP64>(a, workers, threads, s);
+    case 1: return launch_vm_nt<float, MODE_FP32, MODE_FP32>(a, workers, threads, s);
+    case 2: return launch_vm_nt<float, MODE_FP32, MODE_BF16>(a, workers, threads, s);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+template <typename T, int FMODE, int GMODE, int NT>
+static int max_blocks_t() {
+  int nb = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, vm_kernel<T, FMODE, GMODE, NT>, NT, 0) != hipSuccess) return 0;
+  return nb;
+}
+
+int vm_blocks_per_cu(int precision, int threads) {
+  if (threads == 256) {
+    if (precision == 0) return max_blocks_t<double, MODE_FP64, MODE_FP64, 256>();
+    if (precision == 1) return max_blocks_t<float, MODE_FP32, MODE_FP32, 256>();
+    if (precision == 2) return max_blocks_t<float, MODE_FP32, MODE_BF16, 256>();
+  } else if (threads == 512) {
+    if (precision == 0) return max_blocks_t<double, MODE_FP64, MODE_FP64, 512>();
+    if (precision == 1) return max_blocks_t<float, MODE_FP32, MODE_FP32, 512>();
+    if (precision == 2) return max_blocks_t<float, MODE_FP32, MODE_BF16, 512>();
+  }
+  return 0;
+}
+
+// ---------------------------------------------------------------- small kernels
+
+template <typename T>
+__global__ void fill_zero_kernel(T* p, int64_t n) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = T(0);
+}
+
+template <typename D, typename S>
+__global__ void convert_kernel(D* dst, const S* src, int64_t n) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    dst[i] = (D)src[i];
+}
+
+// x[i, c] += d[i, c] for an n x nrhs block with pitches
+template <typename S>
+__global__ void add_correction_kernel(double* x, int ldx, const S* d, int ldd, int n, int nrhs) {
+  const int64_t total = (int64_t)n * nrhs;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t i = e / nrhs, c = e % nrhs;
+    x[i * ldx + c] += (double)d[i * ldd + c];
+  }
+}
+
+// copy an n x nrhs block between pitched buffers with conversion
+template <typename D, typename S>
+__global__ void copy2d_kernel(D* dst, int ldd, const S* src, int lds_, int n, int nrhs) {
+  const int64_t total = (int64_t)n * nrhs;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t i = e / nrhs, c = e % nrhs;
+    dst[i * ldd + c] = (D)src[i * lds_ + c];
+  }
+}
+
+// H-matrix times block: phase 1, t_b = V_b^T x[cols of b] for every low-rank
+// leaf b (one workgroup per leaf; fixed-order reduction).
+__global__ __launch_bounds__(256) void hmv_lowrank_kernel(const MatvecBlock* blocks, const int* lr_list,
+                                                           const double* A, const double* x, int ldx, int nrhs,
+                                                           double* t) {
+  __shared__ double red[256];
+  const MatvecBlock b = blocks[lr_list[blockIdx.x]];
+  const int r = b.rank;
+  for (int q = 0; q < r * nrhs; ++q) {
+    const int kk = q / nrhs, c = q % nrhs;
+    double s = 0.;
+    for (int j = threadIdx.x; j < b.nx; j += blockDim.x)
+      s += A[b.off_v + (int64_t)j * r + kk] * x[(int64_t)(b.col0 + j) * ldx + c];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int h = blockDim.x / 2; h > 0; h >>= 1) {
+      if ((int)threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) t[b.t_off + q] = red[0];
+    __syncthreads();
+  }
+}
+
+// phase 2: for every row tile, out = y - sum of contributions (dense D x and
+// low-rank U t) of the leaves covering the tile, in a fixed order.
+__global__ __launch_bounds__(256) void hmv_rows_kernel(const MatvecBlock* blocks, const MatvecTile* tiles,
+                                                        const int* contrib, const double* A, const double* x, int ldx,
+                                                        const double* t, const double* y, int ldy, double* out,
+                                                        int ldo, int nrhs) {
+  const MatvecTile tl = tiles[blockIdx.x];
+  for (int e = threadIdx.x; e < tl.rows * nrhs; e += blockDim.x) {
+    const int i = e / nrhs, c = e % nrhs;
+    const int row = tl.row0 + i;
+    double acc = 0.;
+    for (int q = tl.first; q < tl.first + tl.count; ++q) {
+      const MatvecBlock b = blocks[contrib[2 * q]];
+      const int li = row - b.row0;
+      if (b.type == 1) {
+        const double* Ar = A + b.off + (int64_t)li * b.nx;
+        double s = 0.;
+        for (int j = 0; j < b.nx; ++j) s += Ar[j] * x[(int64_t)(b.col0 + j) * ldx + c];
+        acc += s;
+      } else {
+        const double* Ur = A + b.off_u + (int64_t)li * b.rank;
+        double s = 0.;
+        for (int kk = 0; kk < b.rank; ++kk) s += Ur[kk] * t[b.t_off + kk * nrhs + c];
+        acc += s;
+      }
+    }
+    out[(int64_t)row * ldo + c] = (y != nullptr ? y[(int64_t)row * ldy + c] : 0.) - acc;
+  }
+}
+
+// per-column sums of squares of an n x nrhs block (one workgroup per column)
+__global__ __launch_bounds__(256) void colnorm2_kernel(const double* r, int ldr, int n, double* out) {
+  __shared__ double red[256];
+  const int c = blockIdx.x;
+  double s = 0.;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) {
+    const double v = r[(int64_t)i * ldr + c];
+    s += v * v;
+  }
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int h = blockDim.x / 2; h > 0; h >>= 1) {
+    if ((int)threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[c] = red[0];
+}
+
+static int grid_for(int64_t n) {
+  int64_t g = (n + 255) / 256;
+  return (int)(g < 1 ? 1 : (g > 4096 ? 4096 : g));
+}
+
+hipError_t launch_fill_zero(void* p, int64_t n, int elem_bytes, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  if (elem_bytes == 8)
+    hipLaunchKernelGGL(fill_zero_kernel<double>, dim3(grid_for(n)), dim3(256), 0, s, (double*)p, n);
+  else
+    hipLaunchKernelGGL(fill_zero_kernel<float>, dim3(grid_for(n)), dim3(256), 0, s, (float*)p, n);
+  return hipGetLastError();
+}
+
+hipError_t launch_convert(void* dst, int dst_bytes, const void* src, int src_bytes, int64_t n, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  if (dst_bytes == 4 && src_bytes == 8)
+    hipLaunchKernelGGL((convert_kernel<float, double>), dim3(grid_for(n)), dim3(256), 0, s, (float*)dst,
+                       (const double*)src, n);
+  else if (dst_bytes == 8 && src_bytes == 4)
+    hipLaunchKernelGGL((convert_kernel<double, float>), dim3(grid_for(n)), dim3(256), 0, s, (double*)dst,
+                       (const float*)src, n);
+  else if (dst_bytes == 8 && src_bytes == 8)
+    return hipMemcpyAsync(dst, src, n * 8, hipMemcpyDeviceToDevice, s);
+  else
+    return hipMemcpyAsync(dst, src, n * 4, hipMemcpyDeviceToDevice, s);
+  return hipGetLastError();
+}
+
+hipError_t launch_copy2d(void* dst, int dst_bytes, int ldd, const void* src, int src_bytes, int lds_, int n, int nrhs,
+                         hipStream_t s) {
+  const int g = grid_for((int64_t)n * nrhs);
+  if (dst_bytes == 8 && src_bytes == 8)
+    hipLaunchKernelGGL((copy2d_kernel<double, double>), dim3(g), dim3(256), 0, s, (double*)dst, ldd,
+                       (const double*)src, lds_, n, nrhs);
+  else if (dst_bytes == 4 && src_bytes == 8)
+    hipLaunchKernelGGL((copy2d_kernel<float, double>), dim3(g), dim3(256), 0, s, (float*)dst, ldd,
+                       (const double*)src, lds_, n, nrhs);
+  else if (dst_bytes == 8 && src_bytes == 4)
+    hipLaunchKernelGGL((copy2d_kernel<double, float>), dim3(g), dim3(256), 0, s, (double*)dst, ldd,
+                       (const float*)src, lds_, n, nrhs);
+  else
+    hipLaunchKernelGGL((copy2d_kernel<float, float>), dim3(g), dim3(256), 0, s, (float*)dst, ldd, (const float*)src,
+                       lds_, n, nrhs);
+  return hipGetLastError();
+}
+
+hipError_t launch_add_correction(double* x, int ldx, const void* d, int d_bytes, int ldd, int n, int nrhs,
+                                 hipStream_t s) {
+  const int g = grid_for((int64_t)n * nrhs);
+  if (d_bytes == 8)
+    hipLaunchKernelGGL(add_correction_kernel<double>, dim3(g), dim3(256), 0, s, x, ldx, (const double*)d, ldd, n,
+                       nrhs);
+  else
+    hipLaunchKernelGGL(add_correction_kernel<float>, dim3(g), dim3(256), 0, s, x, ldx, (const float*)d, ldd, n, nrhs);
+  return hipGetLastError();
+}
+
+hipError_t launch_hmatvec(const MatvecPlanDev& p, const double* A, const double* x, int ldx, const double* y, int ldy,
+                          double* out, int ldo, int nrhs, double* t, hipStream_t s) {
+  if (p.n_lowrank > 0)
+    hipLaunchKernelGGL(hmv_lowrank_kernel, dim3(p.n_lowrank), dim3(256), 0, s, p.blocks, p.lr_list, A, x, ldx, nrhs,
+                       t);
+  hipLaunchKernelGGL(hmv_rows_kernel, dim3(p.n_tiles), dim3(256), 0, s, p.blocks, p.tiles, p.contrib, A, x, ldx, t, y,
+                     ldy, out, ldo, nrhs);
+  return hipGetLastError();
+}
+
+hipError_t launch_colnorm2(const double* r, int ldr, int n, int nrhs, double* out, hipStream_t s) {
+  hipLaunchKernelGGL(colnorm2_kernel, dim3(nrhs), dim3(256), 0, s, r, ldr, n, out);
+  return hipGetLastError();
+}
+
+// MFMA lane-layout probe: D = A B for one 16x16xK tile with A[i][k] = a[i*K+k],
+// B[k][j] = b[k*16+j]; lane l writes its 4 accumulator values to out[4l..].
+__global__ void mfma_probe_kernel(int mode, const double* a, const double* b, double* out) {
+  const int l = threadIdx.x;
+  if (mode == 0) {
+    d4_t c = {0., 0., 0., 0.};
+    c = __builtin_amdgcn_mfma_f64_16x16x4f64(a[(l & 15) * 4 + (l >> 4)], b[(l >> 4) * 16 + (l & 15)], c, 0, 0, 0);
+    for (int r = 0; r < 4; ++r) out[4 * l + r] = c[r];
+  } else if (mode == 1) {
+    f4_t c = {0.f, 0.f, 0.f, 0.f};
+    c = __builtin_amdgcn_mfma_f32_16x16x4f32((float)a[(l & 15) * 4 + (l >> 4)], (float)b[(l >> 4) * 16 + (l & 15)], c,
+                                            0, 0, 0);
+    for (int r = 0; r < 4; ++r) out[4 * l + r] = c[r];
+  } else {
+    f4_t c = {0.f, 0.f, 0.f, 0.f};
+    float av[8], bv[8];
+    for (int e = 0; e < 8; ++e) {
+      av[e] = (float)a[(l & 15) * 32 + (l >> 4) * 8 + e];
+      bv[e] = (float)b[((l >> 4) * 8 + e) * 16 + (l & 15)];
+    }
+    c = Mma<float, MODE_BF16>::mma(av, bv, c);
+    for (int r = 0; r < 4; ++r) out[4 * l + r] = c[r];
+  }
+}
+
+hipError_t launch_mfma_probe(int mode, const double* a, const double* b, double* out, hipStream_t s) {
+  hipLaunchKernelGGL(mfma_probe_kernel, dim3(1), dim3(64), 0, s, mode, a, b, out);
+  return hipGetLastError();
+}
+
+// Single-op test harness: run one execute entry on one workgroup.
+template <typename T, int FMODE, int GMODE>
+__global__ __launch_bounds__(256) void single_op_kernel(const int32_t* words, void* const* ptrs, void* scratch,
+                                                        const void* rnd, int32_t* status) {
+  __shared__ __attribute__((aligned(16))) char lds[kLdsBytes];
+  VmArgs a{};
+  a.ptrs = ptrs;
+  a.rnd = rnd;
+  const int len = exec_entry<T, FMODE, GMODE>(words, a, reinterpret_cast<T*>(scratch), lds);
+  if (threadIdx.x == 0) status[0] = len;
+}
+
+hipError_t launch_single_op(int precision, const int32_t* words, void* const* ptrs, void* scratch, const void* rnd,
+                            int32_t* status, hipStream_t s) {
+  if (precision == 0)
+    hipLaunchKernelGGL((single_op_kernel<double, MODE_FP64, MODE_FP64>), dim3(1), dim3(256), 0, s, words, ptrs,
+                       scratch, rnd, status);
+  else if (precision == 1)
+    hipLaunchKernelGGL((single_op_kernel<float, MODE_FP32, MODE_FP32>), dim3(1), dim3(256), 0, s, words, ptrs,
+                       scratch, rnd, status);
+  else
+    hipLaunchKernelGGL((single_op_kernel<float, MODE_FP32, MODE_BF16>), dim3(1), dim3(256), 0, s, words, ptrs,
+                       scratch, rnd, status);
+  return hipGetLastError();
+}
+
+}  // namespace pp

@@ -1,0 +1,922 @@
+// C ABI, device half: context, uploaded streams, VM launches, and the
+// factor + solve + FP64 refinement driver.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+#include "../../../include/pastel.h"
+#include "../capi_common.h"
+#include "../kernels/kernels.h"
+#include "../matrix/Hierarchical.h"
+
+using ppc::from_exception;
+using ppc::set_error;
+
+namespace {
+
+int hip_fail(hipError_t e, const char* where) {
+  return set_error(PP_ERR_HIP, std::string(where) + ": " + hipGetErrorString(e));
+}
+
+#define HIP_TRY(expr)                                 \
+  do {                                                \
+    hipError_t _e = (expr);                           \
+    if (_e != hipSuccess) return hip_fail(_e, #expr); \
+  } while (0)
+
+// splitmix64 stream -> uniform [-1, 1); the sketch matrix of accum.
+uint64_t splitmix64(uint64_t& state) {
+  uint64_t z = (state += 0x9E3779B97F4A7C15ULL);
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+  return z ^ (z >> 31);
+}
+
+void sketch_values(std::vector<double>& v, int64_t count, uint64_t seed) {
+  v.resize(count);
+  uint64_t st = seed;
+  for (int64_t i = 0; i < count; ++i) v[i] = 2.0 * ((double)(splitmix64(st) >> 11) * 0x1.0p-53) - 1.0;
+}
+
+constexpr uint64_t kSketchSeed = 0x5EEDULL;
+
+int elem_bytes(int precision) { return precision == PP_FP64 ? 8 : 4; }
+
+}  // namespace
+
+struct pp_ctx {
+  int device = 0;
+  uint64_t budget = 0;
+  uint64_t in_use = 0;
+  int cus = 0;
+  hipStream_t stream = nullptr;
+  std::unordered_map<void*, uint64_t> allocs;
+  std::mutex mu;
+};
+
+struct pp_stream {
+  pp_ctx* ctx = nullptr;
+  int workers = 0, n_insts = 0, precision = 0, n_ptrs = 0;
+  int32_t* d_words = nullptr;
+  int64_t* d_off = nullptr;
+  void** d_ptrs = nullptr;
+  uint32_t* d_flags = nullptr;
+  uint32_t* d_epoch = nullptr;
+  int32_t* d_status = nullptr;
+  void* d_scratch = nullptr;
+  int64_t scratch_stride = 0;
+  void* d_rnd = nullptr;
+  uint64_t* d_trace = nullptr;
+  int64_t* d_trace_off = nullptr;
+  int64_t trace_total = 0;
+  bool trace_on = false;
+  std::vector<void*> ptrs;
+  std::vector<std::pair<int, int64_t>> zero_slots;  // slot, elements
+  std::vector<void*> owned;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  int64_t spin_limit = 20LL * 100000000LL;  // 20 s of 100 MHz ticks
+  hipStream_t last = nullptr;
+  bool launched = false;
+};
+
+namespace {
+
+int ctx_alloc(pp_ctx* c, uint64_t bytes, void** p) {
+  std::lock_guard<std::mutex> lk(c->mu);
+  if (c->budget != 0 && c->in_use + bytes > c->budget)
+    return set_error(PP_ERR_NOMEM, "HBM budget exceeded: need " + std::to_string(bytes) + " bytes, " +
+                                       std::to_string(c->budget - c->in_use) + " left");
+  void* d = nullptr;
+  hipError_t e = hipMalloc(&d, bytes > 0 ? bytes : 256);
+  if (e != hipSuccess) return set_error(PP_ERR_NOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
+  c->allocs[d] = bytes;
+  c->in_use += bytes;
+  *p = d;
+  return PP_OK;
+}
+
+int ctx_free(pp_ctx* c, void* p) {
+  if (p == nullptr) return PP_OK;
+  std::lock_guard<std::mutex> lk(c->mu);
+  auto it = c->allocs.find(p);
+  if (it == c->allocs.end()) return set_error(PP_ERR_INVALID, "pp_free: pointer not allocated by this context");
+  c->in_use -= it->second;
+  c->allocs.erase(it);
+  hipError_t e = hipFree(p);
+  if (e != hipSuccess) return hip_fail(e, "hipFree");
+  return PP_OK;
+}
+
+template <typename T>
+int ctx_alloc_t(pp_ctx* c, int64_t count, T** p) {
+  void* v = nullptr;
+  int rc = ctx_alloc(c, (uint64_t)std::max<int64_t>(count, 1) * sizeof(T), &v);
+  *p = static_cast<T*>(v);
+  return rc;
+}
+
+hipStream_t pick(pp_ctx* c, void* s) { return s ? static_cast<hipStream_t>(s) : c->stream; }
+
+int upload_sketch(pp_stream* s, int64_t count, uint64_t seed) {
+  std::vector<double> v;
+  sketch_values(v, std::max<int64_t>(count, 1), seed);
+  int rc = ctx_alloc(s->ctx, v.size() * elem_bytes(s->precision), &s->d_rnd);
+  if (rc) return rc;
+  s->owned.push_back(s->d_rnd);
+  if (s->precision == PP_FP64) {
+    HIP_TRY(hipMemcpy(s->d_rnd, v.data(), v.size() * 8, hipMemcpyHostToDevice));
+  } else {
+    std::vector<float> f(v.begin(), v.end());
+    HIP_TRY(hipMemcpy(s->d_rnd, f.data(), f.size() * 4, hipMemcpyHostToDevice));
+  }
+  return PP_OK;
+}
+
+// Sketch size a program needs: max over accum ops of nx * k1.
+int64_t sketch_need(const pp_program* p) {
+  int64_t need = 1;
+  for (int i = 0; i < p->tree.length(); ++i) {
+    const h_op& o = p->tree.getOp(i);
+    if (o.op == accum) need = std::max<int64_t>(need, (int64_t)o.words[11] * o.words[13]);
+  }
+  return need;
+}
+
+int stream_common(pp_ctx* ctx, pp_stream* s, const std::vector<const std::vector<int32_t>*>& streams, int n_insts,
+                  uint64_t scratch_bytes_per_worker, int64_t sketch_count) {
+  const int W = s->workers;
+  std::vector<int64_t> off(W + 1, 0), toff(W + 1, 0);
+  for (int w = 0; w < W; ++w) {
+    off[w + 1] = off[w] + (int64_t)streams[w]->size();
+    int64_t entries = 0;
+    for (size_t i = 0; i < streams[w]->size();) {
+      const int op = (*streams[w])[i];
+      ++entries;
+      if (op == finish) break;
+      if (op == signal_wait) i += wait_l;
+      else if (op == execute) {
+        const int len = operationLength((*streams[w])[i + 2]);
+        if (len <= 0) return set_error(PP_ERR_INVALID, "stream holds an unknown operation");
+        i += len;
+      } else {
+        return set_error(PP_ERR_INVALID, "stream holds an unknown opcode");
+      }
+    }
+    toff[w + 1] = toff[w] + entries + 1;
+  }
+  std::vector<int32_t> all;
+  all.reserve(off[W]);
+  for (int w = 0; w < W; ++w) all.insert(all.end(), streams[w]->begin(), streams[w]->end());
+  int rc;
+  if ((rc = ctx_alloc_t(ctx, off[W], &s->d_words))) return rc;
+  s->owned.push_back(s->d_words);
+  if ((rc = ctx_alloc_t(ctx, W, &s->d_off))) return rc;
+  s->owned.push_back(s->d_off);
+  if ((rc = ctx_alloc_t(ctx, std::max(n_insts, 1), &s->d_flags))) return rc;
+  s->owned.push_back(s->d_flags);
+  if ((rc = ctx_alloc_t(ctx, 16, &s->d_epoch))) return rc;
+  s->owned.push_back(s->d_epoch);
+  if ((rc = ctx_alloc_t(ctx, 16, &s->d_status))) return rc;
+  s->owned.push_back(s->d_status);
+  if ((rc = ctx_alloc_t(ctx, toff[W], &s->d_trace))) return rc;
+  s->owned.push_back(s->d_trace);
+  if ((rc = ctx_alloc_t(ctx, W, &s->d_trace_off))) return rc;
+  s->owned.push_back(s->d_trace_off);
+  s->trace_total = toff[W];
+  const int eb = elem_bytes(s->precision);
+  s->scratch_stride = (int64_t)((scratch_bytes_per_worker + eb - 1) / eb + 63) / 64 * 64;
+  if ((rc = ctx_alloc(ctx, (uint64_t)std::max<int64_t>(s->scratch_stride, 64) * eb * W, &s->d_scratch))) return rc;
+  s->owned.push_back(s->d_scratch);
+  if ((rc = ctx_alloc_t(ctx, std::max(s->n_ptrs, 1), &s->d_ptrs))) return rc;
+  s->owned.push_back(s->d_ptrs);
+  HIP_TRY(hipMemcpy(s->d_words, all.data(), all.size() * 4, hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(s->d_off, off.data(), W * 8, hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(s->d_trace_off, toff.data(), W * 8, hipMemcpyHostToDevice));
+  HIP_TRY(hipMemset(s->d_flags, 0, sizeof(uint32_t) * std::max(n_insts, 1)));
+  HIP_TRY(hipMemset(s->d_epoch, 0, 64));
+  HIP_TRY(hipMemset(s->d_status, 0, 64));
+  HIP_TRY(hipEventCreate(&s->ev0));
+  HIP_TRY(hipEventCreate(&s->ev1));
+  return upload_sketch(s, sketch_count, kSketchSeed);
+}
+
+int push_ptrs(pp_stream* s) {
+  HIP_TRY(hipMemcpy(s->d_ptrs, s->ptrs.data(), sizeof(void*) * s->ptrs.size(), hipMemcpyHostToDevice));
+  return PP_OK;
+}
+
+void destroy_stream(pp_stream* s) {
+  if (s == nullptr) return;
+  if (s->ev0) (void)hipEventDestroy(s->ev0);
+  if (s->ev1) (void)hipEventDestroy(s->ev1);
+  for (void* p : s->owned) (void)ctx_free(s->ctx, p);
+  delete s;
+}
+
+}  // namespace
+
+extern "C" {
+
+int pp_ctx_create(int device, uint64_t budget_bytes, pp_ctx** out) {
+  try {
+    if (out == nullptr) return set_error(PP_ERR_INVALID, "null out");
+    int n = 0;
+    HIP_TRY(hipGetDeviceCount(&n));
+    if (device < 0 || device >= n) return set_error(PP_ERR_INVALID, "no such device");
+    HIP_TRY(hipSetDevice(device));
+    auto c = std::make_unique<pp_ctx>();
+    c->device = device;
+    c->budget = budget_bytes;
+    hipDeviceProp_t prop;
+    HIP_TRY(hipGetDeviceProperties(&prop, device));
+    c->cus = prop.multiProcessorCount;
+    HIP_TRY(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    *out = c.release();
+    return PP_OK;
+  } catch (...) {
+    return from_exception();
+  }
+}
+
+int pp_ctx_destroy(pp_ctx* ctx) {
+  if (ctx == nullptr) return PP_OK;
+  (void)hipStreamSynchronize(ctx->stream);
+  for (auto& a : ctx->allocs) (void)hipFree(a.first);
+  (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+  return PP_OK;
+}
+
+int pp_ctx_cu_count(const pp_ctx* ctx) { return ctx ? ctx->cus : PP_ERR_INVALID; }
+uint64_t pp_ctx_bytes_in_use(const pp_ctx* ctx) { return ctx ? ctx->in_use : 0; }
+
+int pp_alloc(pp_ctx* ctx, uint64_t bytes, void** dptr) {
+  if (ctx == nullptr || dptr == nullptr) return set_error(PP_ERR_INVALID, "null");
+  return ctx_alloc(ctx, bytes, dptr);
+}
+
+int pp_free(pp_ctx* ctx, void* dptr) {
+  if (ctx == nullptr) return set_error(PP_ERR_INVALID, "null ctx");
+  return ctx_free(ctx, dptr);
+}
+
+int pp_copy_h2d(pp_ctx* ctx, void* dst, const void* src, uint64_t bytes) {
+  if (ctx == nullptr) return set_error(PP_ERR_INVALID, "null ctx");
+  HIP_TRY(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+  return PP_OK;
+}
+
+int pp_copy_d2h(pp_ctx* ctx, void* dst, const void* src, uint64_t bytes) {
+  if (ctx == nullptr) return set_error(PP_ERR_INVALID, "null ctx");
+  HIP_TRY(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+  return PP_OK;
+}
+
+int pp_stream_upload(pp_ctx* ctx, const int32_t* const* words, const int32_t* lengths, int workers,
+                     const void* const* ptr_table, int n_ptrs, uint64_t scratch_bytes_per_worker, int n_insts,
+                     int precision, pp_stream** out) {
+  try {
+    if (ctx == nullptr || words == nullptr || lengths == nullptr || workers < 1 || out == nullptr || n_ptrs < 0 ||
+        precision < 0 || precision > 2)
+      return set_error(PP_ERR_INVALID, "bad upload args");
+    std::vector<std::vector<int32_t>> ws(workers);
+    std::vector<const std::vector<int32_t>*> refs;
+    int64_t need = 1;
+    for (int w = 0; w < workers; ++w) {
+      ws[w].assign(words[w], words[w] + lengths[w]);
+      if (ws[w].empty() || ws[w].back() != finish) ws[w].push_back(finish);
+      for (size_t i = 0; i + 2 < ws[w].size();) {
+        if (ws[w][i] == execute && ws[w][i + 2] == accum && i + 13 < ws[w].size())
+          need = std::max<int64_t>(need, (int64_t)ws[w][i + 11] * ws[w][i + 13]);
+        if (ws[w][i] == signal_wait) i += wait_l;
+        else if (ws[w][i] == execute && operationLength(ws[w][i + 2]) > 0) i += operationLength(ws[w][i + 2]);
+        else break;
+      }
+      refs.push_back(&ws[w]);
+    }
+    auto s = new pp_stream();
+    s->ctx = ctx;
+    s->workers = workers;
+    s->n_insts = n_insts;
+    s->precision = precision;
+    s->n_ptrs = n_ptrs;
+    for (int i = 0; i < n_ptrs; ++i) s->ptrs.push_back(const_cast<void*>(ptr_table[i]));
+    int rc = stream_common(ctx, s, refs, n_insts, scratch_bytes_per_worker, need);
+    if (rc == PP_OK) rc = push_ptrs(s);
+    if (rc != PP_OK) {
+      destroy_stream(s);
+      return rc;
+    }
+    *out = s;
+    return PP_OK;
+  } catch (...) {
+    return from_exception();
+  }
+}
+
+int pp_stream_upload_program(pp_ctx* ctx, const pp_program* prog, const void* const* ptr_table, int n_ptrs,
+                             int precision, pp_stream** out) {
+  try {
+    if (ctx == nullptr || prog == nullptr || out == nullptr || precision < 0 || precision > 2)
+      return set_error(PP_ERR_INVALID, "bad upload args");
+    if (n_ptrs < prog->n_slots) return set_error(PP_ERR_INVALID, "pointer table shorter than the program's slots");
+    auto s = new pp_stream();
+    s->ctx = ctx;
+    s->workers = prog->workers;
+    s->n_insts = prog->dag->getLength();
+    s->precision = precision;
+    s->n_ptrs = n_ptrs;
+    for (int i = 0; i < n_ptrs; ++i) s->ptrs.push_back(const_cast<void*>(ptr_table[i]));
+    std::vector<const std::vector<int32_t>*> refs;
+    for (int w = 0; w < prog->workers; ++w) refs.push_back(&prog->mgr->getStream(w));
+    const int eb = elem_bytes(precision);
+    int rc = stream_common(ctx, s, refs, s->n_insts, (uint64_t)prog->scratch * eb, sketch_need(prog));
+    // temp slots the caller left empty are owned (and zeroed per launch) here
+    for (size_t i = 0; rc == PP_OK && i < prog->temp_elems.size(); ++i) {
+      const int slot = 3 * (int)i + 1;
+      const int64_t n = prog->temp_elems[i];
+      if (n <= 0) continue;
+      if (s->ptrs[slot] == nullptr) {
+        rc = ctx_alloc(ctx, (uint64_t)n * eb, &s->ptrs[slot]);
+        if (rc == PP_OK) s->owned.push_back(s->ptrs[slot]);
+      }
+      s->zero_slots.emplace_back(slot, n);
+    }
+    if (rc == PP_OK) rc = push_ptrs(s);
+    if (rc != PP_OK) {
+      destroy_stream(s);
+      return rc;
+    }
+    *out = s;
+    return PP_OK;
+  } catch (...) {
+    return from_exception();
+  }
+}
+
+int pp_stream_set_slot(pp_ctx* ctx, pp_stream* s, int slot, void* dptr) {
+  if (ctx == nullptr || s == nullptr || slot < 0 || slot >= s->n_ptrs) return set_error(PP_ERR_INVALID, "bad slot");
+  s->ptrs[slot] = dptr;
+  return push_ptrs(s);
+}
+
+int pp_stream_set_watchdog(pp_ctx* ctx, pp_stream* s, int ms) {
+  if (ctx == nullptr || s == nullptr || ms <= 0) return set_error(PP_ERR_INVALID, "bad watchdog");
+  s->spin_limit = (int64_t)ms * 100000LL;
+  return PP_OK;
+}
+
+int pp_stream_enable_trace(pp_ctx* ctx, pp_stream* s, int enable) {
+  if (ctx == nullptr || s == nullptr) return set_error(PP_ERR_INVALID, "null");
+  s->trace_on = enable != 0;
+  return PP_OK;
+}
+
+int pp_max_workers(pp_ctx* ctx, int precision, int wavefronts_per_inst) {
+  if (ctx == nullptr) return set_error(PP_ERR_INVALID, "null ctx");
+  const int per_cu = pp::vm_blocks_per_cu(precision, 64 * wavefronts_per_inst);
+  return per_cu * ctx->cus;
+}
+
+int pp_launch(pp_ctx* ctx, pp_stream* s, int wavefronts_per_inst, void* hip_stream) {
+  try {
+    if (ctx == nullptr || s == nullptr) return set_error(PP_ERR_INVALID, "null");
+    const int threads = 64 * wavefronts_per_inst;
+    if (threads != 256 && threads != 512) return set_error(PP_ERR_INVALID, "wavefronts_per_inst must be 4 or 8");
+    const int per_cu = pp::vm_blocks_per_cu(s->precision, threads);
+    if (per_cu < 1 || s->workers > per_cu * ctx->cus)
+      return set_error(PP_ERR_INVALID, "workers (" + std::to_string(s->workers) + ") exceed co-resident capacity " +
+                                           std::to_string(per_cu * ctx->cus));
+    hipStream_t st = pick(ctx, hip_stream);
+    for (const auto& z : s->zero_slots) {
+      hipError_t e = pp::launch_fill_zero(s->ptrs[z.first], z.second, elem_bytes(s->precision), st);
+      if (e != hipSuccess) return hip_fail(e, "fill_zero");
+    }
+    pp::VmArgs a{};
+    a.words = s->d_words;
+    a.stream_off = s->d_off;
+    a.ptrs = s->d_ptrs;
+    a.flags = s->d_flags;
+    a.scratch = s->d_scratch;
+    a.scratch_stride = s->scratch_stride;
+    a.rnd = s->d_rnd;
+    a.trace = s->trace_on ? s->d_trace : nullptr;
+    a.trace_off = s->d_trace_off;
+    a.status = s->d_status;
+    a.spin_limit = s->spin_limit;
+    a.epoch = s->d_epoch;
+    a.n_insts = s->n_insts;
+    HIP_TRY(hipEventRecord(s->ev0, st));
+    hipError_t e = pp::launch_vm(a, s->precision, s->workers, threads, s->d_epoch, s->d_status, st);
+    if (e != hipSuccess) return hip_fail(e, "launch_vm");
+    HIP_TRY(hipEventRecord(s->ev1, st));
+    s->last = st;
+    s->launched = true;
+    return PP_OK;
+  } catch (...) {
+    return from_exception();
+  }
+}
+
+int pp_wait(pp_ctx* ctx, pp_stream* s, int timeout_ms, int32_t* first_unsignalled) {
+  if (ctx == nullptr || s == nullptr) return set_error(PP_ERR_INVALID, "null");
+  if (first_unsignalled) *first_unsignalled = -1;
+  if (!s->launched) return PP_OK;
+  const auto t0 = std::chrono::steady_clock::now();
+  for (;;) {
+    hipError_t q = hipEventQuery(s->ev1);
+    if (q == hipSuccess) break;
+    if (q != hipErrorNotReady) return hip_fail(q, "hipEventQuery");
+    const double el = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    if (timeout_ms > 0 && el > timeout_ms) return set_error(PP_ERR_TIMEOUT, "host wait timed out");
+    std::this_thread::sleep_for(std::chrono::microseconds(50));
+  }
+  int32_t st[8];
+  HIP_TRY(hipMemcpy(st, s->d_status, sizeof(st), hipMemcpyDeviceToHost));
+  if (st[0] == pp::VM_OK) return PP_OK;
+  if (first_unsignalled) {
+    uint32_t epoch = 0;
+    std::vector<uint32_t> flags(std::max(s->n_insts, 1));
+    HIP_TRY(hipMemcpy(&epoch, s->d_epoch, 4, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(flags.data(), s->d_flags, 4 * flags.size(), hipMemcpyDeviceToHost));
+    for (int i = 0; i < s->n_insts; ++i)
+      if ((int32_t)(flags[i] - epoch) < 0) {
+        *first_unsignalled = i;
+        break;
+      }
+  }
+  const char* what = st[0] == pp::VM_TIMEOUT ? "watchdog: wait timed out"
+                     : st[0] == pp::VM_BAD_OPCODE ? "bad opcode"
+                     : st[0] == pp::VM_BAD_OPERATION ? "bad operation"
+                                                     : "aborted";
+  return set_error(st[0] == pp::VM_TIMEOUT ? PP_ERR_TIMEOUT : PP_ERR_DEVICE,
+                   std::string("VM ") + what + " (worker " + std::to_string(st[1]) + ", id " + std::to_string(st[2]) +
+                       ")");
+}
+
+double pp_stream_last_ms(const pp_stream* s) {
+  if (s == nullptr || !s->launched) return -1.;
+  float ms = 0.f;
+  if (hipEventElapsedTime(&ms, s->ev0, s->ev1) != hipSuccess) return -1.;
+  return ms;
+}
+
+int64_t pp_read_trace(pp_ctx* ctx, pp_stream* s, uint64_t* out, int64_t cap) {
+  if (ctx == nullptr || s == nullptr) return set_error(PP_ERR_INVALID, "null");
+  if (out != nullptr && cap > 0) {
+    hipError_t e = hipMemcpy(out, s->d_trace, sizeof(uint64_t) * std::min(cap, s->trace_total), hipMemcpyDeviceToHost);
+    if (e != hipSuccess) return hip_fail(e, "trace copy");
+  }
+  return s->trace_total;
+}
+
+int pp_stream_destroy(pp_ctx* ctx, pp_stream* s) {
+  if (ctx == nullptr) return set_error(PP_ERR_INVALID, "null ctx");
+  if (s && s->launched) (void)hipEventSynchronize(s->ev1);
+  destroy_stream(s);
+  return PP_OK;
+}
+
+int pp_fill_sketch(pp_ctx* ctx, void* dptr, int64_t count, int precision, uint64_t seed) {
+  if (ctx == nullptr || dptr == nullptr || count < 0) return set_error(PP_ERR_INVALID, "bad args");
+  std::vector<double> v;
+  sketch_values(v, count, seed);
+  if (precision == PP_FP64) {
+    HIP_TRY(hipMemcpy(dptr, v.data(), v.size() * 8, hipMemcpyHostToDevice));
+  } else {
+    std::vector<float> f(v.begin(), v.end());
+    HIP_TRY(hipMemcpy(dptr, f.data(), f.size() * 4, hipMemcpyHostToDevice));
+  }
+  return PP_OK;
+}
+
+int pp_exec_single(pp_ctx* ctx, int precision, const int32_t* words, int n_words, const void* const* ptr_table,
+                   int n_ptrs, uint64_t scratch_bytes, uint64_t seed) {
+  if (ctx == nullptr || words == nullptr || n_words < 3 || ptr_table == nullptr || n_ptrs < 1)
+    return set_error(PP_ERR_INVALID, "bad args");
+  const int len = operationLength(words[2]);
+  if (words[0] != execute || len <= 0 || n_words < len) return set_error(PP_ERR_INVALID, "bad instruction words");
+  for (int i = 3; i < len && i < 8; ++i) (void)i;
+  int32_t* d_w = nullptr;
+  void** d_p = nullptr;
+  void* d_s = nullptr;
+  void* d_r = nullptr;
+  int32_t* d_st = nullptr;
+  int rc = PP_OK;
+  const int eb = elem_bytes(precision);
+  int64_t need = 1;
+  if (words[2] == accum) need = (int64_t)words[11] * words[13];
+  std::vector<double> rv;
+  sketch_values(rv, need, seed);
+  do {
+    if ((rc = ctx_alloc_t(ctx, n_words, &d_w))) break;
+    if ((rc = ctx_alloc_t(ctx, n_ptrs, &d_p))) break;
+    if ((rc = ctx_alloc(ctx, std::max<uint64_t>(scratch_bytes, 256), &d_s))) break;
+    if ((rc = ctx_alloc(ctx, need * eb, &d_r))) break;
+    if ((rc = ctx_alloc_t(ctx, 4, &d_st))) break;
+    hipError_t e = hipMemcpy(d_w, words, 4 * n_words, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(d_p, ptr_table, sizeof(void*) * n_ptrs, hipMemcpyHostToDevice);
+    if (e == hipSuccess) {
+      if (precision == PP_FP64) e = hipMemcpy(d_r, rv.data(), need * 8, hipMemcpyHostToDevice);
+      else {
+        std::vector<float> f(rv.begin(), rv.end());
+        e = hipMemcpy(d_r, f.data(), need * 4, hipMemcpyHostToDevice);
+      }
+    }
+    if (e == hipSuccess) e = pp::launch_single_op(precision, d_w, d_p, d_s, d_r, d_st, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    int32_t st = 0;
+    if (e == hipSuccess) e = hipMemcpy(&st, d_st, 4, hipMemcpyDeviceToHost);
+    if (e != hipSuccess) {
+      rc = hip_fail(e, "exec_single");
+      break;
+    }
+    if (st != len) rc = set_error(PP_ERR_DEVICE, "device rejected the instruction");
+  } while (false);
+  ctx_free(ctx, d_w);
+  ctx_free(ctx, d_p);
+  ctx_free(ctx, d_s);
+  ctx_free(ctx, d_r);
+  ctx_free(ctx, d_st);
+  return rc;
+}
+
+int pp_mfma_probe(pp_ctx* ctx, int mode, const double* a, int na, const double* b, int nb, double* out) {
+  if (ctx == nullptr || a == nullptr || b == nullptr || out == nullptr) return set_error(PP_ERR_INVALID, "null");
+  double *da = nullptr, *db = nullptr, *dout = nullptr;
+  int rc = PP_OK;
+  do {
+    if ((rc = ctx_alloc_t(ctx, na, &da))) break;
+    if ((rc = ctx_alloc_t(ctx, nb, &db))) break;
+    if ((rc = ctx_alloc_t(ctx, 256, &dout))) break;
+    hipError_t e = hipMemcpy(da, a, 8 * na, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(db, b, 8 * nb, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = pp::launch_mfma_probe(mode, da, db, dout, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (e == hipSuccess) e = hipMemcpy(out, dout, 8 * 256, hipMemcpyDeviceToHost);
+    if (e != hipSuccess) rc = hip_fail(e, "mfma_probe");
+  } while (false);
+  ctx_free(ctx, da);
+  ctx_free(ctx, db);
+  ctx_free(ctx, dout);
+  return rc;
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------------ solver
+
+struct pp_solver {
+  pp_ctx* ctx = nullptr;
+  const pp_hmatrix* hm = nullptr;
+  const Hierarchical* h = nullptr;
+  int precision = 0, workers = 0, wpi = 4, nrhs_max = 0, rhs_split = 1, n = 0;
+  int64_t storage = 0;
+  double* d_A = nullptr;  // FP64 matrix (residual)
+  void* d_F = nullptr;    // factor arena (storage precision)
+  pp_program* fprog = nullptr;
+  pp_stream* fstream = nullptr;
+  struct SolveProg {
+    pp_program* prog = nullptr;
+    pp_stream* stream = nullptr;
+  };
+  std::map<int, SolveProg> sprogs;
+  void* d_rhs = nullptr;   // n x nrhs_max, storage precision
+  double* d_r = nullptr;   // n x nrhs_max residual
+  double* d_t = nullptr;   // matvec low-rank products
+  double* d_nrm = nullptr; // 2 * nrhs_max
+  // matvec plan
+  pp::MatvecBlock* d_blocks = nullptr;
+  pp::MatvecTile* d_tiles = nullptr;
+  int* d_contrib = nullptr;
+  int* d_lr = nullptr;
+  pp::MatvecPlanDev plan{};
+  double factor_flops = 0;
+  double last_factor_ms = 0;
+  std::vector<void*> owned;
+};
+
+namespace {
+
+const Hierarchical* hier_of(const pp_hmatrix* h) { return h->h.get(); }
+
+int build_plan(pp_solver* s) {
+  std::vector<const h_block*> lv;
+  s->h->leaves(lv);
+  std::vector<pp::MatvecBlock> blocks;
+  std::vector<int> lr;
+  int64_t t_off = 0;
+  for (const h_block* b : lv) {
+    pp::MatvecBlock m{};
+    m.type = (int)b->type;
+    m.row0 = b->row0;
+    m.col0 = b->col0;
+    m.ny = b->ny;
+    m.nx = b->nx;
+    m.rank = b->rank;
+    m.off = b->off;
+    m.off_u = b->off_u;
+    m.off_v = b->off_v;
+    if (b->type == element_t::low_rank) {
+      m.t_off = t_off;
+      t_off += (int64_t)b->rank * s->nrhs_max;
+      lr.push_back((int)blocks.size());
+    }
+    blocks.push_back(m);
+  }
+  // row tiles: row ranges of the diagonal leaves
+  std::vector<std::pair<int, int>> ranges;
+  for (const h_block* b : lv)
+    if (b->row0 == b->col0 && b->type == element_t::dense) ranges.emplace_back(b->row0, b->ny);
+  std::sort(ranges.begin(), ranges.end());
+  std::vector<pp::MatvecTile> tiles;
+  std::vector<int> contrib;
+  for (const auto& r : ranges) {
+    pp::MatvecTile t{};
+    t.row0 = r.first;
+    t.rows = r.second;
+    t.first = (int)(contrib.size() / 2);
+    for (size_t i = 0; i < blocks.size(); ++i) {
+      const auto& b = blocks[i];
+      if (b.row0 <= r.first && r.first + r.second <= b.row0 + b.ny) {
+        contrib.push_back((int)i);
+        contrib.push_back(0);
+      }
+    }
+    t.count = (int)(contrib.size() / 2) - t.first;
+    tiles.push_back(t);
+  }
+  int64_t covered = 0;
+  for (const auto& t : tiles) covered += t.rows;
+  if (covered != s->n) return set_error(PP_ERR_INVALID, "matvec plan: diagonal leaves do not tile the rows");
+  int rc;
+  if ((rc = ctx_alloc_t(s->ctx, blocks.size(), &s->d_blocks))) return rc;
+  s->owned.push_back(s->d_blocks);
+  if ((rc = ctx_alloc_t(s->ctx, tiles.size(), &s->d_tiles))) return rc;
+  s->owned.push_back(s->d_tiles);
+  if ((rc = ctx_alloc_t(s->ctx, std::max<size_t>(contrib.size(), 2), &s->d_contrib))) return rc;
+  s->owned.push_back(s->d_contrib);
+  if ((rc = ctx_alloc_t(s->ctx, std::max<size_t>(lr.size(), 1), &s->d_lr))) return rc;
+  s->owned.push_back(s->d_lr);
+  if ((rc = ctx_alloc_t(s->ctx, std::max<int64_t>(t_off, 1), &s->d_t))) return rc;
+  s->owned.push_back(s->d_t);
+  HIP_TRY(hipMemcpy(s->d_blocks, blocks.data(), sizeof(pp::MatvecBlock) * blocks.size(), hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(s->d_tiles, tiles.data(), sizeof(pp::MatvecTile) * tiles.size(), hipMemcpyHostToDevice));
+  if (!contrib.empty())
+    HIP_TRY(hipMemcpy(s->d_contrib, contrib.data(), sizeof(int) * contrib.size(), hipMemcpyHostToDevice));
+  if (!lr.empty()) HIP_TRY(hipMemcpy(s->d_lr, lr.data(), sizeof(int) * lr.size(), hipMemcpyHostToDevice));
+  s->plan.blocks = s->d_blocks;
+  s->plan.tiles = s->d_tiles;
+  s->plan.contrib = s->d_contrib;
+  s->plan.lr_list = s->d_lr;
+  s->plan.n_tiles = (int)tiles.size();
+  s->plan.n_lowrank = (int)lr.size();
+  return PP_OK;
+}
+
+void destroy_solver(pp_solver* s) {
+  if (s == nullptr) return;
+  if (s->fstream) pp_stream_destroy(s->ctx, s->fstream);
+  if (s->fprog) pp_program_destroy(s->fprog);
+  for (auto& kv : s->sprogs) {
+    if (kv.second.stream) pp_stream_destroy(s->ctx, kv.second.stream);
+    if (kv.second.prog) pp_program_destroy(kv.second.prog);
+  }
+  for (void* p : s->owned) (void)ctx_free(s->ctx, p);
+  delete s;
+}
+
+int solver_create(pp_ctx* ctx, const pp_hmatrix* hm, const double* storage, bool on_device, int precision, int workers,
+                  int wpi, int nrhs_max, int rhs_split, pp_solver** out) {
+  if (ctx == nullptr || hm == nullptr || storage == nullptr || out == nullptr || precision < 0 || precision > 2 ||
+      workers < 1 || nrhs_max < 1 || rhs_split < 1)
+    return set_error(PP_ERR_INVALID, "bad solver args");
+  auto s = new pp_solver();
+  s->ctx = ctx;
+  s->hm = hm;
+  s->h = hier_of(hm);
+  s->precision = precision;
+  s->workers = workers;
+  s->wpi = wpi;
+  s->nrhs_max = nrhs_max;
+  s->rhs_split = rhs_split;
+  s->n = s->h->getNx_abs();
+  s->storage = s->h->getStorage();
+  int rc = PP_OK;
+  const int eb = elem_bytes(precision);
+  do {
+    if ((rc = ctx_alloc_t(ctx, s->storage, &s->d_A))) break;
+    s->owned.push_back(s->d_A);
+    {
+      hipError_t e = hipMemcpy(s->d_A, storage, 8 * s->storage, on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice);
+      if (e != hipSuccess) {
+        rc = hip_fail(e, "upload matrix");
+        break;
+      }
+    }
+    if (precision == PP_FP64) {
+      if ((rc = ctx_alloc_t(ctx, s->storage, reinterpret_cast<double**>(&s->d_F)))) break;
+    } else {
+      if ((rc = ctx_alloc_t(ctx, s->storage, reinterpret_cast<float**>(&s->d_F)))) break;
+    }
+    s->owned.push_back(s->d_F);
+    if ((rc = ctx_alloc(ctx, (uint64_t)s->n * nrhs_max * eb, &s->d_rhs))) break;
+    s->owned.push_back(s->d_rhs);
+    if ((rc = ctx_alloc_t(ctx, (int64_t)s->n * nrhs_max, &s->d_r))) break;
+    s->owned.push_back(s->d_r);
+    if ((rc = ctx_alloc_t(ctx, 2 * nrhs_max, &s->d_nrm))) break;
+    s->owned.push_back(s->d_nrm);
+    if ((rc = build_plan(s))) break;
+    if ((rc = pp_program_factor(hm, workers, &s->fprog))) break;
+    void* ptrs[3] = {s->d_F, nullptr, s->d_rhs};
+    if ((rc = pp_stream_upload_program(ctx, s->fprog, ptrs, 3, precision, &s->fstream))) break;
+    s->factor_flops = (double)s->fprog->dag->getFlops();
+  } while (false);
+  if (rc != PP_OK) {
+    destroy_solver(s);
+    return rc;
+  }
+  *out = s;
+  return PP_OK;
+}
+
+int solve_prog(pp_solver* s, int nrhs, pp_solver::SolveProg** out) {
+  auto it = s->sprogs.find(nrhs);
+  if (it == s->sprogs.end()) {
+    pp_solver::SolveProg sp;
+    int rc = pp_program_solve(s->hm, nrhs, nrhs, s->rhs_split, s->workers, &sp.prog);
+    if (rc) return rc;
+    void* ptrs[3] = {s->d_F, nullptr, s->d_rhs};
+    rc = pp_stream_upload_program(s->ctx, sp.prog, ptrs, 3, s->precision, &sp.stream);
+    if (rc) {
+      pp_program_destroy(sp.prog);
+      return rc;
+    }
+    it = s->sprogs.emplace(nrhs, sp).first;
+  }
+  *out = &it->second;
+  return PP_OK;
+}
+
+// d_rhs <- src (n x nrhs, pitch ld, FP64), solve in place with the factors
+int apply_inverse(pp_solver* s, const double* src, int ld, int nrhs, hipStream_t st) {
+  pp_solver::SolveProg* sp = nullptr;
+  int rc = solve_prog(s, nrhs, &sp);
+  if (rc) return rc;
+  const int eb = elem_bytes(s->precision);
+  hipError_t e = pp::launch_copy2d(s->d_rhs, eb, nrhs, src, 8, ld, s->n, nrhs, st);
+  if (e != hipSuccess) return hip_fail(e, "copy rhs");
+  rc = pp_launch(s->ctx, sp->stream, s->wpi, st);
+  if (rc) return rc;
+  return PP_OK;
+}
+
+int wait_stream(pp_solver* s, pp_stream* st_) {
+  int32_t first = -1;
+  return pp_wait(s->ctx, st_, 0, &first);
+}
+
+}  // namespace
+
+extern "C" {
+
+int pp_solver_create(pp_ctx* ctx, const pp_hmatrix* h, const double* storage, int precision, int workers,
+                     int wavefronts_per_inst, int nrhs_max, int rhs_split, pp_solver** out) {
+  try {
+    return solver_create(ctx, h, storage, false, precision, workers, wavefronts_per_inst, nrhs_max, rhs_split, out);
+  } catch (...) {
+    return from_exception();
+  }
+}
+
+int pp_solver_create_device(pp_ctx* ctx, const pp_hmatrix* h, const double* storage_dev, int precision, int workers,
+                            int wavefronts_per_inst, int nrhs_max, int rhs_split, pp_solver** out) {
+  try {
+    return solver_create(ctx, h, storage_dev, true, precision, workers, wavefronts_per_inst, nrhs_max, rhs_split, out);
+  } catch (...) {
+    return from_exception();
+  }
+}
+
+int pp_solver_factor(pp_solver* s, void* hip_stream) {
+  try {
+    if (s == nullptr) return set_error(PP_ERR_INVALID, "null solver");
+    hipStream_t st = pick(s->ctx, hip_stream);
+    hipError_t e = pp::launch_convert(s->d_F, elem_bytes(s->precision), s->d_A, 8, s->storage, st);
+    if (e != hipSuccess) return hip_fail(e, "restore factor arena");
+    int rc = pp_launch(s->ctx, s->fstream, s->wpi, st);
+    if (rc) return rc;
+    rc = wait_stream(s, s->fstream);
+    if (rc) return rc;
+    s->last_factor_ms = pp_stream_last_ms(s->fstream);
+    return PP_OK;
+  } catch (...) {
+    return from_exception();
+  }
+}
+
+int pp_solver_residual(pp_solver* s, const double* b, const double* x, double* r, int nrhs, int ld, void* hip_stream) {
+  if (s == nullptr || b == nullptr || x == nullptr || r == nullptr || nrhs < 1 || nrhs > s->nrhs_max || ld < nrhs)
+    return set_error(PP_ERR_INVALID, "bad residual args");
+  hipStream_t st = pick(s->ctx, hip_stream);
+  hipError_t e = pp::launch_hmatvec(s->plan, s->d_A, x, ld, b, ld, r, ld, nrhs, s->d_t, st);
+  if (e != hipSuccess) return hip_fail(e, "hmatvec");
+  HIP_TRY(hipStreamSynchronize(st));
+  return PP_OK;
+}
+
+int pp_solver_solve(pp_solver* s, const double* b, double* x, int nrhs, int ld, double tol, int max_iter,
+                    void* hip_stream, pp_solve_report* rep) {
+  try {
+    if (s == nullptr || b == nullptr || x == nullptr || nrhs < 1 || nrhs > s->nrhs_max || ld < nrhs || max_iter < 0)
+      return set_error(PP_ERR_INVALID, "bad solve args");
+    hipStream_t st = pick(s->ctx, hip_stream);
+    const auto t0 = std::chrono::steady_clock::now();
+    const int eb = elem_bytes(s->precision);
+    // ||b|| per column
+    std::vector<double> bn(nrhs), rn(nrhs);
+    hipError_t e = pp::launch_colnorm2(b, ld, s->n, nrhs, s->d_nrm + s->nrhs_max, st);
+    if (e != hipSuccess) return hip_fail(e, "norm b");
+    // x = F^{-1} b
+    int rc = apply_inverse(s, b, ld, nrhs, st);
+    if (rc) return rc;
+    e = pp::launch_copy2d(x, 8, ld, s->d_rhs, eb, nrhs, s->n, nrhs, st);
+    if (e != hipSuccess) return hip_fail(e, "copy x");
+    pp_solver::SolveProg* sp = &s->sprogs[nrhs];
+    rc = wait_stream(s, sp->stream);
+    if (rc) return rc;
+    HIP_TRY(hipMemcpy(bn.data(), s->d_nrm + s->nrhs_max, 8 * nrhs, hipMemcpyDeviceToHost));
+    double res = 0.;
+    int it = 0;
+    bool conv = false;
+    for (;;) {
+      // r = b - A x (FP64)
+      e = pp::launch_hmatvec(s->plan, s->d_A, x, ld, b, ld, s->d_r, nrhs, nrhs, s->d_t, st);
+      if (e != hipSuccess) return hip_fail(e, "hmatvec");
+      e = pp::launch_colnorm2(s->d_r, nrhs, s->n, nrhs, s->d_nrm, st);
+      if (e != hipSuccess) return hip_fail(e, "norm r");
+      HIP_TRY(hipMemcpyAsync(rn.data(), s->d_nrm, 8 * nrhs, hipMemcpyDeviceToHost, st));
+      HIP_TRY(hipStreamSynchronize(st));
+      res = 0.;
+      for (int c = 0; c < nrhs; ++c) {
+        const double rel = bn[c] > 0 ? std::sqrt(rn[c] / bn[c]) : std::sqrt(rn[c]);
+        res = std::max(res, rel);
+      }
+      if (res <= tol) {
+        conv = true;
+        break;
+      }
+      if (it >= max_iter) break;
+      ++it;
+      rc = apply_inverse(s, s->d_r, nrhs, nrhs, st);
+      if (rc) return rc;
+      e = pp::launch_add_correction(x, ld, s->d_rhs, eb, nrhs, s->n, nrhs, st);
+      if (e != hipSuccess) return hip_fail(e, "update x");
+      rc = wait_stream(s, sp->stream);
+      if (rc) return rc;
+    }
+    if (rep) {
+      rep->iterations = it;
+      rep->converged = conv ? 1 : 0;
+      rep->residual = res;
+      rep->factor_ms = s->last_factor_ms;
+      rep->total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+      rep->solve_ms = rep->total_ms;
+    }
+    return PP_OK;
+  } catch (...) {
+    return from_exception();
+  }
+}
+
+int pp_solver_factor_arena(pp_solver* s, void** dptr, int* eb) {
+  if (s == nullptr || dptr == nullptr || eb == nullptr) return set_error(PP_ERR_INVALID, "null");
+  *dptr = s->d_F;
+  *eb = elem_bytes(s->precision);
+  return PP_OK;
+}
+
+int pp_solver_destroy(pp_solver* s) {
+  destroy_solver(s);
+  return PP_OK;
+}
+
+pp_stream* pp_solver_factor_stream(pp_solver* s) { return s ? s->fstream : nullptr; }
+double pp_solver_factor_flops(const pp_solver* s) { return s ? s->factor_flops : 0.; }
+
+double pp_solver_solve_flops(const pp_solver* s, int nrhs) {
+  if (s == nullptr) return 0.;
+  return 2.0 * (double)s->h->storedEntries() * nrhs;
+}
+
+}  // extern "C"

@@ -1,0 +1,205 @@
+"""GPU parity of the device routines, one VM instruction at a time, against
+the CPU oracle (oracle.c restates src/kernel.cu) -- and the MFMA lane layouts
+the routines rely on.
+
+Tolerances: FP64 routines agree with the oracle to 1e-12 relative (different
+summation order only); FP32 to 2e-5; BF16-input MFMA to 2e-2.
+"""
+import numpy as np
+import pytest
+import torch
+
+pytestmark = pytest.mark.gpu
+
+FP64, FP32, MIXED = 0, 1, 2
+TOL = {FP64: 1e-12, FP32: 2e-5, MIXED: 2e-2}
+
+
+def dev(a, prec=FP64):
+    t = torch.from_numpy(np.ascontiguousarray(a, np.float64 if prec == FP64 else np.float32)).cuda()
+    return t
+
+
+def host(t):
+    torch.cuda.synchronize()
+    return t.double().cpu().numpy()
+
+
+def rel(a, b):
+    return np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300)
+
+
+@pytest.mark.parametrize("mode", [0, 1, 2])
+def test_mfma_lane_layout(ctx, mode):
+    """D = A B for one 16x16 tile: lane l, register r holds D[4(l/16)+r][l%16];
+    A lanes supply A[l%16][kpl (l/16) + e], B lanes B[kpl (l/16) + e][l%16]."""
+    k = 32 if mode == 2 else 4
+    rng = np.random.default_rng(mode)
+    a = rng.integers(-4, 5, (16, k)).astype(np.float64)
+    b = rng.integers(-4, 5, (k, 16)).astype(np.float64)
+    out = ctx.mfma_probe(mode, a.ravel(), b.ravel())
+    d = a @ b
+    for l in range(64):
+        for r in range(4):
+            assert out[4 * l + r] == d[4 * (l // 16) + r, l % 16], (l, r)
+
+
+def run(ctx, prec, words, bufs, scratch_elems=1 << 18):
+    """Execute one instruction; slots are the buffers in order."""
+    w = np.array([1, 0] + list(words), np.int32)
+    ctx.exec_single(prec, w, [b.data_ptr() for b in bufs], scratch_bytes=8 * scratch_elems)
+    torch.cuda.synchronize()
+
+
+def diag_dominant(rng, n, m=None):
+    m = m or n
+    a = rng.standard_normal((n, m))
+    a[np.arange(min(n, m)), np.arange(min(n, m))] += 2.0 * max(n, m)
+    return a
+
+
+@pytest.mark.parametrize("prec", [FP64, FP32])
+@pytest.mark.parametrize("n,m", [(256, 256), (128, 128), (200, 200), (256, 100), (90, 300)])
+def test_getrf(ctx, orc, prec, n, m):
+    rng = np.random.default_rng(n + m)
+    a = diag_dominant(rng, n, m)
+    ld = m + 3
+    buf = np.zeros((n, ld))
+    buf[:, :m] = a
+    d = dev(buf, prec)
+    run(ctx, prec, [1, 0, 0, m, n, ld], [d])
+    want = orc.getrf(a)
+    assert rel(host(d)[:, :m], want) < TOL[prec] * 10
+
+
+@pytest.mark.parametrize("prec", [FP64, FP32])
+@pytest.mark.parametrize("ny,nx,bt", [(256, 32, 0), (256, 256, 0), (128, 7, 0), (256, 32, 1), (100, 40, 1)])
+def test_trsml(ctx, orc, prec, ny, nx, bt):
+    rng = np.random.default_rng(ny * 7 + nx)
+    l = np.tril(rng.standard_normal((ny, ny)) / ny, -1) + np.eye(ny)
+    b = rng.standard_normal((nx, ny) if bt else (ny, nx))
+    dl, db = dev(l, prec), dev(b, prec)
+    run(ctx, prec, [2, 0, 1, 0, 0, nx, ny, ny, b.shape[1], ny, bt], [db, dl])
+    want = orc.trsml(b, l, b_t=bool(bt))
+    assert rel(host(db), want) < TOL[prec] * 10
+
+
+@pytest.mark.parametrize("prec", [FP64, FP32])
+@pytest.mark.parametrize("ny,nx,bt", [(256, 256, 0), (32, 256, 1), (77, 130, 0), (19, 200, 1)])
+def test_trsmr(ctx, orc, prec, ny, nx, bt):
+    rng = np.random.default_rng(ny * 5 + nx)
+    u = np.triu(rng.standard_normal((nx, nx)) / nx) + 2.0 * np.eye(nx)
+    b = rng.standard_normal((nx, ny) if bt else (ny, nx))
+    du, db = dev(u, prec), dev(b, prec)
+    run(ctx, prec, [3, 0, 1, 0, 0, nx, ny, nx, b.shape[1], nx, bt], [db, du])
+    want = orc.trsmr(b, u, b_t=bool(bt))
+    assert rel(host(db), want) < TOL[prec] * 10
+
+
+@pytest.mark.parametrize("prec", [FP64, FP32])
+@pytest.mark.parametrize("ny,nx", [(256, 16), (256, 1), (131, 33)])
+def test_trsmu(ctx, orc, prec, ny, nx):
+    rng = np.random.default_rng(ny + 3 * nx)
+    u = np.triu(rng.standard_normal((ny, ny)) / ny) + 2.0 * np.eye(ny)
+    b = rng.standard_normal((ny, nx))
+    du, db = dev(u, prec), dev(b, prec)
+    run(ctx, prec, [9, 0, 1, 0, 0, nx, ny, ny, nx, ny, 0], [db, du])
+    assert rel(host(db), orc.trsmu(b, u)) < TOL[prec] * 10
+
+
+GEMM_SHAPES = [(256, 256, 256), (37, 53, 29), (4096, 32, 32), (32, 32, 4096), (1, 300, 17), (256, 256, 16),
+               (65, 33, 100)]
+
+
+@pytest.mark.parametrize("prec", [FP64, FP32, MIXED])
+@pytest.mark.parametrize("m,n,k", GEMM_SHAPES)
+@pytest.mark.parametrize("at,bt", [(0, 0), (1, 0), (0, 1), (1, 1)])
+@pytest.mark.parametrize("op", [4, 5])
+def test_gemm(ctx, orc, prec, m, n, k, at, bt, op):
+    if prec == MIXED and (at, bt) != (0, 0) and (m, n, k) != (37, 53, 29):
+        pytest.skip("BF16 transposes covered on one ragged shape")
+    rng = np.random.default_rng(m * 31 + n * 7 + k + at * 2 + bt)
+    mm = rng.standard_normal((m, n))
+    a = rng.standard_normal((k, m) if at else (m, k))
+    b = rng.standard_normal((n, k) if bt else (k, n))
+    if prec == MIXED:  # exactly representable operands
+        a = np.round(a * 8) / 8
+        b = np.round(b * 8) / 8
+    dm, da, db = dev(mm, prec), dev(a, prec), dev(b, prec)
+    run(ctx, prec, [op, 0, 1, 2, 0, 0, 0, m, n, k, n, a.shape[1], b.shape[1], at, bt], [dm, da, db])
+    want = orc.gemm(-1.0 if op == 4 else 1.0, 1.0, mm, a, b, bool(at), bool(bt))
+    scale = np.max(np.abs(want)) + np.sqrt(k) * 3
+    assert np.max(np.abs(host(dm) - want)) / scale < TOL[prec] * (1 if prec != FP32 else 10)
+
+
+def _gemm3_words(m, n, k, l, trans, control):
+    at, bt, ct = trans
+    return [6, 0, 1, 2, 3, 0, 0, 0, 0, m, n, k, l, n, (m if at else k), (k if bt else l), (l if ct else n),
+            at, bt, ct, control]
+
+
+@pytest.mark.parametrize("prec", [FP64, FP32])
+@pytest.mark.parametrize("control", [0, 1])
+@pytest.mark.parametrize("trans", [(0, 0, 0), (0, 1, 0), (1, 0, 1)])
+def test_gemm3(ctx, orc, prec, control, trans):
+    m, n, k, l = 96, 80, 32, 40
+    rng = np.random.default_rng(control * 10 + sum(trans))
+    at, bt, ct = trans
+    mm = rng.standard_normal((m, n))
+    a = rng.standard_normal((k, m) if at else (m, k))
+    b = rng.standard_normal((l, k) if bt else (k, l))
+    c = rng.standard_normal((n, l) if ct else (l, n))
+    bufs = [dev(x, prec) for x in (mm, a, b, c)]
+    run(ctx, prec, _gemm3_words(m, n, k, l, trans, control), bufs)
+    op = lambda x, t: x.T if t else x
+    want = mm - op(a, at) @ op(b, bt) @ op(c, ct)
+    assert rel(host(bufs[0]), want) < TOL[prec] * 10
+
+
+@pytest.mark.parametrize("prec", [FP64, FP32])
+@pytest.mark.parametrize("control", [0, 1, 2, 3, 4])
+def test_gemm4(ctx, orc, prec, control):
+    m, n, k, l, o = 64, 72, 24, 20, 28
+    rng = np.random.default_rng(control)
+    mm = rng.standard_normal((m, n))
+    a, b = rng.standard_normal((m, k)), rng.standard_normal((l, k))   # B transposed
+    c, d = rng.standard_normal((l, o)), rng.standard_normal((n, o))   # D transposed
+    bufs = [dev(x, prec) for x in (mm, a, b, c, d)]
+    t1 = {0: m * l, 1: k * o, 2: k * o, 3: l * n, 4: m * l}[control]
+    words = [7, 0, 1, 2, 3, 4, 0, 0, 0, 0, 0, m, n, k, l, o, n, k, k, o, o, 0, 1, 0, 1, control, (t1 + 31) // 32 * 32]
+    run(ctx, prec, words, bufs)
+    want = mm - a @ b.T @ c @ d.T
+    assert rel(host(bufs[0]), want) < TOL[prec] * 10
+
+
+@pytest.mark.parametrize("prec", [FP64, FP32, MIXED])
+@pytest.mark.parametrize("nx,ny,k1,k2", [(256, 256, 32, 32), (300, 500, 16, 24), (1024, 2048, 32, 32),
+                                         (128, 96, 48, 8)])
+def test_accum(ctx, orc, pp, prec, nx, ny, k1, k2):
+    """U1 V1^T after accum equals the oracle's (Givens-based) projection;
+    the factors themselves may differ by column signs."""
+    rng = np.random.default_rng(nx + ny + k1 + k2)
+    # a numerically rank-(k1) sum so the projection is well conditioned
+    u1 = rng.standard_normal((ny, k1))
+    v1 = rng.standard_normal((nx, k1)) * (0.5 ** np.arange(k1))
+    u2 = rng.standard_normal((ny, k2)) * 1e-3
+    v2 = rng.standard_normal((nx, k2))
+    rnd = pp.sketch(pp.SKETCH_SEED, nx * k1)
+    wu, wv = orc.accum(u1, v1, u2, v2, rnd)
+    want = wu @ wv.T
+    bufs = [dev(x, prec) for x in (u1, v1, u2, v2)]
+    o1 = (ny * k1 + 31) // 32 * 32
+    o2 = (o1 + nx * k1 + 31) // 32 * 32
+    words = [8, 0, 1, 2, 3, 0, 0, 0, 0, nx, ny, k1, k2, k1, k1, k2, k2, o1, o2]
+    run(ctx, prec, words, bufs, scratch_elems=o2 + ny * k1 + 4 * k1 * max(k1, k2) + 4096)
+    gu, gv = host(bufs[0]), host(bufs[1])
+    got = gu @ gv.T
+    assert rel(got, want) < {FP64: 1e-10, FP32: 1e-4, MIXED: 5e-2}[prec]
+    if prec == FP64:  # Q orthonormal
+        assert np.max(np.abs(gu.T @ gu - np.eye(k1))) < 1e-12
+
+
+def test_single_rejects_unknown_operation(ctx, pp):
+    buf = torch.zeros(64, dtype=torch.float64, device="cuda")
+    with pytest.raises(pp.PastelError):
+        ctx.exec_single(0, np.array([1, 0, 42, 0, 0, 0, 0, 0], np.int32), [buf.data_ptr()])

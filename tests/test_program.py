@@ -1,0 +1,180 @@
+"""Host pipeline: h-op generation, DAG, schedule, instruction encoding, and
+the CPU oracle executing the encoded streams.
+
+Pins:
+  * FLOP model: the all-dense structure's instruction FLOPs sum to the
+    reference's dense-LU closed form, whose value at N = 1024 is the
+    spreadsheet's 715,303,424 (docs/SC/pspl results.xlsx!G2);
+  * DAG: edges re-derived from the encoded words with an all-pairs
+    checkOverlap (the reference's own build where available);
+  * schedule: the program's per-worker queues equal the Python restatement
+    of rts/scheduler.cpp run on the program's DAG;
+  * numerics: the oracle executing the streams reproduces a dense no-pivot LU
+    (all-dense case) and solves the H-matrix system (HODLR case).
+"""
+import numpy as np
+import pytest
+
+from oracle import scheduler as sref
+from tests import wordmaps
+
+
+def test_dense_flop_closed_form(pp):
+    assert pp.dense_lu_flops(1024) == 715303424
+    # N = 32768: 2.346e13 (docs/SC/sc-extended-abstract.tex:116)
+    assert round(pp.dense_lu_flops(32768) / 1e13, 3) == 2.346
+    for n, leaf in [(1024, 128), (512, 64), (768, 128)]:
+        hm = pp.HMatrix("dense", n, leaf=leaf)
+        assert pp.Program.factor(hm, 4).info()["flops"] == pp.dense_lu_flops(n)
+
+
+def test_hodlr_structure(pp):
+    hm = pp.HMatrix("hodlr", 16384, levels=6, rank=32)
+    lv = hm.leaves()
+    dense = [l for l in lv if l["type"] == "dense"]
+    lowr = [l for l in lv if l["type"] == "lowrank"]
+    assert len(dense) == 64 and all(l["ny"] == 256 and l["row0"] == l["col0"] for l in dense)
+    assert len(lowr) == 2 * (2 ** 6 - 1) and all(l["rank"] == 32 for l in lowr)
+    assert hm.storage >= 64 * 256 * 256 + sum((l["nx"] + l["ny"]) * 32 for l in lowr)
+    info = pp.Program.factor(hm, 256).info()
+    assert 1100 < info["n_insts"] < 1400
+    assert 5.0e9 < info["flops"] < 6.5e9
+
+
+def _decode_program(prog):
+    n = prog.info()["n_insts"]
+    words = [prog.inst_words(i) for i in range(n)]
+    return n, words
+
+
+@pytest.mark.parametrize("kind", ["dense", "hodlr"])
+def test_dag_edges_all_pairs(pp, orc, kind):
+    if kind == "dense":
+        hm = pp.HMatrix("dense", 512, leaf=128)
+    else:
+        hm = pp.HMatrix("hodlr", 2048, levels=3, rank=8)
+    prog = pp.Program.factor(hm, 4)
+    n, words = _decode_program(prog)
+    maps = [wordmaps.operand_maps(w) for w in words]
+    if orc.ref_available():
+        R = orc.ref()
+        overlap = lambda a, b: bool(R.ref_check_overlap(*a, *b))
+    else:
+        overlap = pp.check_overlap
+    for j in range(n):
+        want = []
+        for i in range(j):
+            mask = 0
+            for ro_f, mf in maps[i]:
+                for ro_t, mt in maps[j]:
+                    lo_f, hi_f = mf[0], mf[0] + mf[1] * mf[3]
+                    lo_t, hi_t = mt[0], mt[0] + mt[1] * mt[3]
+                    if hi_f <= lo_t or hi_t <= lo_f or not overlap(mf, mt):
+                        continue
+                    if not ro_f and ro_t:
+                        mask |= 1
+                    if ro_f and not ro_t:
+                        mask |= 2
+                    if not ro_f and not ro_t:
+                        mask |= 4
+            if mask:
+                want.append((i, mask))
+        assert prog.preds(j) == want, f"instruction {j}"
+
+
+@pytest.mark.parametrize("workers", [1, 3, 16, 256])
+def test_program_schedule_bit_exact(pp, workers):
+    hm = pp.HMatrix("hodlr", 8192, levels=5, rank=16)
+    prog = pp.Program.factor(hm, workers)
+    n = prog.info()["n_insts"]
+    pred = [prog.preds(j) for j in range(n)]
+    succ = [[] for _ in range(n)]
+    for j in range(n):
+        for i, m in pred[j]:
+            succ[i].append((j, m))
+    flops = [prog.inst_flops(i) for i in range(n)]
+    want = sref.schedule(n, flops, succ, pred, workers, pp.min_inst_flops())
+    got = [prog.schedule(w) for w in range(workers)]
+    assert got == want
+
+
+def test_streams_encode_schedule(pp):
+    hm = pp.HMatrix("hodlr", 4096, levels=4, rank=16)
+    W = 8
+    prog = pp.Program.factor(hm, W)
+    n = prog.info()["n_insts"]
+    seen = []
+    for w, stream in enumerate(prog.streams()):
+        ents = wordmaps.split_stream(stream)
+        assert ents[-1][0] == "finish"
+        sched = prog.schedule(w)
+        body = ents[:-1]
+        assert len(body) == len(sched)
+        for (kind, words), (inst, ex, _) in zip(body, sched):
+            assert (kind == "exec") == ex and int(words[1]) == inst
+            if ex:
+                assert list(words) == list(prog.inst_words(inst))
+                seen.append(inst)
+            else:
+                assert 0 <= int(words[2]) <= 63
+    assert sorted(seen) == list(range(n))
+
+
+def test_oracle_executes_dense_lu(pp, orc):
+    """All-dense H-LU through encoded streams == dense no-pivot LU."""
+    n = 512
+    hm, kern, arena = pp.build_config(0, n, leaf=128)
+    a = pp.densify(hm, arena)
+    prog = pp.Program.factor(hm, 3)
+    fac = arena.copy()
+    orc.run_streams(prog.streams(), [fac, None, None], prog.info()["n_insts"], 1 << 16, threads=1)
+    lu_h = pp.densify(hm, fac)
+    lu_d = orc.dense_lu(a)
+    assert np.max(np.abs(lu_h - lu_d)) / np.max(np.abs(lu_d)) < 1e-12
+    l, u = pp.lu_factors(lu_h)
+    assert np.linalg.norm(a - l @ u) / np.linalg.norm(a) < 1e-14
+
+
+def test_oracle_threaded_matches_sequential(pp, orc):
+    hm, kern, arena = pp.build_config(1, 2048, levels=3, rank=16)
+    prog = pp.Program.factor(hm, 4)
+    info = prog.info()
+    f1, f2 = arena.copy(), arena.copy()
+    t1, t2 = np.zeros(max(info["temp_elems"], 1)), np.zeros(max(info["temp_elems"], 1))
+    rnd = pp.sketch(pp.SKETCH_SEED, 1 << 16)
+    orc.run_streams(prog.streams(), [f1, t1, None], info["n_insts"], info["scratch_elems"], rnd=rnd, threads=1)
+    orc.run_streams(prog.streams(), [f2, t2, None], info["n_insts"], info["scratch_elems"], rnd=rnd, threads=4)
+    assert np.array_equal(f1, f2)
+
+
+def test_oracle_hodlr_factor_and_solve(pp, orc):
+    """HODLR H-LU + H-solve through the encoded streams: the solve residual
+    against the densified A_H is at the level of the recompression error, and
+    FP64 refinement with the H-LU factors drives it to 1e-12."""
+    n = 2048
+    hm, kern, arena = pp.build_config(1, n, levels=3, rank=32)
+    a = pp.densify(hm, arena)
+    prog = pp.Program.factor(hm, 4)
+    info = prog.info()
+    fac = arena.copy()
+    tmp = np.zeros(max(info["temp_elems"], 1))
+    rnd = pp.sketch(pp.SKETCH_SEED, 1 << 16)
+    orc.run_streams(prog.streams(), [fac, tmp, None], info["n_insts"], info["scratch_elems"], rnd=rnd)
+    nrhs = 4
+    b = pp.rhs(1, n, nrhs)
+    sprog = pp.Program.solve(hm, nrhs, 4)
+
+    def apply_inv(r):
+        x = np.ascontiguousarray(r.copy())
+        orc.run_streams(sprog.streams(), [fac, None, x], sprog.info()["n_insts"], sprog.info()["scratch_elems"])
+        return x
+
+    x = apply_inv(b)
+    res0 = orc.dense_residual(a, x, b)
+    assert res0 < 1e-3
+    for _ in range(12):
+        r = b - a @ x
+        if np.max(np.linalg.norm(r, axis=0) / np.linalg.norm(b, axis=0)) < 1e-12:
+            break
+        x = x + apply_inv(r)
+    assert orc.dense_residual(a, x, b) < 1e-12

@@ -197,6 +197,20 @@ int pp_solver_create(pp_ctx* ctx, const pp_hmatrix* h, const double* storage, in
 /* Same, with the FP64 arena already on the device (copied, not adopted). */
 int pp_solver_create_device(pp_ctx* ctx, const pp_hmatrix* h, const double* storage_dev, int precision, int workers,
                             int wavefronts_per_inst, int nrhs_max, int rhs_split, pp_solver** out);
+/* One solver over `count` H-matrices handled as a block-diagonal system:
+ * one persistent launch factors all of them, one solves all of them
+ * (batched execution).  storages[i] is member i's FP64 arena (host memory,
+ * or device memory when on_device != 0).  Right-hand sides and solutions
+ * are the members' rows stacked in order (pp_solver_member gives offsets). */
+int pp_solver_create_batch(pp_ctx* ctx, const pp_hmatrix* const* hs, int count, const double* const* storages,
+                           int on_device, int precision, int workers, int wavefronts_per_inst, int nrhs_max,
+                           int rhs_split, pp_solver** out);
+int pp_solver_num_members(const pp_solver* s);
+int64_t pp_solver_rows(const pp_solver* s);
+/* Row offset, size and last refined residual of member i. */
+int pp_solver_member(const pp_solver* s, int i, int64_t* row_off, int* n, double* last_residual);
+/* Factor through a captured hipGraph (captured on first use per stream). */
+int pp_solver_use_graph(pp_solver* s, int enable);
 /* Factor in place (restores the working copy from the FP64 matrix first). */
 int pp_solver_factor(pp_solver* s, void* hip_stream);
 /* x = A^{-1} b by the factors, refined in FP64 until the relative residual is
@@ -212,6 +226,7 @@ int pp_solver_destroy(pp_solver* s);
 /* Stream of the factorization (for tracing / timing). */
 pp_stream* pp_solver_factor_stream(pp_solver* s);
 double pp_solver_factor_flops(const pp_solver* s);
+double pp_solver_last_factor_ms(const pp_solver* s);
 double pp_solver_solve_flops(const pp_solver* s, int nrhs);
 
 #ifdef __cplusplus

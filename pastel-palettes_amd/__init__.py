@@ -100,6 +100,13 @@ def lib():
         "pp_solver_factor_stream": (vp, [vp]),
         "pp_solver_factor_flops": (dp, [vp]),
         "pp_solver_solve_flops": (dp, [vp, C.c_int]),
+        "pp_solver_create_batch": (C.c_int, [vp, P(vp), C.c_int, P(vp), C.c_int, C.c_int, C.c_int, C.c_int,
+                                             C.c_int, C.c_int, P(vp)]),
+        "pp_solver_num_members": (C.c_int, [vp]),
+        "pp_solver_rows": (i64, [vp]),
+        "pp_solver_member": (C.c_int, [vp, C.c_int, P(i64), P(C.c_int), P(dp)]),
+        "pp_solver_use_graph": (C.c_int, [vp, C.c_int]),
+        "pp_solver_last_factor_ms": (dp, [vp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -415,6 +422,47 @@ class Solver:
             check(lib().pp_solver_create_device(ctx.h, hm.h, C.c_void_p(int(storage)), precision, workers,
                                                 wavefronts, nrhs_max, rhs_split, C.byref(h)))
         self.ctx, self.hm, self.h, self.precision, self.workers = ctx, hm, h, precision, workers
+
+    @classmethod
+    def batch(cls, ctx, hms, storages, precision=FP64, workers=None, wavefronts=4, nrhs_max=4, rhs_split=1):
+        """One solver over many H-matrices (block-diagonal batch): one launch
+        factors all, one launch solves all.  storages: host arrays or device
+        pointers (ints)."""
+        self = cls.__new__(cls)
+        if workers is None:
+            workers = min(ctx.cu_count, 256)
+        on_dev = not isinstance(storages[0], np.ndarray)
+        if on_dev:
+            keep = None
+            arr = (C.c_void_p * len(storages))(*[int(p) for p in storages])
+        else:
+            keep = [np.ascontiguousarray(a, np.float64) for a in storages]
+            arr = (C.c_void_p * len(keep))(*[a.ctypes.data for a in keep])
+        hs = (C.c_void_p * len(hms))(*[m.h for m in hms])
+        h = C.c_void_p()
+        check(lib().pp_solver_create_batch(ctx.h, hs, len(hms), arr, 1 if on_dev else 0, precision, workers,
+                                           wavefronts, nrhs_max, rhs_split, C.byref(h)))
+        del keep
+        self.ctx, self.hm, self.h, self.precision, self.workers = ctx, list(hms), h, precision, workers
+        return self
+
+    @property
+    def rows(self):
+        return int(lib().pp_solver_rows(self.h))
+
+    def members(self):
+        out = []
+        ro, n, r = C.c_int64(), C.c_int(), C.c_double()
+        for i in range(check(lib().pp_solver_num_members(self.h))):
+            check(lib().pp_solver_member(self.h, i, C.byref(ro), C.byref(n), C.byref(r)))
+            out.append(dict(row_off=ro.value, n=n.value, residual=r.value))
+        return out
+
+    def use_graph(self, on=True):
+        check(lib().pp_solver_use_graph(self.h, 1 if on else 0))
+
+    def last_factor_ms(self):
+        return float(lib().pp_solver_last_factor_ms(self.h))
 
     def factor(self, hip_stream=None):
         check(lib().pp_solver_factor(self.h, hip_stream))

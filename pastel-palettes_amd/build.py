@@ -25,6 +25,7 @@ HOST_SOURCES = [
     "instructions/instructions_manager.cpp",
     "capi_host.cpp",
     "runtime/runtime.cpp",
+    "runtime/solver.cpp",
 ]
 HIP_SOURCES = ["kernels/vm.hip"]
 

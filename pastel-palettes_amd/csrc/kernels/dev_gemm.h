@@ -44,6 +44,8 @@ template <>
 struct Mma<double, MODE_FP64> {
   using acc_t = d4_t;
   static constexpr int KSTEP = 4, KPL = 1;
+  // accumulator r of lane l holds row (l / 16) + 4 r, column l % 16 of the tile
+  __device__ static __forceinline__ int orow(int l, int r) { return (l >> 4) + 4 * r; }
   __device__ static __forceinline__ acc_t zero() { return acc_t{0., 0., 0., 0.}; }
   __device__ static __forceinline__ acc_t mma(const double* a, const double* b, acc_t c) {
     return __builtin_amdgcn_mfma_f64_16x16x4f64(a[0], b[0], c, 0, 0, 0);
@@ -54,6 +56,8 @@ template <>
 struct Mma<float, MODE_FP32> {
   using acc_t = f4_t;
   static constexpr int KSTEP = 4, KPL = 1;
+  // accumulator r of lane l holds row 4 (l / 16) + r, column l % 16 of the tile
+  __device__ static __forceinline__ int orow(int l, int r) { return 4 * (l >> 4) + r; }
   __device__ static __forceinline__ acc_t zero() { return acc_t{0.f, 0.f, 0.f, 0.f}; }
   __device__ static __forceinline__ acc_t mma(const float* a, const float* b, acc_t c) {
     return __builtin_amdgcn_mfma_f32_16x16x4f32(a[0], b[0], c, 0, 0, 0);
@@ -64,6 +68,7 @@ template <>
 struct Mma<float, MODE_BF16> {
   using acc_t = f4_t;
   static constexpr int KSTEP = 32, KPL = 8;
+  __device__ static __forceinline__ int orow(int l, int r) { return 4 * (l >> 4) + r; }
   __device__ static __forceinline__ acc_t zero() { return acc_t{0.f, 0.f, 0.f, 0.f}; }
   __device__ static __forceinline__ acc_t mma(const float* a, const float* b, acc_t c) {
     s8_t av, bv;
@@ -164,7 +169,7 @@ __device__ void gemm_block(int m, int n, int k, T alpha, T beta, T* C, int ldc, 
     for (int idx = threadIdx.x; idx < 64 * 16; idx += blockDim.x) {
       const int ln = idx >> 4, q = idx & 15;
       const int i = q >> 3, j = (q >> 2) & 1, r = q & 3;
-      const int row = i * 16 + (ln >> 4) * 4 + r, col = j * 16 + (ln & 15);
+      const int row = i * 16 + P::orow(ln, r), col = j * 16 + (ln & 15);
       if (row < m && col < n) {
         T s = red[(size_t)ln * 16 + q];
         for (int sl = 1; sl < slots; ++sl) s += red[((size_t)sl * 64 + ln) * 16 + q];
@@ -197,7 +202,7 @@ __device__ void gemm_block(int m, int n, int k, T alpha, T beta, T* C, int ldc, 
       for (int j = 0; j < 2; ++j)
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-          const int row = r0 + i * 16 + (l >> 4) * 4 + r, col = c0 + j * 16 + (l & 15);
+          const int row = r0 + i * 16 + P::orow(l, r), col = c0 + j * 16 + (l & 15);
           if (row < m && col < n) {
             T* c = &C[(int64_t)row * ldc + col];
             *c = (beta == T(0)) ? alpha * acc[i][j][r] : beta * *c + alpha * acc[i][j][r];

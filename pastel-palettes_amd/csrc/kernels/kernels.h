@@ -41,6 +41,7 @@ hipError_t launch_add_correction(double* x, int ldx, const void* d, int d_bytes,
                                  hipStream_t s);
 hipError_t launch_hmatvec(const MatvecPlanDev& p, const double* A, const double* x, int ldx, const double* y, int ldy,
                           double* out, int ldo, int nrhs, double* t, hipStream_t s);
+hipError_t launch_segnorm2(const double* r, int ldr, const int* seg, int nseg, int nrhs, double* out, hipStream_t s);
 hipError_t launch_colnorm2(const double* r, int ldr, int n, int nrhs, double* out, hipStream_t s);
 hipError_t launch_mfma_probe(int mode, const double* a, const double* b, double* out, hipStream_t s);
 hipError_t launch_single_op(int precision, const int32_t* words, void* const* ptrs, void* scratch, const void* rnd,

@@ -363,6 +363,26 @@ __global__ __launch_bounds__(256) void colnorm2_kernel(const double* r, int ldr,
   if (threadIdx.x == 0) out[c] = red[0];
 }
 
+// per (segment, column) sums of squares; seg holds (row0, rows) pairs
+__global__ __launch_bounds__(256) void segnorm2_kernel(const double* r, int ldr, const int* seg, int nrhs,
+                                                        double* out) {
+  __shared__ double red[256];
+  const int g = blockIdx.x / nrhs, c = blockIdx.x % nrhs;
+  const int row0 = seg[2 * g], rows = seg[2 * g + 1];
+  double s = 0.;
+  for (int i = threadIdx.x; i < rows; i += blockDim.x) {
+    const double v = r[(int64_t)(row0 + i) * ldr + c];
+    s += v * v;
+  }
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int h = blockDim.x / 2; h > 0; h >>= 1) {
+    if ((int)threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[blockIdx.x] = red[0];
+}
+
 static int grid_for(int64_t n) {
   int64_t g = (n + 255) / 256;
   return (int)(g < 1 ? 1 : (g > 4096 ? 4096 : g));
@@ -385,10 +405,12 @@ hipError_t launch_convert(void* dst, int dst_bytes, const void* src, int src_byt
   else if (dst_bytes == 8 && src_bytes == 4)
     hipLaunchKernelGGL((convert_kernel<double, float>), dim3(grid_for(n)), dim3(256), 0, s, (double*)dst,
                        (const float*)src, n);
-  else if (dst_bytes == 8 && src_bytes == 8)
-    return hipMemcpyAsync(dst, src, n * 8, hipMemcpyDeviceToDevice, s);
+  else if (dst_bytes == 8 && src_bytes == 8)  // kernels, not memcpy nodes, so graphs capture them plainly
+    hipLaunchKernelGGL((convert_kernel<double, double>), dim3(grid_for(n)), dim3(256), 0, s, (double*)dst,
+                       (const double*)src, n);
   else
-    return hipMemcpyAsync(dst, src, n * 4, hipMemcpyDeviceToDevice, s);
+    hipLaunchKernelGGL((convert_kernel<float, float>), dim3(grid_for(n)), dim3(256), 0, s, (float*)dst,
+                       (const float*)src, n);
   return hipGetLastError();
 }
 
@@ -428,6 +450,11 @@ hipError_t launch_hmatvec(const MatvecPlanDev& p, const double* A, const double*
                        t);
   hipLaunchKernelGGL(hmv_rows_kernel, dim3(p.n_tiles), dim3(256), 0, s, p.blocks, p.tiles, p.contrib, A, x, ldx, t, y,
                      ldy, out, ldo, nrhs);
+  return hipGetLastError();
+}
+
+hipError_t launch_segnorm2(const double* r, int ldr, const int* seg, int nseg, int nrhs, double* out, hipStream_t s) {
+  hipLaunchKernelGGL(segnorm2_kernel, dim3(nseg * nrhs), dim3(256), 0, s, r, ldr, seg, nrhs, out);
   return hipGetLastError();
 }
 

@@ -1,95 +1,18 @@
-// C ABI, device half: context, uploaded streams, VM launches, and the
-// factor + solve + FP64 refinement driver.
-#include <hip/hip_runtime.h>
+// C ABI, device half: context, uploaded streams and VM launches.
+#include "runtime_internal.h"
 
-#include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <cstring>
 #include <map>
 #include <memory>
-#include <mutex>
-#include <string>
 #include <thread>
-#include <unordered_map>
-#include <vector>
 
-#include "../../../include/pastel.h"
-#include "../capi_common.h"
-#include "../kernels/kernels.h"
-#include "../matrix/Hierarchical.h"
-
+using namespace ppr;
 using ppc::from_exception;
 using ppc::set_error;
 
-namespace {
-
-int hip_fail(hipError_t e, const char* where) {
-  return set_error(PP_ERR_HIP, std::string(where) + ": " + hipGetErrorString(e));
-}
-
-#define HIP_TRY(expr)                                 \
-  do {                                                \
-    hipError_t _e = (expr);                           \
-    if (_e != hipSuccess) return hip_fail(_e, #expr); \
-  } while (0)
-
-// splitmix64 stream -> uniform [-1, 1); the sketch matrix of accum.
-uint64_t splitmix64(uint64_t& state) {
-  uint64_t z = (state += 0x9E3779B97F4A7C15ULL);
-  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
-  z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
-  return z ^ (z >> 31);
-}
-
-void sketch_values(std::vector<double>& v, int64_t count, uint64_t seed) {
-  v.resize(count);
-  uint64_t st = seed;
-  for (int64_t i = 0; i < count; ++i) v[i] = 2.0 * ((double)(splitmix64(st) >> 11) * 0x1.0p-53) - 1.0;
-}
-
-constexpr uint64_t kSketchSeed = 0x5EEDULL;
-
-int elem_bytes(int precision) { return precision == PP_FP64 ? 8 : 4; }
-
-}  // namespace
-
-struct pp_ctx {
-  int device = 0;
-  uint64_t budget = 0;
-  uint64_t in_use = 0;
-  int cus = 0;
-  hipStream_t stream = nullptr;
-  std::unordered_map<void*, uint64_t> allocs;
-  std::mutex mu;
-};
-
-struct pp_stream {
-  pp_ctx* ctx = nullptr;
-  int workers = 0, n_insts = 0, precision = 0, n_ptrs = 0;
-  int32_t* d_words = nullptr;
-  int64_t* d_off = nullptr;
-  void** d_ptrs = nullptr;
-  uint32_t* d_flags = nullptr;
-  uint32_t* d_epoch = nullptr;
-  int32_t* d_status = nullptr;
-  void* d_scratch = nullptr;
-  int64_t scratch_stride = 0;
-  void* d_rnd = nullptr;
-  uint64_t* d_trace = nullptr;
-  int64_t* d_trace_off = nullptr;
-  int64_t trace_total = 0;
-  bool trace_on = false;
-  std::vector<void*> ptrs;
-  std::vector<std::pair<int, int64_t>> zero_slots;  // slot, elements
-  std::vector<void*> owned;
-  hipEvent_t ev0 = nullptr, ev1 = nullptr;
-  int64_t spin_limit = 20LL * 100000000LL;  // 20 s of 100 MHz ticks
-  hipStream_t last = nullptr;
-  bool launched = false;
-};
-
-namespace {
+namespace ppr {
 
 int ctx_alloc(pp_ctx* c, uint64_t bytes, void** p) {
   std::lock_guard<std::mutex> lk(c->mu);
@@ -117,15 +40,43 @@ int ctx_free(pp_ctx* c, void* p) {
   return PP_OK;
 }
 
-template <typename T>
-int ctx_alloc_t(pp_ctx* c, int64_t count, T** p) {
-  void* v = nullptr;
-  int rc = ctx_alloc(c, (uint64_t)std::max<int64_t>(count, 1) * sizeof(T), &v);
-  *p = static_cast<T*>(v);
-  return rc;
+int enqueue_launch(pp_ctx* ctx, pp_stream* s, int wavefronts_per_inst, hipStream_t st, bool record_events) {
+  const int threads = 64 * wavefronts_per_inst;
+  if (threads != 256 && threads != 512) return set_error(PP_ERR_INVALID, "wavefronts_per_inst must be 4 or 8");
+  const int per_cu = pp::vm_blocks_per_cu(s->precision, threads);
+  if (per_cu < 1 || s->workers > per_cu * ctx->cus)
+    return set_error(PP_ERR_INVALID, "workers (" + std::to_string(s->workers) + ") exceed co-resident capacity " +
+                                         std::to_string(per_cu * ctx->cus));
+  for (const auto& z : s->zero_slots) {
+    hipError_t e = pp::launch_fill_zero(s->ptrs[z.first], z.second, elem_bytes(s->precision), st);
+    if (e != hipSuccess) return hip_fail(e, "fill_zero");
+  }
+  pp::VmArgs a{};
+  a.words = s->d_words;
+  a.stream_off = s->d_off;
+  a.ptrs = s->d_ptrs;
+  a.flags = s->d_flags;
+  a.scratch = s->d_scratch;
+  a.scratch_stride = s->scratch_stride;
+  a.rnd = s->d_rnd;
+  a.trace = s->trace_on ? s->d_trace : nullptr;
+  a.trace_off = s->d_trace_off;
+  a.status = s->d_status;
+  a.spin_limit = s->spin_limit;
+  a.epoch = s->d_epoch;
+  a.n_insts = s->n_insts;
+  if (record_events) HIP_TRY(hipEventRecord(s->ev0, st));
+  hipError_t e = pp::launch_vm(a, s->precision, s->workers, threads, s->d_epoch, s->d_status, st);
+  if (e != hipSuccess) return hip_fail(e, "launch_vm");
+  if (record_events) HIP_TRY(hipEventRecord(s->ev1, st));
+  s->last = st;
+  s->launched = true;
+  return PP_OK;
 }
 
-hipStream_t pick(pp_ctx* c, void* s) { return s ? static_cast<hipStream_t>(s) : c->stream; }
+}  // namespace ppr
+
+namespace {
 
 int upload_sketch(pp_stream* s, int64_t count, uint64_t seed) {
   std::vector<double> v;
@@ -391,38 +342,7 @@ int pp_max_workers(pp_ctx* ctx, int precision, int wavefronts_per_inst) {
 int pp_launch(pp_ctx* ctx, pp_stream* s, int wavefronts_per_inst, void* hip_stream) {
   try {
     if (ctx == nullptr || s == nullptr) return set_error(PP_ERR_INVALID, "null");
-    const int threads = 64 * wavefronts_per_inst;
-    if (threads != 256 && threads != 512) return set_error(PP_ERR_INVALID, "wavefronts_per_inst must be 4 or 8");
-    const int per_cu = pp::vm_blocks_per_cu(s->precision, threads);
-    if (per_cu < 1 || s->workers > per_cu * ctx->cus)
-      return set_error(PP_ERR_INVALID, "workers (" + std::to_string(s->workers) + ") exceed co-resident capacity " +
-                                           std::to_string(per_cu * ctx->cus));
-    hipStream_t st = pick(ctx, hip_stream);
-    for (const auto& z : s->zero_slots) {
-      hipError_t e = pp::launch_fill_zero(s->ptrs[z.first], z.second, elem_bytes(s->precision), st);
-      if (e != hipSuccess) return hip_fail(e, "fill_zero");
-    }
-    pp::VmArgs a{};
-    a.words = s->d_words;
-    a.stream_off = s->d_off;
-    a.ptrs = s->d_ptrs;
-    a.flags = s->d_flags;
-    a.scratch = s->d_scratch;
-    a.scratch_stride = s->scratch_stride;
-    a.rnd = s->d_rnd;
-    a.trace = s->trace_on ? s->d_trace : nullptr;
-    a.trace_off = s->d_trace_off;
-    a.status = s->d_status;
-    a.spin_limit = s->spin_limit;
-    a.epoch = s->d_epoch;
-    a.n_insts = s->n_insts;
-    HIP_TRY(hipEventRecord(s->ev0, st));
-    hipError_t e = pp::launch_vm(a, s->precision, s->workers, threads, s->d_epoch, s->d_status, st);
-    if (e != hipSuccess) return hip_fail(e, "launch_vm");
-    HIP_TRY(hipEventRecord(s->ev1, st));
-    s->last = st;
-    s->launched = true;
-    return PP_OK;
+    return enqueue_launch(ctx, s, wavefronts_per_inst, pick(ctx, hip_stream), true);
   } catch (...) {
     return from_exception();
   }
@@ -574,349 +494,3 @@ int pp_mfma_probe(pp_ctx* ctx, int mode, const double* a, int na, const double* 
 
 }  // extern "C"
 
-// ------------------------------------------------------------------ solver
-
-struct pp_solver {
-  pp_ctx* ctx = nullptr;
-  const pp_hmatrix* hm = nullptr;
-  const Hierarchical* h = nullptr;
-  int precision = 0, workers = 0, wpi = 4, nrhs_max = 0, rhs_split = 1, n = 0;
-  int64_t storage = 0;
-  double* d_A = nullptr;  // FP64 matrix (residual)
-  void* d_F = nullptr;    // factor arena (storage precision)
-  pp_program* fprog = nullptr;
-  pp_stream* fstream = nullptr;
-  struct SolveProg {
-    pp_program* prog = nullptr;
-    pp_stream* stream = nullptr;
-  };
-  std::map<int, SolveProg> sprogs;
-  void* d_rhs = nullptr;   // n x nrhs_max, storage precision
-  double* d_r = nullptr;   // n x nrhs_max residual
-  double* d_t = nullptr;   // matvec low-rank products
-  double* d_nrm = nullptr; // 2 * nrhs_max
-  // matvec plan
-  pp::MatvecBlock* d_blocks = nullptr;
-  pp::MatvecTile* d_tiles = nullptr;
-  int* d_contrib = nullptr;
-  int* d_lr = nullptr;
-  pp::MatvecPlanDev plan{};
-  double factor_flops = 0;
-  double last_factor_ms = 0;
-  std::vector<void*> owned;
-};
-
-namespace {
-
-const Hierarchical* hier_of(const pp_hmatrix* h) { return h->h.get(); }
-
-int build_plan(pp_solver* s) {
-  std::vector<const h_block*> lv;
-  s->h->leaves(lv);
-  std::vector<pp::MatvecBlock> blocks;
-  std::vector<int> lr;
-  int64_t t_off = 0;
-  for (const h_block* b : lv) {
-    pp::MatvecBlock m{};
-    m.type = (int)b->type;
-    m.row0 = b->row0;
-    m.col0 = b->col0;
-    m.ny = b->ny;
-    m.nx = b->nx;
-    m.rank = b->rank;
-    m.off = b->off;
-    m.off_u = b->off_u;
-    m.off_v = b->off_v;
-    if (b->type == element_t::low_rank) {
-      m.t_off = t_off;
-      t_off += (int64_t)b->rank * s->nrhs_max;
-      lr.push_back((int)blocks.size());
-    }
-    blocks.push_back(m);
-  }
-  // row tiles: row ranges of the diagonal leaves
-  std::vector<std::pair<int, int>> ranges;
-  for (const h_block* b : lv)
-    if (b->row0 == b->col0 && b->type == element_t::dense) ranges.emplace_back(b->row0, b->ny);
-  std::sort(ranges.begin(), ranges.end());
-  std::vector<pp::MatvecTile> tiles;
-  std::vector<int> contrib;
-  for (const auto& r : ranges) {
-    pp::MatvecTile t{};
-    t.row0 = r.first;
-    t.rows = r.second;
-    t.first = (int)(contrib.size() / 2);
-    for (size_t i = 0; i < blocks.size(); ++i) {
-      const auto& b = blocks[i];
-      if (b.row0 <= r.first && r.first + r.second <= b.row0 + b.ny) {
-        contrib.push_back((int)i);
-        contrib.push_back(0);
-      }
-    }
-    t.count = (int)(contrib.size() / 2) - t.first;
-    tiles.push_back(t);
-  }
-  int64_t covered = 0;
-  for (const auto& t : tiles) covered += t.rows;
-  if (covered != s->n) return set_error(PP_ERR_INVALID, "matvec plan: diagonal leaves do not tile the rows");
-  int rc;
-  if ((rc = ctx_alloc_t(s->ctx, blocks.size(), &s->d_blocks))) return rc;
-  s->owned.push_back(s->d_blocks);
-  if ((rc = ctx_alloc_t(s->ctx, tiles.size(), &s->d_tiles))) return rc;
-  s->owned.push_back(s->d_tiles);
-  if ((rc = ctx_alloc_t(s->ctx, std::max<size_t>(contrib.size(), 2), &s->d_contrib))) return rc;
-  s->owned.push_back(s->d_contrib);
-  if ((rc = ctx_alloc_t(s->ctx, std::max<size_t>(lr.size(), 1), &s->d_lr))) return rc;
-  s->owned.push_back(s->d_lr);
-  if ((rc = ctx_alloc_t(s->ctx, std::max<int64_t>(t_off, 1), &s->d_t))) return rc;
-  s->owned.push_back(s->d_t);
-  HIP_TRY(hipMemcpy(s->d_blocks, blocks.data(), sizeof(pp::MatvecBlock) * blocks.size(), hipMemcpyHostToDevice));
-  HIP_TRY(hipMemcpy(s->d_tiles, tiles.data(), sizeof(pp::MatvecTile) * tiles.size(), hipMemcpyHostToDevice));
-  if (!contrib.empty())
-    HIP_TRY(hipMemcpy(s->d_contrib, contrib.data(), sizeof(int) * contrib.size(), hipMemcpyHostToDevice));
-  if (!lr.empty()) HIP_TRY(hipMemcpy(s->d_lr, lr.data(), sizeof(int) * lr.size(), hipMemcpyHostToDevice));
-  s->plan.blocks = s->d_blocks;
-  s->plan.tiles = s->d_tiles;
-  s->plan.contrib = s->d_contrib;
-  s->plan.lr_list = s->d_lr;
-  s->plan.n_tiles = (int)tiles.size();
-  s->plan.n_lowrank = (int)lr.size();
-  return PP_OK;
-}
-
-void destroy_solver(pp_solver* s) {
-  if (s == nullptr) return;
-  if (s->fstream) pp_stream_destroy(s->ctx, s->fstream);
-  if (s->fprog) pp_program_destroy(s->fprog);
-  for (auto& kv : s->sprogs) {
-    if (kv.second.stream) pp_stream_destroy(s->ctx, kv.second.stream);
-    if (kv.second.prog) pp_program_destroy(kv.second.prog);
-  }
-  for (void* p : s->owned) (void)ctx_free(s->ctx, p);
-  delete s;
-}
-
-int solver_create(pp_ctx* ctx, const pp_hmatrix* hm, const double* storage, bool on_device, int precision, int workers,
-                  int wpi, int nrhs_max, int rhs_split, pp_solver** out) {
-  if (ctx == nullptr || hm == nullptr || storage == nullptr || out == nullptr || precision < 0 || precision > 2 ||
-      workers < 1 || nrhs_max < 1 || rhs_split < 1)
-    return set_error(PP_ERR_INVALID, "bad solver args");
-  auto s = new pp_solver();
-  s->ctx = ctx;
-  s->hm = hm;
-  s->h = hier_of(hm);
-  s->precision = precision;
-  s->workers = workers;
-  s->wpi = wpi;
-  s->nrhs_max = nrhs_max;
-  s->rhs_split = rhs_split;
-  s->n = s->h->getNx_abs();
-  s->storage = s->h->getStorage();
-  int rc = PP_OK;
-  const int eb = elem_bytes(precision);
-  do {
-    if ((rc = ctx_alloc_t(ctx, s->storage, &s->d_A))) break;
-    s->owned.push_back(s->d_A);
-    {
-      hipError_t e = hipMemcpy(s->d_A, storage, 8 * s->storage, on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice);
-      if (e != hipSuccess) {
-        rc = hip_fail(e, "upload matrix");
-        break;
-      }
-    }
-    if (precision == PP_FP64) {
-      if ((rc = ctx_alloc_t(ctx, s->storage, reinterpret_cast<double**>(&s->d_F)))) break;
-    } else {
-      if ((rc = ctx_alloc_t(ctx, s->storage, reinterpret_cast<float**>(&s->d_F)))) break;
-    }
-    s->owned.push_back(s->d_F);
-    if ((rc = ctx_alloc(ctx, (uint64_t)s->n * nrhs_max * eb, &s->d_rhs))) break;
-    s->owned.push_back(s->d_rhs);
-    if ((rc = ctx_alloc_t(ctx, (int64_t)s->n * nrhs_max, &s->d_r))) break;
-    s->owned.push_back(s->d_r);
-    if ((rc = ctx_alloc_t(ctx, 2 * nrhs_max, &s->d_nrm))) break;
-    s->owned.push_back(s->d_nrm);
-    if ((rc = build_plan(s))) break;
-    if ((rc = pp_program_factor(hm, workers, &s->fprog))) break;
-    void* ptrs[3] = {s->d_F, nullptr, s->d_rhs};
-    if ((rc = pp_stream_upload_program(ctx, s->fprog, ptrs, 3, precision, &s->fstream))) break;
-    s->factor_flops = (double)s->fprog->dag->getFlops();
-  } while (false);
-  if (rc != PP_OK) {
-    destroy_solver(s);
-    return rc;
-  }
-  *out = s;
-  return PP_OK;
-}
-
-int solve_prog(pp_solver* s, int nrhs, pp_solver::SolveProg** out) {
-  auto it = s->sprogs.find(nrhs);
-  if (it == s->sprogs.end()) {
-    pp_solver::SolveProg sp;
-    int rc = pp_program_solve(s->hm, nrhs, nrhs, s->rhs_split, s->workers, &sp.prog);
-    if (rc) return rc;
-    void* ptrs[3] = {s->d_F, nullptr, s->d_rhs};
-    rc = pp_stream_upload_program(s->ctx, sp.prog, ptrs, 3, s->precision, &sp.stream);
-    if (rc) {
-      pp_program_destroy(sp.prog);
-      return rc;
-    }
-    it = s->sprogs.emplace(nrhs, sp).first;
-  }
-  *out = &it->second;
-  return PP_OK;
-}
-
-// d_rhs <- src (n x nrhs, pitch ld, FP64), solve in place with the factors
-int apply_inverse(pp_solver* s, const double* src, int ld, int nrhs, hipStream_t st) {
-  pp_solver::SolveProg* sp = nullptr;
-  int rc = solve_prog(s, nrhs, &sp);
-  if (rc) return rc;
-  const int eb = elem_bytes(s->precision);
-  hipError_t e = pp::launch_copy2d(s->d_rhs, eb, nrhs, src, 8, ld, s->n, nrhs, st);
-  if (e != hipSuccess) return hip_fail(e, "copy rhs");
-  rc = pp_launch(s->ctx, sp->stream, s->wpi, st);
-  if (rc) return rc;
-  return PP_OK;
-}
-
-int wait_stream(pp_solver* s, pp_stream* st_) {
-  int32_t first = -1;
-  return pp_wait(s->ctx, st_, 0, &first);
-}
-
-}  // namespace
-
-extern "C" {
-
-int pp_solver_create(pp_ctx* ctx, const pp_hmatrix* h, const double* storage, int precision, int workers,
-                     int wavefronts_per_inst, int nrhs_max, int rhs_split, pp_solver** out) {
-  try {
-    return solver_create(ctx, h, storage, false, precision, workers, wavefronts_per_inst, nrhs_max, rhs_split, out);
-  } catch (...) {
-    return from_exception();
-  }
-}
-
-int pp_solver_create_device(pp_ctx* ctx, const pp_hmatrix* h, const double* storage_dev, int precision, int workers,
-                            int wavefronts_per_inst, int nrhs_max, int rhs_split, pp_solver** out) {
-  try {
-    return solver_create(ctx, h, storage_dev, true, precision, workers, wavefronts_per_inst, nrhs_max, rhs_split, out);
-  } catch (...) {
-    return from_exception();
-  }
-}
-
-int pp_solver_factor(pp_solver* s, void* hip_stream) {
-  try {
-    if (s == nullptr) return set_error(PP_ERR_INVALID, "null solver");
-    hipStream_t st = pick(s->ctx, hip_stream);
-    hipError_t e = pp::launch_convert(s->d_F, elem_bytes(s->precision), s->d_A, 8, s->storage, st);
-    if (e != hipSuccess) return hip_fail(e, "restore factor arena");
-    int rc = pp_launch(s->ctx, s->fstream, s->wpi, st);
-    if (rc) return rc;
-    rc = wait_stream(s, s->fstream);
-    if (rc) return rc;
-    s->last_factor_ms = pp_stream_last_ms(s->fstream);
-    return PP_OK;
-  } catch (...) {
-    return from_exception();
-  }
-}
-
-int pp_solver_residual(pp_solver* s, const double* b, const double* x, double* r, int nrhs, int ld, void* hip_stream) {
-  if (s == nullptr || b == nullptr || x == nullptr || r == nullptr || nrhs < 1 || nrhs > s->nrhs_max || ld < nrhs)
-    return set_error(PP_ERR_INVALID, "bad residual args");
-  hipStream_t st = pick(s->ctx, hip_stream);
-  hipError_t e = pp::launch_hmatvec(s->plan, s->d_A, x, ld, b, ld, r, ld, nrhs, s->d_t, st);
-  if (e != hipSuccess) return hip_fail(e, "hmatvec");
-  HIP_TRY(hipStreamSynchronize(st));
-  return PP_OK;
-}
-
-int pp_solver_solve(pp_solver* s, const double* b, double* x, int nrhs, int ld, double tol, int max_iter,
-                    void* hip_stream, pp_solve_report* rep) {
-  try {
-    if (s == nullptr || b == nullptr || x == nullptr || nrhs < 1 || nrhs > s->nrhs_max || ld < nrhs || max_iter < 0)
-      return set_error(PP_ERR_INVALID, "bad solve args");
-    hipStream_t st = pick(s->ctx, hip_stream);
-    const auto t0 = std::chrono::steady_clock::now();
-    const int eb = elem_bytes(s->precision);
-    // ||b|| per column
-    std::vector<double> bn(nrhs), rn(nrhs);
-    hipError_t e = pp::launch_colnorm2(b, ld, s->n, nrhs, s->d_nrm + s->nrhs_max, st);
-    if (e != hipSuccess) return hip_fail(e, "norm b");
-    // x = F^{-1} b
-    int rc = apply_inverse(s, b, ld, nrhs, st);
-    if (rc) return rc;
-    e = pp::launch_copy2d(x, 8, ld, s->d_rhs, eb, nrhs, s->n, nrhs, st);
-    if (e != hipSuccess) return hip_fail(e, "copy x");
-    pp_solver::SolveProg* sp = &s->sprogs[nrhs];
-    rc = wait_stream(s, sp->stream);
-    if (rc) return rc;
-    HIP_TRY(hipMemcpy(bn.data(), s->d_nrm + s->nrhs_max, 8 * nrhs, hipMemcpyDeviceToHost));
-    double res = 0.;
-    int it = 0;
-    bool conv = false;
-    for (;;) {
-      // r = b - A x (FP64)
-      e = pp::launch_hmatvec(s->plan, s->d_A, x, ld, b, ld, s->d_r, nrhs, nrhs, s->d_t, st);
-      if (e != hipSuccess) return hip_fail(e, "hmatvec");
-      e = pp::launch_colnorm2(s->d_r, nrhs, s->n, nrhs, s->d_nrm, st);
-      if (e != hipSuccess) return hip_fail(e, "norm r");
-      HIP_TRY(hipMemcpyAsync(rn.data(), s->d_nrm, 8 * nrhs, hipMemcpyDeviceToHost, st));
-      HIP_TRY(hipStreamSynchronize(st));
-      res = 0.;
-      for (int c = 0; c < nrhs; ++c) {
-        const double rel = bn[c] > 0 ? std::sqrt(rn[c] / bn[c]) : std::sqrt(rn[c]);
-        res = std::max(res, rel);
-      }
-      if (res <= tol) {
-        conv = true;
-        break;
-      }
-      if (it >= max_iter) break;
-      ++it;
-      rc = apply_inverse(s, s->d_r, nrhs, nrhs, st);
-      if (rc) return rc;
-      e = pp::launch_add_correction(x, ld, s->d_rhs, eb, nrhs, s->n, nrhs, st);
-      if (e != hipSuccess) return hip_fail(e, "update x");
-      rc = wait_stream(s, sp->stream);
-      if (rc) return rc;
-    }
-    if (rep) {
-      rep->iterations = it;
-      rep->converged = conv ? 1 : 0;
-      rep->residual = res;
-      rep->factor_ms = s->last_factor_ms;
-      rep->total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
-      rep->solve_ms = rep->total_ms;
-    }
-    return PP_OK;
-  } catch (...) {
-    return from_exception();
-  }
-}
-
-int pp_solver_factor_arena(pp_solver* s, void** dptr, int* eb) {
-  if (s == nullptr || dptr == nullptr || eb == nullptr) return set_error(PP_ERR_INVALID, "null");
-  *dptr = s->d_F;
-  *eb = elem_bytes(s->precision);
-  return PP_OK;
-}
-
-int pp_solver_destroy(pp_solver* s) {
-  destroy_solver(s);
-  return PP_OK;
-}
-
-pp_stream* pp_solver_factor_stream(pp_solver* s) { return s ? s->fstream : nullptr; }
-double pp_solver_factor_flops(const pp_solver* s) { return s ? s->factor_flops : 0.; }
-
-double pp_solver_solve_flops(const pp_solver* s, int nrhs) {
-  if (s == nullptr) return 0.;
-  return 2.0 * (double)s->h->storedEntries() * nrhs;
-}
-
-}  // extern "C"

@@ -1,0 +1,477 @@
+// Factor + solve + FP64 iterative refinement over one or many H-matrices.
+//
+// A batch of H-matrices is handled as one block-diagonal system: their FP64
+// arenas are concatenated (member i at arena_off[i]), their rows are stacked
+// (member i at row_off[i]), one program factors all of them in a single
+// persistent launch and one program solves all of them.  Refinement runs on
+// the stacked system with per-member, per-column convergence checks.
+//
+// Precision: the factor arena holds the storage type of the precision mode
+// (FP64, or FP32 for FP32 / mixed); the FP64 arena stays resident for the
+// residual r = b - A_H x, which is always computed in FP64.
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../matrix/Hierarchical.h"
+#include "runtime_internal.h"
+
+using namespace ppr;
+using ppc::from_exception;
+using ppc::set_error;
+
+struct pp_solver {
+  pp_ctx* ctx = nullptr;
+  std::vector<const pp_hmatrix*> hms;
+  std::vector<int64_t> arena_off;  // elements
+  std::vector<int64_t> row_off;
+  int64_t n_total = 0, storage_total = 0;
+  int precision = 0, workers = 0, wpi = 4, nrhs_max = 0, rhs_split = 1;
+  double* d_A = nullptr;  // FP64 arenas (residual)
+  void* d_F = nullptr;    // factor arenas (storage precision)
+  pp_program* fprog = nullptr;
+  pp_stream* fstream = nullptr;
+  struct SolveProg {
+    pp_program* prog = nullptr;
+    pp_stream* stream = nullptr;
+  };
+  std::map<int, SolveProg> sprogs;
+  void* d_rhs = nullptr;    // n_total x nrhs_max, storage precision
+  double* d_r = nullptr;    // n_total x nrhs_max residual
+  double* d_t = nullptr;    // matvec low-rank products
+  double* d_nrm = nullptr;  // [members * nrhs_max] r norms, then the same for b
+  int* d_seg = nullptr;     // member row ranges (row0, rows)
+  pp::MatvecBlock* d_blocks = nullptr;
+  pp::MatvecTile* d_tiles = nullptr;
+  int* d_contrib = nullptr;
+  int* d_lr = nullptr;
+  pp::MatvecPlanDev plan{};
+  double factor_flops = 0;
+  double last_factor_ms = 0;
+  std::vector<double> member_res;
+  bool use_graph = false;
+  hipGraphExec_t factor_graph = nullptr;
+  hipStream_t graph_stream = nullptr;
+  std::vector<void*> owned;
+};
+
+namespace {
+
+const Hierarchical* hier(const pp_hmatrix* h) { return h->h.get(); }
+
+int build_plan(pp_solver* s) {
+  std::vector<pp::MatvecBlock> blocks;
+  std::vector<int> lr;
+  std::vector<std::pair<int64_t, int>> ranges;  // global row0, rows
+  int64_t t_off = 0;
+  for (size_t m = 0; m < s->hms.size(); ++m) {
+    std::vector<const h_block*> lv;
+    hier(s->hms[m])->leaves(lv);
+    const int64_t ro = s->row_off[m], ao = s->arena_off[m];
+    for (const h_block* b : lv) {
+      pp::MatvecBlock mb{};
+      mb.type = (int)b->type;
+      mb.row0 = (int32_t)(b->row0 + ro);
+      mb.col0 = (int32_t)(b->col0 + ro);
+      mb.ny = b->ny;
+      mb.nx = b->nx;
+      mb.rank = b->rank;
+      mb.off = b->off + ao;
+      mb.off_u = b->off_u + ao;
+      mb.off_v = b->off_v + ao;
+      if (b->type == element_t::low_rank) {
+        mb.t_off = t_off;
+        t_off += (int64_t)b->rank * s->nrhs_max;
+        lr.push_back((int)blocks.size());
+      }
+      if (b->row0 == b->col0 && b->type == element_t::dense) ranges.emplace_back(b->row0 + ro, b->ny);
+      blocks.push_back(mb);
+    }
+  }
+  if (s->n_total > 0x7fffffffLL) return set_error(PP_ERR_INVALID, "stacked rows exceed int32");
+  std::sort(ranges.begin(), ranges.end());
+  // tiles: each diagonal leaf's rows with the leaves of its member covering them
+  std::vector<pp::MatvecTile> tiles;
+  std::vector<int> contrib;
+  for (const auto& r : ranges) {
+    pp::MatvecTile t{};
+    t.row0 = (int32_t)r.first;
+    t.rows = r.second;
+    t.first = (int)(contrib.size() / 2);
+    for (size_t i = 0; i < blocks.size(); ++i) {
+      const auto& b = blocks[i];
+      if (b.row0 <= r.first && r.first + r.second <= (int64_t)b.row0 + b.ny) {
+        contrib.push_back((int)i);
+        contrib.push_back(0);
+      }
+    }
+    t.count = (int)(contrib.size() / 2) - t.first;
+    tiles.push_back(t);
+  }
+  int64_t covered = 0;
+  for (const auto& t : tiles) covered += t.rows;
+  if (covered != s->n_total) return set_error(PP_ERR_INVALID, "matvec plan: diagonal leaves do not tile the rows");
+  std::vector<int> seg;
+  for (size_t m = 0; m < s->hms.size(); ++m) {
+    seg.push_back((int)s->row_off[m]);
+    seg.push_back(hier(s->hms[m])->getNx_abs());
+  }
+  int rc;
+  if ((rc = ctx_alloc_t(s->ctx, blocks.size(), &s->d_blocks))) return rc;
+  s->owned.push_back(s->d_blocks);
+  if ((rc = ctx_alloc_t(s->ctx, tiles.size(), &s->d_tiles))) return rc;
+  s->owned.push_back(s->d_tiles);
+  if ((rc = ctx_alloc_t(s->ctx, std::max<size_t>(contrib.size(), 2), &s->d_contrib))) return rc;
+  s->owned.push_back(s->d_contrib);
+  if ((rc = ctx_alloc_t(s->ctx, std::max<size_t>(lr.size(), 1), &s->d_lr))) return rc;
+  s->owned.push_back(s->d_lr);
+  if ((rc = ctx_alloc_t(s->ctx, std::max<int64_t>(t_off, 1), &s->d_t))) return rc;
+  s->owned.push_back(s->d_t);
+  if ((rc = ctx_alloc_t(s->ctx, seg.size(), &s->d_seg))) return rc;
+  s->owned.push_back(s->d_seg);
+  HIP_TRY(hipMemcpy(s->d_blocks, blocks.data(), sizeof(pp::MatvecBlock) * blocks.size(), hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(s->d_tiles, tiles.data(), sizeof(pp::MatvecTile) * tiles.size(), hipMemcpyHostToDevice));
+  if (!contrib.empty())
+    HIP_TRY(hipMemcpy(s->d_contrib, contrib.data(), sizeof(int) * contrib.size(), hipMemcpyHostToDevice));
+  if (!lr.empty()) HIP_TRY(hipMemcpy(s->d_lr, lr.data(), sizeof(int) * lr.size(), hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(s->d_seg, seg.data(), sizeof(int) * seg.size(), hipMemcpyHostToDevice));
+  s->plan.blocks = s->d_blocks;
+  s->plan.tiles = s->d_tiles;
+  s->plan.contrib = s->d_contrib;
+  s->plan.lr_list = s->d_lr;
+  s->plan.n_tiles = (int)tiles.size();
+  s->plan.n_lowrank = (int)lr.size();
+  return PP_OK;
+}
+
+void destroy_solver(pp_solver* s) {
+  if (s == nullptr) return;
+  if (s->factor_graph) (void)hipGraphExecDestroy(s->factor_graph);
+  if (s->fstream) pp_stream_destroy(s->ctx, s->fstream);
+  if (s->fprog) pp_program_destroy(s->fprog);
+  for (auto& kv : s->sprogs) {
+    if (kv.second.stream) pp_stream_destroy(s->ctx, kv.second.stream);
+    if (kv.second.prog) pp_program_destroy(kv.second.prog);
+  }
+  for (void* p : s->owned) (void)ctx_free(s->ctx, p);
+  delete s;
+}
+
+// pointer table of a program over the members: slot 3i = factor arena of
+// member i, 3i+1 = temp (owned by the stream), 3i+2 = its rhs rows.
+std::vector<void*> slot_table(pp_solver* s, int nrhs) {
+  const int eb = elem_bytes(s->precision);
+  std::vector<void*> ptrs;
+  for (size_t m = 0; m < s->hms.size(); ++m) {
+    ptrs.push_back(static_cast<char*>(s->d_F) + s->arena_off[m] * eb);
+    ptrs.push_back(nullptr);
+    ptrs.push_back(static_cast<char*>(s->d_rhs) + s->row_off[m] * nrhs * eb);
+  }
+  return ptrs;
+}
+
+int solver_create(pp_ctx* ctx, const pp_hmatrix* const* hs, int count, const double* const* storages, bool on_device,
+                  int precision, int workers, int wpi, int nrhs_max, int rhs_split, pp_solver** out) {
+  if (ctx == nullptr || hs == nullptr || count < 1 || storages == nullptr || out == nullptr || precision < 0 ||
+      precision > 2 || workers < 1 || nrhs_max < 1 || rhs_split < 1 || (wpi != 4 && wpi != 8))
+    return set_error(PP_ERR_INVALID, "bad solver args");
+  auto s = new pp_solver();
+  s->ctx = ctx;
+  s->precision = precision;
+  s->workers = workers;
+  s->wpi = wpi;
+  s->nrhs_max = nrhs_max;
+  s->rhs_split = rhs_split;
+  for (int i = 0; i < count; ++i) {
+    if (hs[i] == nullptr || storages[i] == nullptr) {
+      destroy_solver(s);
+      return set_error(PP_ERR_INVALID, "null member");
+    }
+    s->hms.push_back(hs[i]);
+    s->arena_off.push_back(s->storage_total);
+    s->row_off.push_back(s->n_total);
+    s->storage_total += (hier(hs[i])->getStorage() + 63) / 64 * 64;
+    s->n_total += hier(hs[i])->getNx_abs();
+  }
+  int rc = PP_OK;
+  const int eb = elem_bytes(precision);
+  do {
+    if ((rc = ctx_alloc_t(ctx, s->storage_total, &s->d_A))) break;
+    s->owned.push_back(s->d_A);
+    for (int i = 0; i < count && rc == PP_OK; ++i) {
+      hipError_t e = hipMemcpy(s->d_A + s->arena_off[i], storages[i], 8 * hier(hs[i])->getStorage(),
+                               on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice);
+      if (e != hipSuccess) rc = hip_fail(e, "upload matrix");
+    }
+    if (rc) break;
+    if ((rc = ctx_alloc(ctx, (uint64_t)s->storage_total * eb, &s->d_F))) break;
+    s->owned.push_back(s->d_F);
+    if ((rc = ctx_alloc(ctx, (uint64_t)s->n_total * nrhs_max * eb, &s->d_rhs))) break;
+    s->owned.push_back(s->d_rhs);
+    if ((rc = ctx_alloc_t(ctx, s->n_total * nrhs_max, &s->d_r))) break;
+    s->owned.push_back(s->d_r);
+    if ((rc = ctx_alloc_t(ctx, 2 * (int64_t)count * nrhs_max, &s->d_nrm))) break;
+    s->owned.push_back(s->d_nrm);
+    if ((rc = build_plan(s))) break;
+    if ((rc = pp_program_factor_batch(hs, count, workers, &s->fprog))) break;
+    std::vector<void*> ptrs = slot_table(s, nrhs_max);
+    if ((rc = pp_stream_upload_program(ctx, s->fprog, const_cast<const void* const*>(ptrs.data()), (int)ptrs.size(),
+                                       precision, &s->fstream)))
+      break;
+    s->factor_flops = (double)s->fprog->dag->getFlops();
+    s->member_res.assign(count, 0.);
+  } while (false);
+  if (rc != PP_OK) {
+    destroy_solver(s);
+    return rc;
+  }
+  *out = s;
+  return PP_OK;
+}
+
+int solve_prog(pp_solver* s, int nrhs, pp_solver::SolveProg** out) {
+  auto it = s->sprogs.find(nrhs);
+  if (it == s->sprogs.end()) {
+    pp_solver::SolveProg sp;
+    std::vector<int32_t> nr(s->hms.size(), nrhs), ld(s->hms.size(), nrhs);
+    int rc = pp_program_solve_batch(s->hms.data(), (int)s->hms.size(), nr.data(), ld.data(), s->rhs_split, s->workers,
+                                    &sp.prog);
+    if (rc) return rc;
+    std::vector<void*> ptrs = slot_table(s, nrhs);
+    rc = pp_stream_upload_program(s->ctx, sp.prog, const_cast<const void* const*>(ptrs.data()), (int)ptrs.size(),
+                                  s->precision, &sp.stream);
+    if (rc) {
+      pp_program_destroy(sp.prog);
+      return rc;
+    }
+    it = s->sprogs.emplace(nrhs, sp).first;
+  }
+  *out = &it->second;
+  return PP_OK;
+}
+
+// Check the device status words of a stream after its work completed.
+int check_status(pp_stream* st) {
+  int32_t v[4];
+  HIP_TRY(hipMemcpy(v, st->d_status, sizeof(v), hipMemcpyDeviceToHost));
+  if (v[0] == 0) return PP_OK;
+  return set_error(v[0] == 1 ? PP_ERR_TIMEOUT : PP_ERR_DEVICE,
+                   "VM error " + std::to_string(v[0]) + " (worker " + std::to_string(v[1]) + ", id " +
+                       std::to_string(v[2]) + ")");
+}
+
+int enqueue_factor(pp_solver* s, hipStream_t st, bool events) {
+  hipError_t e = pp::launch_convert(s->d_F, elem_bytes(s->precision), s->d_A, 8, s->storage_total, st);
+  if (e != hipSuccess) return hip_fail(e, "restore factor arena");
+  return enqueue_launch(s->ctx, s->fstream, s->wpi, st, events);
+}
+
+}  // namespace
+
+extern "C" {
+
+int pp_solver_create(pp_ctx* ctx, const pp_hmatrix* h, const double* storage, int precision, int workers,
+                     int wavefronts_per_inst, int nrhs_max, int rhs_split, pp_solver** out) {
+  try {
+    return solver_create(ctx, &h, 1, &storage, false, precision, workers, wavefronts_per_inst, nrhs_max, rhs_split,
+                         out);
+  } catch (...) {
+    return from_exception();
+  }
+}
+
+int pp_solver_create_device(pp_ctx* ctx, const pp_hmatrix* h, const double* storage_dev, int precision, int workers,
+                            int wavefronts_per_inst, int nrhs_max, int rhs_split, pp_solver** out) {
+  try {
+    return solver_create(ctx, &h, 1, &storage_dev, true, precision, workers, wavefronts_per_inst, nrhs_max, rhs_split,
+                         out);
+  } catch (...) {
+    return from_exception();
+  }
+}
+
+int pp_solver_create_batch(pp_ctx* ctx, const pp_hmatrix* const* hs, int count, const double* const* storages,
+                           int on_device, int precision, int workers, int wavefronts_per_inst, int nrhs_max,
+                           int rhs_split, pp_solver** out) {
+  try {
+    return solver_create(ctx, hs, count, storages, on_device != 0, precision, workers, wavefronts_per_inst, nrhs_max,
+                         rhs_split, out);
+  } catch (...) {
+    return from_exception();
+  }
+}
+
+int pp_solver_num_members(const pp_solver* s) { return s ? (int)s->hms.size() : PP_ERR_INVALID; }
+
+int64_t pp_solver_rows(const pp_solver* s) { return s ? s->n_total : PP_ERR_INVALID; }
+
+int pp_solver_member(const pp_solver* s, int i, int64_t* row_off, int* n, double* last_residual) {
+  if (s == nullptr || i < 0 || i >= (int)s->hms.size()) return set_error(PP_ERR_INVALID, "bad member");
+  if (row_off) *row_off = s->row_off[i];
+  if (n) *n = hier(s->hms[i])->getNx_abs();
+  if (last_residual) *last_residual = s->member_res[i];
+  return PP_OK;
+}
+
+int pp_solver_use_graph(pp_solver* s, int enable) {
+  if (s == nullptr) return set_error(PP_ERR_INVALID, "null solver");
+  s->use_graph = enable != 0;
+  return PP_OK;
+}
+
+int pp_solver_factor(pp_solver* s, void* hip_stream) {
+  try {
+    if (s == nullptr) return set_error(PP_ERR_INVALID, "null solver");
+    hipStream_t st = pick(s->ctx, hip_stream);
+    if (!s->use_graph) {
+      int rc = enqueue_factor(s, st, true);
+      if (rc) return rc;
+      rc = pp_wait(s->ctx, s->fstream, 0, nullptr);
+      if (rc) return rc;
+      s->last_factor_ms = pp_stream_last_ms(s->fstream);
+      return PP_OK;
+    }
+    // capture once (restore + temp fills + epoch bump + VM), replay after
+    if (s->factor_graph == nullptr || s->graph_stream != st) {
+      if (s->factor_graph) {
+        HIP_TRY(hipGraphExecDestroy(s->factor_graph));
+        s->factor_graph = nullptr;
+      }
+      hipGraph_t g = nullptr;
+      HIP_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+      const int rc = enqueue_factor(s, st, false);
+      hipError_t e = hipStreamEndCapture(st, &g);
+      if (rc) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+      }
+      if (e != hipSuccess) return hip_fail(e, "hipStreamEndCapture");
+      e = hipGraphInstantiate(&s->factor_graph, g, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(g);
+      if (e != hipSuccess) return hip_fail(e, "hipGraphInstantiate");
+      s->graph_stream = st;
+    }
+    HIP_TRY(hipEventRecord(s->fstream->ev0, st));
+    HIP_TRY(hipGraphLaunch(s->factor_graph, st));
+    HIP_TRY(hipEventRecord(s->fstream->ev1, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    float ms = 0.f;
+    HIP_TRY(hipEventElapsedTime(&ms, s->fstream->ev0, s->fstream->ev1));
+    s->last_factor_ms = ms;
+    return check_status(s->fstream);
+  } catch (...) {
+    return from_exception();
+  }
+}
+
+int pp_solver_residual(pp_solver* s, const double* b, const double* x, double* r, int nrhs, int ld, void* hip_stream) {
+  if (s == nullptr || b == nullptr || x == nullptr || r == nullptr || nrhs < 1 || nrhs > s->nrhs_max || ld < nrhs)
+    return set_error(PP_ERR_INVALID, "bad residual args");
+  hipStream_t st = pick(s->ctx, hip_stream);
+  hipError_t e = pp::launch_hmatvec(s->plan, s->d_A, x, ld, b, ld, r, ld, nrhs, s->d_t, st);
+  if (e != hipSuccess) return hip_fail(e, "hmatvec");
+  HIP_TRY(hipStreamSynchronize(st));
+  return PP_OK;
+}
+
+int pp_solver_solve(pp_solver* s, const double* b, double* x, int nrhs, int ld, double tol, int max_iter,
+                    void* hip_stream, pp_solve_report* rep) {
+  try {
+    if (s == nullptr || b == nullptr || x == nullptr || nrhs < 1 || nrhs > s->nrhs_max || ld < nrhs || max_iter < 0)
+      return set_error(PP_ERR_INVALID, "bad solve args");
+    hipStream_t st = pick(s->ctx, hip_stream);
+    const auto t0 = std::chrono::steady_clock::now();
+    const int eb = elem_bytes(s->precision);
+    const int M = (int)s->hms.size();
+    pp_solver::SolveProg* sp = nullptr;
+    int rc = solve_prog(s, nrhs, &sp);
+    if (rc) return rc;
+    std::vector<double> bn(M * nrhs), rn(M * nrhs);
+    double* d_bn = s->d_nrm + (int64_t)M * s->nrhs_max;
+    hipError_t e = pp::launch_segnorm2(b, ld, s->d_seg, M, nrhs, d_bn, st);
+    if (e != hipSuccess) return hip_fail(e, "norm b");
+    // x = F^{-1} b
+    e = pp::launch_copy2d(s->d_rhs, eb, nrhs, b, 8, ld, (int)s->n_total, nrhs, st);
+    if (e != hipSuccess) return hip_fail(e, "copy rhs");
+    if ((rc = enqueue_launch(s->ctx, sp->stream, s->wpi, st, true))) return rc;
+    e = pp::launch_copy2d(x, 8, ld, s->d_rhs, eb, nrhs, (int)s->n_total, nrhs, st);
+    if (e != hipSuccess) return hip_fail(e, "copy x");
+    HIP_TRY(hipMemcpyAsync(bn.data(), d_bn, 8 * bn.size(), hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    if ((rc = check_status(sp->stream))) return rc;
+    double res = 0.;
+    int it = 0;
+    bool conv = false;
+    for (;;) {
+      // r = b - A x (FP64)
+      e = pp::launch_hmatvec(s->plan, s->d_A, x, ld, b, ld, s->d_r, nrhs, nrhs, s->d_t, st);
+      if (e != hipSuccess) return hip_fail(e, "hmatvec");
+      e = pp::launch_segnorm2(s->d_r, nrhs, s->d_seg, M, nrhs, s->d_nrm, st);
+      if (e != hipSuccess) return hip_fail(e, "norm r");
+      HIP_TRY(hipMemcpyAsync(rn.data(), s->d_nrm, 8 * rn.size(), hipMemcpyDeviceToHost, st));
+      HIP_TRY(hipStreamSynchronize(st));
+      res = 0.;
+      for (int m = 0; m < M; ++m) {
+        double worst = 0.;
+        for (int c = 0; c < nrhs; ++c) {
+          const double bb = bn[m * nrhs + c], rr = rn[m * nrhs + c];
+          worst = std::max(worst, bb > 0 ? std::sqrt(rr / bb) : std::sqrt(rr));
+        }
+        s->member_res[m] = worst;
+        res = std::max(res, worst);
+      }
+      if (res <= tol) {
+        conv = true;
+        break;
+      }
+      if (it >= max_iter) break;
+      ++it;
+      e = pp::launch_copy2d(s->d_rhs, eb, nrhs, s->d_r, 8, nrhs, (int)s->n_total, nrhs, st);
+      if (e != hipSuccess) return hip_fail(e, "copy residual");
+      if ((rc = enqueue_launch(s->ctx, sp->stream, s->wpi, st, true))) return rc;
+      e = pp::launch_add_correction(x, ld, s->d_rhs, eb, nrhs, (int)s->n_total, nrhs, st);
+      if (e != hipSuccess) return hip_fail(e, "update x");
+    }
+    if ((rc = check_status(sp->stream))) return rc;
+    if (rep) {
+      rep->iterations = it;
+      rep->converged = conv ? 1 : 0;
+      rep->residual = res;
+      rep->factor_ms = s->last_factor_ms;
+      rep->total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+      rep->solve_ms = rep->total_ms;
+    }
+    return PP_OK;
+  } catch (...) {
+    return from_exception();
+  }
+}
+
+int pp_solver_factor_arena(pp_solver* s, void** dptr, int* eb) {
+  if (s == nullptr || dptr == nullptr || eb == nullptr) return set_error(PP_ERR_INVALID, "null");
+  *dptr = s->d_F;
+  *eb = elem_bytes(s->precision);
+  return PP_OK;
+}
+
+int pp_solver_destroy(pp_solver* s) {
+  destroy_solver(s);
+  return PP_OK;
+}
+
+pp_stream* pp_solver_factor_stream(pp_solver* s) { return s ? s->fstream : nullptr; }
+double pp_solver_factor_flops(const pp_solver* s) { return s ? s->factor_flops : 0.; }
+double pp_solver_last_factor_ms(const pp_solver* s) { return s ? s->last_factor_ms : -1.; }
+
+double pp_solver_solve_flops(const pp_solver* s, int nrhs) {
+  if (s == nullptr) return 0.;
+  double e = 0.;
+  for (const pp_hmatrix* h : s->hms) e += (double)hier(h)->storedEntries();
+  return 2.0 * e * nrhs;
+}
+
+}  // extern "C"

@@ -31,8 +31,10 @@ def rel(a, b):
 
 @pytest.mark.parametrize("mode", [0, 1, 2])
 def test_mfma_lane_layout(ctx, mode):
-    """D = A B for one 16x16 tile: lane l, register r holds D[4(l/16)+r][l%16];
-    A lanes supply A[l%16][kpl (l/16) + e], B lanes B[kpl (l/16) + e][l%16]."""
+    """D = A B for one 16x16 tile.  A lanes supply A[l%16][kpl (l/16) + e],
+    B lanes B[kpl (l/16) + e][l%16]; accumulator r of lane l holds
+    D[(l/16) + 4r][l%16] for v_mfma_f64_16x16x4f64 and D[4(l/16) + r][l%16]
+    for the FP32 and BF16 instructions."""
     k = 32 if mode == 2 else 4
     rng = np.random.default_rng(mode)
     a = rng.integers(-4, 5, (16, k)).astype(np.float64)
@@ -41,7 +43,8 @@ def test_mfma_lane_layout(ctx, mode):
     d = a @ b
     for l in range(64):
         for r in range(4):
-            assert out[4 * l + r] == d[4 * (l // 16) + r, l % 16], (l, r)
+            row = (l // 16) + 4 * r if mode == 0 else 4 * (l // 16) + r
+            assert out[4 * l + r] == d[row, l % 16], (l, r)
 
 
 def run(ctx, prec, words, bufs, scratch_elems=1 << 18):

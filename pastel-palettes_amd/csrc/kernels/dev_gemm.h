@@ -100,7 +100,32 @@ __device__ __forceinline__ void wave_tile(typename Mma<T, MODE>::acc_t acc[2][2]
   const int ra0 = r0 + li, ra1 = r0 + 16 + li;
   const int cb0 = c0 + li, cb1 = c0 + 16 + li;
   const bool va0 = ra0 < m, va1 = ra1 < m, vb0 = cb0 < n, vb1 = cb1 < n;
-  for (int kk = k_begin; kk < k_end; kk += k_stride) {
+  // Full chunks of U k-steps: every fragment load of the chunk is issued
+  // before the first MFMA, so the loads overlap instead of paying one memory
+  // round trip per k-step.
+  constexpr int U = (MODE == MODE_BF16) ? 2 : 4;
+  int kk = k_begin;
+  for (; kk + (U - 1) * k_stride + P::KSTEP <= k_end; kk += U * k_stride) {
+    T a0[U][P::KPL], a1[U][P::KPL], b0[U][P::KPL], b1[U][P::KPL];
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+#pragma unroll
+      for (int e = 0; e < P::KPL; ++e) {
+        const int kx = kk + u * k_stride + lk + e;
+        a0[u][e] = va0 ? opget(A, lda, aT, ra0, kx) : T(0);
+        a1[u][e] = va1 ? opget(A, lda, aT, ra1, kx) : T(0);
+        b0[u][e] = vb0 ? opget(B, ldb, !bT, cb0, kx) : T(0);
+        b1[u][e] = vb1 ? opget(B, ldb, !bT, cb1, kx) : T(0);
+      }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      acc[0][0] = P::mma(a0[u], b0[u], acc[0][0]);
+      acc[0][1] = P::mma(a0[u], b1[u], acc[0][1]);
+      acc[1][0] = P::mma(a1[u], b0[u], acc[1][0]);
+      acc[1][1] = P::mma(a1[u], b1[u], acc[1][1]);
+    }
+  }
+  for (; kk < k_end; kk += k_stride) {
     T a0[P::KPL], a1[P::KPL], b0[P::KPL], b1[P::KPL];
     if (kk + P::KSTEP <= k_end) {
 #pragma unroll
@@ -194,6 +219,17 @@ __device__ void gemm_block(int m, int n, int k, T alpha, T beta, T* C, int ldc, 
   for (int t = 0; t < tiles_m * tiles_n; ++t) {
     const int r0 = (t / tiles_n) * tm + my_m * 32, c0 = (t % tiles_n) * tn + my_n * 32;
     if (r0 >= m || c0 >= n) continue;
+    // the old C tile is fetched before the K loop so its latency overlaps it
+    T cold[2][2][4];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+      for (int j = 0; j < 2; ++j)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int row = r0 + i * 16 + P::orow(l, r), col = c0 + j * 16 + (l & 15);
+          cold[i][j][r] = (beta != T(0) && row < m && col < n) ? C[(int64_t)row * ldc + col] : T(0);
+        }
     acc_t acc[2][2] = {{P::zero(), P::zero()}, {P::zero(), P::zero()}};
     wave_tile<T, MODE>(acc, m, n, r0, c0, 0, k, P::KSTEP, A, lda, aT, B, ldb, bT);
 #pragma unroll
@@ -203,10 +239,8 @@ __device__ void gemm_block(int m, int n, int k, T alpha, T beta, T* C, int ldc, 
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
           const int row = r0 + i * 16 + P::orow(l, r), col = c0 + j * 16 + (l & 15);
-          if (row < m && col < n) {
-            T* c = &C[(int64_t)row * ldc + col];
-            *c = (beta == T(0)) ? alpha * acc[i][j][r] : beta * *c + alpha * acc[i][j][r];
-          }
+          if (row < m && col < n)
+            C[(int64_t)row * ldc + col] = beta * cold[i][j][r] + alpha * acc[i][j][r];
         }
   }
   __syncthreads();

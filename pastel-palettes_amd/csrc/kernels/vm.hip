@@ -160,7 +160,11 @@ __global__ __launch_bounds__(NT) void vm_kernel(VmArgs a) {
         int ok = 1;
         const uint64_t t0 = wall_clock64();
         int64_t polls = 0;
-        while ((int32_t)(__hip_atomic_load(&a.flags[id], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) - epoch) < 0) {
+        // Poll with relaxed agent-scope loads (they read the coherent level
+        // without invalidating caches); one acquire fence after the flag is
+        // seen makes the producer's data visible.  An acquire load per poll
+        // would invalidate this XCD's L2 on every iteration.
+        while ((int32_t)(__hip_atomic_load(&a.flags[id], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) - epoch) < 0) {
           if ((++polls & 31) == 0) {
             if (__hip_atomic_load(&a.status[3], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) {
               ok = 0;
@@ -175,11 +179,13 @@ __global__ __launch_bounds__(NT) void vm_kernel(VmArgs a) {
           if (hint > 8) __builtin_amdgcn_s_sleep(8);
           else __builtin_amdgcn_s_sleep(1);
         }
+        // one acquire for the workgroup: invalidates this CU's L1 and the
+        // XCD's L2 copies so the producer's written-back data is read fresh
+        if (ok) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
         ctl[entry & 1] = ok;
       }
       __syncthreads();
       if (!ctl[entry & 1]) break;
-      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
       pc += wait_l;
       continue;
     }
@@ -193,9 +199,14 @@ __global__ __launch_bounds__(NT) void vm_kernel(VmArgs a) {
       if (threadIdx.x == 0) record_error(a.status, VM_BAD_OPERATION, w, id);
       break;
     }
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    // every wave waits until its stores are acknowledged by L2; after the
+    // barrier one release (L2 write-back) covers the whole workgroup
+    __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
-    if (threadIdx.x == 0) __hip_atomic_store(&a.flags[id], epoch, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    if (threadIdx.x == 0) {
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+      __hip_atomic_store(&a.flags[id], epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
     pc += len;
   }
 }

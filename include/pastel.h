@@ -172,6 +172,10 @@ int pp_max_workers(pp_ctx* ctx, int precision, int wavefronts_per_inst);
  * routines. */
 int pp_exec_single(pp_ctx* ctx, int precision, const int32_t* words, int n_words, const void* const* ptr_table,
                    int n_ptrs, uint64_t scratch_bytes, uint64_t seed);
+/* Same, `reps` times back to back in one launch of 64 * wavefronts threads;
+ * *ms receives the mean kernel time per execution (per-op latency). */
+int pp_exec_single_timed(pp_ctx* ctx, int precision, const int32_t* words, int n_words, const void* const* ptr_table,
+                         int n_ptrs, uint64_t scratch_bytes, uint64_t seed, int wavefronts, int reps, double* ms);
 /* Fill a device buffer with the sketch matrix used by accum (count values,
  * uniform in [-1, 1) from the in-library generator). */
 int pp_fill_sketch(pp_ctx* ctx, void* dptr, int64_t count, int precision, uint64_t seed);

@@ -88,6 +88,8 @@ def lib():
         "pp_stream_last_ms": (dp, [vp]),
         "pp_max_workers": (C.c_int, [vp, C.c_int, C.c_int]),
         "pp_exec_single": (C.c_int, [vp, C.c_int, P(i32), C.c_int, P(vp), C.c_int, u64, u64]),
+        "pp_exec_single_timed": (C.c_int, [vp, C.c_int, P(i32), C.c_int, P(vp), C.c_int, u64, u64, C.c_int, C.c_int,
+                                           P(dp)]),
         "pp_fill_sketch": (C.c_int, [vp, vp, i64, C.c_int, u64]),
         "pp_mfma_probe": (C.c_int, [vp, C.c_int, P(dp), C.c_int, P(dp), C.c_int, P(dp)]),
         "pp_solver_create": (C.c_int, [vp, vp, P(dp), C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, P(vp)]),
@@ -336,10 +338,20 @@ class Context:
     def max_workers(self, precision=FP64, wavefronts=4):
         return check(lib().pp_max_workers(self.h, precision, wavefronts))
 
-    def exec_single(self, precision, words, ptrs, scratch_bytes=1 << 20, seed=0x5EED):
+    def exec_single(self, precision, words, ptrs, scratch_bytes=1 << 20, seed=0x5EED, wavefronts=4):
         w = np.ascontiguousarray(words, dtype=np.int32)
         arr = (C.c_void_p * len(ptrs))(*[int(p) for p in ptrs])
-        check(lib().pp_exec_single(self.h, precision, _p(w, C.c_int32), len(w), arr, len(ptrs), scratch_bytes, seed))
+        check(lib().pp_exec_single_timed(self.h, precision, _p(w, C.c_int32), len(w), arr, len(ptrs), scratch_bytes,
+                                         seed, wavefronts, 1, None))
+
+    def time_single(self, precision, words, ptrs, scratch_bytes=1 << 22, wavefronts=4, reps=20, seed=0x5EED):
+        """Mean kernel time (ms) of one instruction executed `reps` times in place."""
+        w = np.ascontiguousarray(words, dtype=np.int32)
+        arr = (C.c_void_p * len(ptrs))(*[int(p) for p in ptrs])
+        ms = C.c_double()
+        check(lib().pp_exec_single_timed(self.h, precision, _p(w, C.c_int32), len(w), arr, len(ptrs), scratch_bytes,
+                                         seed, wavefronts, reps, C.byref(ms)))
+        return ms.value
 
     def mfma_probe(self, mode, a, b):
         a = np.ascontiguousarray(a, np.float64)

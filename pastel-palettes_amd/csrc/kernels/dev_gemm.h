@@ -23,7 +23,7 @@ typedef short s8_t __attribute__((ext_vector_type(8)));
 enum : int { MODE_FP64 = 0, MODE_FP32 = 1, MODE_BF16 = 2 };
 
 // LDS arena shared by all routines of one worker (bytes).
-constexpr int kLdsBytes = 64 * 1024;
+constexpr int kLdsBytes = 144 * 1024;
 
 __device__ __forceinline__ int lane_id() { return threadIdx.x & 63; }
 __device__ __forceinline__ int wave_id() { return threadIdx.x >> 6; }

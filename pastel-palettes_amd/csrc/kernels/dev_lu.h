@@ -19,6 +19,7 @@
 #pragma once
 
 #include "dev_gemm.h"
+#include "dev_leaf.h"
 
 namespace pp {
 
@@ -28,6 +29,7 @@ constexpr int PB = 16;  // panel width
 // M is ny x nx, pitch ld; requires ny <= 2 * blockDim.x.
 template <typename T, int FMODE>
 __device__ void getrf_block(T* M, int nx, int ny, int ld, void* lds) {
+  if (getrf_lds<T>(M, nx, ny, ld, lds, kLdsBytes)) return;  // whole block fits in LDS
   T* piv = reinterpret_cast<T*>(lds);            // [PB] current pivot row
   T* l11 = piv + PB;                             // [PB][PB] diagonal block
   void* glds = reinterpret_cast<char*>(lds) + 8 * 1024;  // gemm scratch after 8 KB
@@ -119,6 +121,7 @@ __device__ __forceinline__ void stage_block(T* dst, const T* src, int ld, int r0
 // B (ny_b x nx_b logical) <- L^{-1} B, L unit lower ny_b x ny_b.
 template <typename T, int FMODE>
 __device__ void trsml_block(T* B, const T* L, int nx_b, int ny_b, int ld_b, int ld_l, bool b_T, void* lds) {
+  if (!b_T && trsm_rows<T, 0>(B, L, ny_b, nx_b, ld_b, ld_l, lds, kLdsBytes)) return;
   T* lb = reinterpret_cast<T*>(lds);
   void* glds = reinterpret_cast<char*>(lds) + 8 * 1024;
   for (int p = 0; p < ny_b; p += PB) {
@@ -160,6 +163,8 @@ __device__ void trsml_block(T* B, const T* L, int nx_b, int ny_b, int ld_b, int 
 // B (ny_b x nx_b logical) <- B U^{-1}, U upper non-unit nx_b x nx_b.
 template <typename T, int FMODE>
 __device__ void trsmr_block(T* B, const T* U, int nx_b, int ny_b, int ld_b, int ld_u, bool b_T, void* lds) {
+  // transposed B (the V factor of a low-rank block): U^T Bs_new = Bs
+  if (b_T && trsm_rows<T, 1>(B, U, nx_b, ny_b, ld_b, ld_u, lds, kLdsBytes)) return;
   T* ub = reinterpret_cast<T*>(lds);
   void* glds = reinterpret_cast<char*>(lds) + 8 * 1024;
   for (int p = 0; p < nx_b; p += PB) {
@@ -205,6 +210,7 @@ __device__ void trsmr_block(T* B, const T* U, int nx_b, int ny_b, int ld_b, int 
 // B (ny_b x nx_b) <- U^{-1} B, U upper non-unit ny_b x ny_b (backward).
 template <typename T, int FMODE>
 __device__ void trsmu_block(T* B, const T* U, int nx_b, int ny_b, int ld_b, int ld_u, void* lds) {
+  if (trsm_rows<T, 2>(B, U, ny_b, nx_b, ld_b, ld_u, lds, kLdsBytes)) return;
   T* ub = reinterpret_cast<T*>(lds);
   void* glds = reinterpret_cast<char*>(lds) + 8 * 1024;
   const int nblk = (ny_b + PB - 1) / PB;

@@ -44,7 +44,7 @@ hipError_t launch_hmatvec(const MatvecPlanDev& p, const double* A, const double*
 hipError_t launch_segnorm2(const double* r, int ldr, const int* seg, int nseg, int nrhs, double* out, hipStream_t s);
 hipError_t launch_colnorm2(const double* r, int ldr, int n, int nrhs, double* out, hipStream_t s);
 hipError_t launch_mfma_probe(int mode, const double* a, const double* b, double* out, hipStream_t s);
-hipError_t launch_single_op(int precision, const int32_t* words, void* const* ptrs, void* scratch, const void* rnd,
-                            int32_t* status, hipStream_t s);
+hipError_t launch_single_op(int precision, int threads, int reps, const int32_t* words, void* const* ptrs,
+                            void* scratch, const void* rnd, int32_t* status, hipStream_t s);
 
 }  // namespace pp

@@ -504,29 +504,42 @@ hipError_t launch_mfma_probe(int mode, const double* a, const double* b, double*
   return hipGetLastError();
 }
 
-// Single-op test harness: run one execute entry on one workgroup.
-template <typename T, int FMODE, int GMODE>
-__global__ __launch_bounds__(256) void single_op_kernel(const int32_t* words, void* const* ptrs, void* scratch,
-                                                        const void* rnd, int32_t* status) {
+// Single-op harness: one workgroup runs one execute entry `reps` times
+// (tests use reps = 1; latency measurements use more, in place).
+template <typename T, int FMODE, int GMODE, int NT>
+__global__ __launch_bounds__(NT) void single_op_kernel(const int32_t* words, void* const* ptrs, void* scratch,
+                                                       const void* rnd, int32_t* status, int reps) {
   __shared__ __attribute__((aligned(16))) char lds[kLdsBytes];
   VmArgs a{};
   a.ptrs = ptrs;
   a.rnd = rnd;
-  const int len = exec_entry<T, FMODE, GMODE>(words, a, reinterpret_cast<T*>(scratch), lds);
+  int len = -1;
+  for (int r = 0; r < reps; ++r) {
+    len = exec_entry<T, FMODE, GMODE>(words, a, reinterpret_cast<T*>(scratch), lds);
+    __syncthreads();
+  }
   if (threadIdx.x == 0) status[0] = len;
 }
 
-hipError_t launch_single_op(int precision, const int32_t* words, void* const* ptrs, void* scratch, const void* rnd,
-                            int32_t* status, hipStream_t s) {
-  if (precision == 0)
-    hipLaunchKernelGGL((single_op_kernel<double, MODE_FP64, MODE_FP64>), dim3(1), dim3(256), 0, s, words, ptrs,
-                       scratch, rnd, status);
-  else if (precision == 1)
-    hipLaunchKernelGGL((single_op_kernel<float, MODE_FP32, MODE_FP32>), dim3(1), dim3(256), 0, s, words, ptrs,
-                       scratch, rnd, status);
+template <typename T, int FMODE, int GMODE>
+static void launch_single_t(int threads, const int32_t* words, void* const* ptrs, void* scratch, const void* rnd,
+                            int32_t* status, int reps, hipStream_t s) {
+  if (threads == 512)
+    hipLaunchKernelGGL((single_op_kernel<T, FMODE, GMODE, 512>), dim3(1), dim3(512), 0, s, words, ptrs, scratch, rnd,
+                       status, reps);
   else
-    hipLaunchKernelGGL((single_op_kernel<float, MODE_FP32, MODE_BF16>), dim3(1), dim3(256), 0, s, words, ptrs,
-                       scratch, rnd, status);
+    hipLaunchKernelGGL((single_op_kernel<T, FMODE, GMODE, 256>), dim3(1), dim3(256), 0, s, words, ptrs, scratch, rnd,
+                       status, reps);
+}
+
+hipError_t launch_single_op(int precision, int threads, int reps, const int32_t* words, void* const* ptrs,
+                            void* scratch, const void* rnd, int32_t* status, hipStream_t s) {
+  if (precision == 0)
+    launch_single_t<double, MODE_FP64, MODE_FP64>(threads, words, ptrs, scratch, rnd, status, reps, s);
+  else if (precision == 1)
+    launch_single_t<float, MODE_FP32, MODE_FP32>(threads, words, ptrs, scratch, rnd, status, reps, s);
+  else
+    launch_single_t<float, MODE_FP32, MODE_BF16>(threads, words, ptrs, scratch, rnd, status, reps, s);
   return hipGetLastError();
 }
 

@@ -420,18 +420,19 @@ int pp_fill_sketch(pp_ctx* ctx, void* dptr, int64_t count, int precision, uint64
   return PP_OK;
 }
 
-int pp_exec_single(pp_ctx* ctx, int precision, const int32_t* words, int n_words, const void* const* ptr_table,
-                   int n_ptrs, uint64_t scratch_bytes, uint64_t seed) {
-  if (ctx == nullptr || words == nullptr || n_words < 3 || ptr_table == nullptr || n_ptrs < 1)
+int pp_exec_single_timed(pp_ctx* ctx, int precision, const int32_t* words, int n_words, const void* const* ptr_table,
+                         int n_ptrs, uint64_t scratch_bytes, uint64_t seed, int wavefronts, int reps, double* ms) {
+  if (ctx == nullptr || words == nullptr || n_words < 3 || ptr_table == nullptr || n_ptrs < 1 || reps < 1 ||
+      (wavefronts != 4 && wavefronts != 8))
     return set_error(PP_ERR_INVALID, "bad args");
   const int len = operationLength(words[2]);
   if (words[0] != execute || len <= 0 || n_words < len) return set_error(PP_ERR_INVALID, "bad instruction words");
-  for (int i = 3; i < len && i < 8; ++i) (void)i;
   int32_t* d_w = nullptr;
   void** d_p = nullptr;
   void* d_s = nullptr;
   void* d_r = nullptr;
   int32_t* d_st = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
   int rc = PP_OK;
   const int eb = elem_bytes(precision);
   int64_t need = 1;
@@ -453,22 +454,37 @@ int pp_exec_single(pp_ctx* ctx, int precision, const int32_t* words, int n_words
         e = hipMemcpy(d_r, f.data(), need * 4, hipMemcpyHostToDevice);
       }
     }
-    if (e == hipSuccess) e = pp::launch_single_op(precision, d_w, d_p, d_s, d_r, d_st, ctx->stream);
+    if (e == hipSuccess) e = hipEventCreate(&e0);
+    if (e == hipSuccess) e = hipEventCreate(&e1);
+    if (e == hipSuccess) e = hipEventRecord(e0, ctx->stream);
+    if (e == hipSuccess)
+      e = pp::launch_single_op(precision, 64 * wavefronts, reps, d_w, d_p, d_s, d_r, d_st, ctx->stream);
+    if (e == hipSuccess) e = hipEventRecord(e1, ctx->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
     int32_t st = 0;
     if (e == hipSuccess) e = hipMemcpy(&st, d_st, 4, hipMemcpyDeviceToHost);
+    float t = 0.f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&t, e0, e1);
     if (e != hipSuccess) {
       rc = hip_fail(e, "exec_single");
       break;
     }
+    if (ms) *ms = t / reps;
     if (st != len) rc = set_error(PP_ERR_DEVICE, "device rejected the instruction");
   } while (false);
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
   ctx_free(ctx, d_w);
   ctx_free(ctx, d_p);
   ctx_free(ctx, d_s);
   ctx_free(ctx, d_r);
   ctx_free(ctx, d_st);
   return rc;
+}
+
+int pp_exec_single(pp_ctx* ctx, int precision, const int32_t* words, int n_words, const void* const* ptr_table,
+                   int n_ptrs, uint64_t scratch_bytes, uint64_t seed) {
+  return pp_exec_single_timed(ctx, precision, words, n_words, ptr_table, n_ptrs, scratch_bytes, seed, 4, 1, nullptr);
 }
 
 int pp_mfma_probe(pp_ctx* ctx, int mode, const double* a, int na, const double* b, int nb, double* out) {

@@ -47,10 +47,10 @@ def test_mfma_lane_layout(ctx, mode):
             assert out[4 * l + r] == d[row, l % 16], (l, r)
 
 
-def run(ctx, prec, words, bufs, scratch_elems=1 << 18):
+def run(ctx, prec, words, bufs, scratch_elems=1 << 18, wf=4):
     """Execute one instruction; slots are the buffers in order."""
     w = np.array([1, 0] + list(words), np.int32)
-    ctx.exec_single(prec, w, [b.data_ptr() for b in bufs], scratch_bytes=8 * scratch_elems)
+    ctx.exec_single(prec, w, [b.data_ptr() for b in bufs], scratch_bytes=8 * scratch_elems, wavefronts=wf)
     torch.cuda.synchronize()
 
 
@@ -62,51 +62,58 @@ def diag_dominant(rng, n, m=None):
 
 
 @pytest.mark.parametrize("prec", [FP64, FP32])
-@pytest.mark.parametrize("n,m", [(256, 256), (128, 128), (200, 200), (256, 100), (90, 300)])
-def test_getrf(ctx, orc, prec, n, m):
+@pytest.mark.parametrize("n,m", [(256, 256), (128, 128), (200, 200), (256, 100), (90, 300), (100, 100), (128, 64),
+                                 (64, 128), (180, 180)])
+@pytest.mark.parametrize("wf", [4, 8])
+def test_getrf(ctx, orc, prec, n, m, wf):
     rng = np.random.default_rng(n + m)
     a = diag_dominant(rng, n, m)
     ld = m + 3
     buf = np.zeros((n, ld))
     buf[:, :m] = a
     d = dev(buf, prec)
-    run(ctx, prec, [1, 0, 0, m, n, ld], [d])
+    run(ctx, prec, [1, 0, 0, m, n, ld], [d], wf=wf)
     want = orc.getrf(a)
     assert rel(host(d)[:, :m], want) < TOL[prec] * 10
 
 
 @pytest.mark.parametrize("prec", [FP64, FP32])
-@pytest.mark.parametrize("ny,nx,bt", [(256, 32, 0), (256, 256, 0), (128, 7, 0), (256, 32, 1), (100, 40, 1)])
-def test_trsml(ctx, orc, prec, ny, nx, bt):
+@pytest.mark.parametrize("ny,nx,bt", [(256, 32, 0), (256, 256, 0), (128, 7, 0), (256, 32, 1), (100, 40, 1),
+                                      (128, 64, 0), (37, 16, 0)])
+@pytest.mark.parametrize("wf", [4, 8])
+def test_trsml(ctx, orc, prec, ny, nx, bt, wf):
     rng = np.random.default_rng(ny * 7 + nx)
     l = np.tril(rng.standard_normal((ny, ny)) / ny, -1) + np.eye(ny)
     b = rng.standard_normal((nx, ny) if bt else (ny, nx))
     dl, db = dev(l, prec), dev(b, prec)
-    run(ctx, prec, [2, 0, 1, 0, 0, nx, ny, ny, b.shape[1], ny, bt], [db, dl])
+    run(ctx, prec, [2, 0, 1, 0, 0, nx, ny, ny, b.shape[1], ny, bt], [db, dl], wf=wf)
     want = orc.trsml(b, l, b_t=bool(bt))
     assert rel(host(db), want) < TOL[prec] * 10
 
 
 @pytest.mark.parametrize("prec", [FP64, FP32])
-@pytest.mark.parametrize("ny,nx,bt", [(256, 256, 0), (32, 256, 1), (77, 130, 0), (19, 200, 1)])
-def test_trsmr(ctx, orc, prec, ny, nx, bt):
+@pytest.mark.parametrize("ny,nx,bt", [(256, 256, 0), (32, 256, 1), (77, 130, 0), (19, 200, 1), (32, 128, 1),
+                                      (64, 33, 1)])
+@pytest.mark.parametrize("wf", [4, 8])
+def test_trsmr(ctx, orc, prec, ny, nx, bt, wf):
     rng = np.random.default_rng(ny * 5 + nx)
     u = np.triu(rng.standard_normal((nx, nx)) / nx) + 2.0 * np.eye(nx)
     b = rng.standard_normal((nx, ny) if bt else (ny, nx))
     du, db = dev(u, prec), dev(b, prec)
-    run(ctx, prec, [3, 0, 1, 0, 0, nx, ny, nx, b.shape[1], nx, bt], [db, du])
+    run(ctx, prec, [3, 0, 1, 0, 0, nx, ny, nx, b.shape[1], nx, bt], [db, du], wf=wf)
     want = orc.trsmr(b, u, b_t=bool(bt))
     assert rel(host(db), want) < TOL[prec] * 10
 
 
 @pytest.mark.parametrize("prec", [FP64, FP32])
-@pytest.mark.parametrize("ny,nx", [(256, 16), (256, 1), (131, 33)])
-def test_trsmu(ctx, orc, prec, ny, nx):
+@pytest.mark.parametrize("ny,nx", [(256, 16), (256, 1), (131, 33), (128, 64), (256, 100)])
+@pytest.mark.parametrize("wf", [4, 8])
+def test_trsmu(ctx, orc, prec, ny, nx, wf):
     rng = np.random.default_rng(ny + 3 * nx)
     u = np.triu(rng.standard_normal((ny, ny)) / ny) + 2.0 * np.eye(ny)
     b = rng.standard_normal((ny, nx))
     du, db = dev(u, prec), dev(b, prec)
-    run(ctx, prec, [9, 0, 1, 0, 0, nx, ny, ny, nx, ny, 0], [db, du])
+    run(ctx, prec, [9, 0, 1, 0, 0, nx, ny, ny, nx, ny, 0], [db, du], wf=wf)
     assert rel(host(db), orc.trsmu(b, u)) < TOL[prec] * 10
 
 

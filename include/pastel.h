@@ -148,7 +148,8 @@ int pp_stream_upload_program(pp_ctx* ctx, const pp_program* prog, const void* co
 /* Rebind pointer-table slot i. */
 int pp_stream_set_slot(pp_ctx* ctx, pp_stream* s, int slot, void* dptr);
 /* Launch the persistent VM (kernel_dynamic, src/kernel.cu:1375) on hip_stream
- * (NULL = the context stream); threads = 64 * wavefronts per worker. */
+ * (NULL = the context stream); threads = 64 * wavefronts per worker (4 wavefronts:
+ * one 256-thread workgroup per worker). */
 int pp_launch(pp_ctx* ctx, pp_stream* s, int wavefronts_per_inst, void* hip_stream);
 /* Wait for the launch; on a VM error reports the first unsignalled instruction
  * (or the faulting one) in *first_unsignalled (may be NULL). */

@@ -112,19 +112,32 @@ dview dev_temp::request(int ny, int nx) {
   return v;
 }
 
+// Scratch of one accum: Y (ny x k1), Vn (nx x k1), Q (ny x k1), then the
+// small area: S (k1 x kmax), the Gram matrix / Cholesky factor (k1 x k1), the
+// inverses of its 32 x 32 diagonal blocks, and tau / w of the Householder
+// fallback (2 kmax).  Every part starts 32-element aligned.
+int64_t accum_small_elems(int k1, int k2) {
+  const int64_t kmax = std::max(k1, k2);
+  return align32(k1 * kmax) + align32((int64_t)k1 * k1) + (int64_t)((k1 + 31) / 32) * 1024 + align32(2 * kmax);
+}
+
 void accum_scratch(int nx, int ny, int k1, int k2, int64_t* offset1, int64_t* offset2, int64_t* total) {
-  const int kmax = std::max(k1, k2);
   *offset1 = align32((int64_t)ny * k1);
   *offset2 = align32(*offset1 + (int64_t)nx * k1);
-  *total = align32(*offset2 + (int64_t)ny * k1) + align32(4LL * k1 * kmax + 2LL * kmax);
+  *total = align32(*offset2 + (int64_t)ny * k1) + accum_small_elems(k1, k2);
 }
 
 // ------------------------------------------------------------------ emitters
+
+// Device getrf / trsm keep inverses of the 32 x 32 diagonal blocks of an
+// n x n triangular factor in the worker scratch.
+int64_t tri_scratch_elems(int n) { return (int64_t)((n + 31) / 32) * 32 * 32; }
 
 void emit_getrf(h_ops_tree* tree, const dview& m) {
   h_op o = make_op(getrf, {m.ptr, off32(m.off), m.nx, m.ny, m.ld});
   o.data.addData(false, m.map());
   o.flops = flops_getrf(m.nx, m.ny);
+  o.scratch = tri_scratch_elems(32);
   tree->push(std::move(o));
 }
 
@@ -135,6 +148,7 @@ void emit_trsml(h_ops_tree* tree, const mview& b, const dview& l) {
   o.data.addData(false, b.v.map());
   o.data.addData(true, l.map());
   o.flops = flops_trsml(nx_b, ny_b);
+  o.scratch = tri_scratch_elems(ny_b);
   tree->push(std::move(o));
 }
 
@@ -145,6 +159,7 @@ void emit_trsmr(h_ops_tree* tree, const mview& b, const dview& u) {
   o.data.addData(false, b.v.map());
   o.data.addData(true, u.map());
   o.flops = flops_trsmr(nx_b, ny_b);
+  o.scratch = tri_scratch_elems(nx_b);
   tree->push(std::move(o));
 }
 
@@ -154,6 +169,7 @@ void emit_trsmu(h_ops_tree* tree, const dview& b, const dview& u) {
   o.data.addData(false, b.map());
   o.data.addData(true, u.map());
   o.flops = (long long)b.nx * b.ny * b.ny;
+  o.scratch = tri_scratch_elems(b.ny);
   tree->push(std::move(o));
 }
 

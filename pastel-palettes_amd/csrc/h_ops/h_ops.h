@@ -157,4 +157,6 @@ void emit_accum(h_ops_tree* tree, const dview& u1, const dview& v1, const dview&
 
 // Scratch layout of accum for (nx, ny, k1, k2): Y at 0, V at offset1, Q at
 // offset2, then small k1 x max(k1,k2) panels.
+int64_t accum_small_elems(int k1, int k2);
 void accum_scratch(int nx, int ny, int k1, int k2, int64_t* offset1, int64_t* offset2, int64_t* total);
+int64_t tri_scratch_elems(int n);

@@ -5,10 +5,23 @@
 // explicit row pitch; op(X) = X or X^T selected by a flag.  Arithmetic:
 //   * FP64 storage: v_mfma_f64_16x16x4f64;
 //   * FP32 storage: v_mfma_f32_16x16x4f32, or in BF16 mode the operands are
-//     rounded to bfloat16 and fed to v_mfma_f32_16x16x32_bf16 (FP32
-//     accumulation).
-// A wavefront owns a 32x32 output tile (2x2 MFMA tiles of 16x16); the
-// workgroup's waves tile the output (or split K for small outputs).
+//     rounded to bfloat16 in registers (v_cvt_pk_bf16_f32) and fed to
+//     v_mfma_f32_16x16x32_bf16 (FP32 accumulation).
+//
+// GEMM organisation (gemm_block).  Every wave owns whole 32 x 32 output tiles
+// (2 x 2 MFMA tiles) and runs independently: the operand fragments are loaded
+// straight from global memory (the operands of the H-LU kernels are L2
+// resident) into registers, one K chunk ahead of the MFMAs that consume the
+// current chunk, over a flattened (tile, chunk) loop, so load latency hides
+// behind the MFMAs of a chunk and there is no barrier inside the product.  The
+// accumulator starts from the C tile.  When the output has fewer tiles than
+// the workgroup has waves, gk waves share a tile and split its K chunks; their
+// partial sums are added through LDS in a fixed order (deterministic results).
+//
+// K permutation: inside a wave's chunk of KW values, MFMA step s of lane group
+// g = lane / 16 uses k = EPL g + s, so the EPL operand values of one lane are
+// contiguous along k (one or two 16-byte loads when op(X) is stored
+// k-contiguous).
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -16,17 +29,32 @@
 
 namespace pp {
 
+typedef double d2_t __attribute__((ext_vector_type(2)));
 typedef double d4_t __attribute__((ext_vector_type(4)));
 typedef float f4_t __attribute__((ext_vector_type(4)));
 typedef short s8_t __attribute__((ext_vector_type(8)));
 
 enum : int { MODE_FP64 = 0, MODE_FP32 = 1, MODE_BF16 = 2 };
 
-// LDS arena shared by all routines of one worker (bytes).
+// LDS arena shared by all routines of one worker (bytes).  It is declared at
+// namespace scope so that every routine addresses it as LDS (ds_* instructions)
+// rather than through generic pointers.
 constexpr int kLdsBytes = 144 * 1024;
+__shared__ __attribute__((aligned(16))) char pp_lds[kLdsBytes];
+// Pointers into the arena carry the LDS address space explicitly (PP_LDS), so
+// accesses compile to ds_* instructions even through selects and loops.
+#define PP_LDS __attribute__((address_space(3)))
+__device__ __forceinline__ PP_LDS char* lds_base() { return (PP_LDS char*)pp_lds; }
 
 __device__ __forceinline__ int lane_id() { return threadIdx.x & 63; }
 __device__ __forceinline__ int wave_id() { return threadIdx.x >> 6; }
+
+// Compile-time integer, used to unroll loops whose register buffers must be
+// indexed statically.
+template <int N>
+struct IC {
+  static constexpr int value = N;
+};
 
 // Round-to-nearest-even float -> bfloat16 bits.
 __device__ __forceinline__ short f2bf(float x) {
@@ -36,212 +64,320 @@ __device__ __forceinline__ short f2bf(float x) {
   return (short)(u >> 16);
 }
 
-// MFMA policy: K per instruction, operand elements per lane, accumulator.
+// Global loads / stores through the global address space (no flat-address
+// coupling with the LDS counter).
+template <typename V>
+__device__ __forceinline__ V gload(const void* p) {
+  typedef const __attribute__((address_space(1))) V* gptr_t;
+  return *(gptr_t)(p);
+}
+template <typename V>
+__device__ __forceinline__ void gstore(void* p, V v) {
+  typedef __attribute__((address_space(1))) V* gptr_t;
+  *(gptr_t)(p) = v;
+}
+
+// MFMA policy.  KSTEP / KPL / mma describe one instruction (lane l supplies
+// A[l % 16][KPL (l / 16) + e] and B[KPL (l / 16) + e][l % 16]); KW / EPL /
+// mma_chunk describe the K chunk a wave consumes per fragment load (EPL
+// contiguous k values per lane).
 template <typename T, int MODE>
 struct Mma;
 
 template <>
 struct Mma<double, MODE_FP64> {
   using acc_t = d4_t;
-  static constexpr int KSTEP = 4, KPL = 1;
+  static constexpr int KSTEP = 4, KPL = 1, KW = 16, EPL = 4;
   // accumulator r of lane l holds row (l / 16) + 4 r, column l % 16 of the tile
   __device__ static __forceinline__ int orow(int l, int r) { return (l >> 4) + 4 * r; }
   __device__ static __forceinline__ acc_t zero() { return acc_t{0., 0., 0., 0.}; }
   __device__ static __forceinline__ acc_t mma(const double* a, const double* b, acc_t c) {
     return __builtin_amdgcn_mfma_f64_16x16x4f64(a[0], b[0], c, 0, 0, 0);
   }
+  __device__ static __forceinline__ acc_t mma_chunk(const double (&a)[EPL], const double (&b)[EPL], acc_t c) {
+    c = __builtin_amdgcn_mfma_f64_16x16x4f64(a[0], b[0], c, 0, 0, 0);
+    c = __builtin_amdgcn_mfma_f64_16x16x4f64(a[1], b[1], c, 0, 0, 0);
+    c = __builtin_amdgcn_mfma_f64_16x16x4f64(a[2], b[2], c, 0, 0, 0);
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a[3], b[3], c, 0, 0, 0);
+  }
 };
 
 template <>
 struct Mma<float, MODE_FP32> {
   using acc_t = f4_t;
-  static constexpr int KSTEP = 4, KPL = 1;
+  static constexpr int KSTEP = 4, KPL = 1, KW = 16, EPL = 4;
   // accumulator r of lane l holds row 4 (l / 16) + r, column l % 16 of the tile
   __device__ static __forceinline__ int orow(int l, int r) { return 4 * (l >> 4) + r; }
   __device__ static __forceinline__ acc_t zero() { return acc_t{0.f, 0.f, 0.f, 0.f}; }
   __device__ static __forceinline__ acc_t mma(const float* a, const float* b, acc_t c) {
     return __builtin_amdgcn_mfma_f32_16x16x4f32(a[0], b[0], c, 0, 0, 0);
   }
+  __device__ static __forceinline__ acc_t mma_chunk(const float (&a)[EPL], const float (&b)[EPL], acc_t c) {
+    c = __builtin_amdgcn_mfma_f32_16x16x4f32(a[0], b[0], c, 0, 0, 0);
+    c = __builtin_amdgcn_mfma_f32_16x16x4f32(a[1], b[1], c, 0, 0, 0);
+    c = __builtin_amdgcn_mfma_f32_16x16x4f32(a[2], b[2], c, 0, 0, 0);
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a[3], b[3], c, 0, 0, 0);
+  }
 };
 
 template <>
 struct Mma<float, MODE_BF16> {
   using acc_t = f4_t;
-  static constexpr int KSTEP = 32, KPL = 8;
+  static constexpr int KSTEP = 32, KPL = 8, KW = 32, EPL = 8;
   __device__ static __forceinline__ int orow(int l, int r) { return 4 * (l >> 4) + r; }
   __device__ static __forceinline__ acc_t zero() { return acc_t{0.f, 0.f, 0.f, 0.f}; }
-  __device__ static __forceinline__ acc_t mma(const float* a, const float* b, acc_t c) {
-    s8_t av, bv;
+  __device__ static __forceinline__ s8_t to_bf16(const float* a) {
+    s8_t v;
 #pragma unroll
-    for (int e = 0; e < 8; ++e) {
-      av[e] = f2bf(a[e]);
-      bv[e] = f2bf(b[e]);
-    }
-    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, bv, c, 0, 0, 0);
+    for (int e = 0; e < 8; ++e) v[e] = __builtin_bit_cast(short, (__bf16)a[e]);  // RNE
+    return v;
+  }
+  __device__ static __forceinline__ acc_t mma(const float* a, const float* b, acc_t c) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(to_bf16(a), to_bf16(b), c, 0, 0, 0);
+  }
+  __device__ static __forceinline__ acc_t mma_chunk(const float (&a)[EPL], const float (&b)[EPL], acc_t c) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(to_bf16(a), to_bf16(b), c, 0, 0, 0);
   }
 };
 
-// Element (i, k) of op(A) where A is row-major with pitch ld.
-template <typename T>
-__device__ __forceinline__ T opget(const T* A, int ld, bool t, int i, int k) {
-  return t ? A[(int64_t)k * ld + i] : A[(int64_t)i * ld + k];
-}
+// ---------------------------------------------------------------- operand fragments
+// One lane's view of a 16-wide operand group: lane (g = l / 16, i = l % 16)
+// reads op(X)(x0 + i, k0 + EPL g + e), e < EPL, where op(X)(r, k) is
+// X[r ld + k] (along_k) or X[k ld + r].  Loads are never predicated and their
+// values are not inspected until the MFMAs that use them (branch-free loads
+// keep the compiler's memory-counter waits exact, so the prefetched chunk
+// stays in flight while the current one is multiplied):
+//   * rows outside op(X) read row 0; they only feed output rows / columns
+//     that are never stored;
+//   * k at or beyond kmax reads k = kmax - 1 (finite data); the A operand of
+//     such k is zeroed just before its MFMA, which cancels the product.
+template <typename T, int EPL>
+struct OperandLane {
+  const T* p;    // &op(X)(clamped x0 + i, 0)
+  int64_t step;  // element distance between consecutive k
 
-// One wave accumulates a 32x32 tile C[r0.., c0..] over k in [k_begin, k_end)
-// stepping by k_stride (multiple of KSTEP).  acc[i][j] covers rows
-// r0+16i.., cols c0+16j...
-template <typename T, int MODE>
-__device__ __forceinline__ void wave_tile(typename Mma<T, MODE>::acc_t acc[2][2], int m, int n, int r0, int c0,
-                                          int k_begin, int k_end, int k_stride, const T* A, int lda, bool aT,
-                                          const T* B, int ldb, bool bT) {
-  using P = Mma<T, MODE>;
-  const int l = lane_id();
-  const int li = l & 15, lk = (l >> 4) * P::KPL;
-  const int ra0 = r0 + li, ra1 = r0 + 16 + li;
-  const int cb0 = c0 + li, cb1 = c0 + 16 + li;
-  const bool va0 = ra0 < m, va1 = ra1 < m, vb0 = cb0 < n, vb1 = cb1 < n;
-  // Full chunks of U k-steps: every fragment load of the chunk is issued
-  // before the first MFMA, so the loads overlap instead of paying one memory
-  // round trip per k-step.
-  constexpr int U = (MODE == MODE_BF16) ? 2 : 4;
-  int kk = k_begin;
-  for (; kk + (U - 1) * k_stride + P::KSTEP <= k_end; kk += U * k_stride) {
-    T a0[U][P::KPL], a1[U][P::KPL], b0[U][P::KPL], b1[U][P::KPL];
-#pragma unroll
-    for (int u = 0; u < U; ++u)
-#pragma unroll
-      for (int e = 0; e < P::KPL; ++e) {
-        const int kx = kk + u * k_stride + lk + e;
-        a0[u][e] = va0 ? opget(A, lda, aT, ra0, kx) : T(0);
-        a1[u][e] = va1 ? opget(A, lda, aT, ra1, kx) : T(0);
-        b0[u][e] = vb0 ? opget(B, ldb, !bT, cb0, kx) : T(0);
-        b1[u][e] = vb1 ? opget(B, ldb, !bT, cb1, kx) : T(0);
-      }
-#pragma unroll
-    for (int u = 0; u < U; ++u) {
-      acc[0][0] = P::mma(a0[u], b0[u], acc[0][0]);
-      acc[0][1] = P::mma(a0[u], b1[u], acc[0][1]);
-      acc[1][0] = P::mma(a1[u], b0[u], acc[1][0]);
-      acc[1][1] = P::mma(a1[u], b1[u], acc[1][1]);
-    }
+  __device__ __forceinline__ void init(const T* X, int ld, bool along_k, int x, int xmax) {
+    const int xs = x < xmax ? x : 0;
+    step = along_k ? 1 : ld;
+    p = along_k ? X + (int64_t)xs * ld : X + xs;
   }
-  for (; kk < k_end; kk += k_stride) {
-    T a0[P::KPL], a1[P::KPL], b0[P::KPL], b1[P::KPL];
-    if (kk + P::KSTEP <= k_end) {
+
+  // VEC: op(X) is k-contiguous with 16-byte aligned rows and k is a whole
+  // number of chunks.  Otherwise element loads with the k clamp.
+  template <bool VEC>
+  __device__ __forceinline__ void load(T (&f)[EPL], int kg, int kmax) const {
+    if constexpr (VEC) {
+      constexpr int V = 16 / sizeof(T);
+      typedef T vec_t __attribute__((ext_vector_type(V)));
 #pragma unroll
-      for (int e = 0; e < P::KPL; ++e) {
-        const int kx = kk + lk + e;
-        a0[e] = va0 ? opget(A, lda, aT, ra0, kx) : T(0);
-        a1[e] = va1 ? opget(A, lda, aT, ra1, kx) : T(0);
-        b0[e] = vb0 ? opget(B, ldb, !bT, cb0, kx) : T(0);
-        b1[e] = vb1 ? opget(B, ldb, !bT, cb1, kx) : T(0);
+      for (int h = 0; h < EPL / V; ++h) {
+        const vec_t v = gload<vec_t>(p + kg + h * V);
+#pragma unroll
+        for (int e = 0; e < V; ++e) f[h * V + e] = v[e];
       }
     } else {
 #pragma unroll
-      for (int e = 0; e < P::KPL; ++e) {
-        const int kx = kk + lk + e;
-        const bool vk = kx < k_end;
-        a0[e] = (va0 && vk) ? opget(A, lda, aT, ra0, kx) : T(0);
-        a1[e] = (va1 && vk) ? opget(A, lda, aT, ra1, kx) : T(0);
-        b0[e] = (vb0 && vk) ? opget(B, ldb, !bT, cb0, kx) : T(0);
-        b1[e] = (vb1 && vk) ? opget(B, ldb, !bT, cb1, kx) : T(0);
+      for (int e = 0; e < EPL; ++e) {
+        const int kk = kg + e;
+        f[e] = gload<T>(p + (int64_t)(kk < kmax ? kk : kmax - 1) * step);
       }
     }
-    acc[0][0] = P::mma(a0, b0, acc[0][0]);
-    acc[0][1] = P::mma(a0, b1, acc[0][1]);
-    acc[1][0] = P::mma(a1, b0, acc[1][0]);
-    acc[1][1] = P::mma(a1, b1, acc[1][1]);
+  }
+};
+
+// ---------------------------------------------------------------- gemm
+// Per-wave tile loop of gemm_block for one operand-load variant.
+template <typename T, int MODE, bool AV, bool BV>
+__device__ __forceinline__ void gemm_tiles(int m, int n, int k, T alpha, T beta, T* C, int ldc, const T* A, int lda,
+                                           bool a_along, const T* B, int ldb, bool b_along, int tiles_n, int grp,
+                                           int groups, int my_tiles, int ks, int gk, int my_chunks,
+                                           typename Mma<T, MODE>::acc_t (&acc)[2][2], int& r0, int& c0) {
+  using P = Mma<T, MODE>;
+  constexpr int KW = P::KW, EPL = P::EPL;
+  const int l = lane_id(), g = l >> 4, li = l & 15;
+  T fa[2][2][EPL], fb[2][2][EPL];  // [buffer][16-wide group][element]
+  int kgb[2] = {0, 0};              // first k of this lane in each buffer
+  OperandLane<T, EPL> la[2], lb[2];
+  // alpha is +-1 at every call site: C_new = alpha (alpha beta C + A B), so
+  // the accumulator starts at alpha beta C and the store multiplies by alpha
+  const T ab = alpha * beta;
+  auto setup_tile = [&](int t) {
+    r0 = (t / tiles_n) * 32;
+    c0 = (t % tiles_n) * 32;
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      la[h].init(A, lda, a_along, r0 + 16 * h + li, m);
+      lb[h].init(B, ldb, b_along, c0 + 16 * h + li, n);
+    }
+  };
+  auto load_chunk = [&](auto buf_c, int c) {
+    constexpr int buf = decltype(buf_c)::value;
+    const int kg = c * KW + EPL * g;
+    kgb[buf] = kg;
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      la[h].template load<AV>(fa[buf][h], kg, k);
+      lb[h].template load<BV>(fb[buf][h], kg, k);
+    }
+  };
+  // every active wave has at least one chunk (k > 0 and gk <= chunks)
+  const int chunks_eff = my_chunks;
+  const int total = my_tiles * chunks_eff;
+  if (total == 0) return;
+  int ti = 0, ci = 0;
+  setup_tile(grp);
+  load_chunk(IC<0>{}, ks);
+  auto step = [&](auto cur_c) {
+    constexpr int cur = decltype(cur_c)::value;
+    if (ci == 0) {
+      // the whole C tile is requested before any value is used (one memory
+      // round trip per tile); addresses are clamped as in OperandLane
+      T cv[2][2][4];
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            const int row = r0 + i * 16 + P::orow(l, r), col = c0 + j * 16 + li;
+            cv[i][j][r] = gload<T>(C + (int64_t)(row < m ? row : m - 1) * ldc + (col < n ? col : n - 1));
+          }
+      __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            const int row = r0 + i * 16 + P::orow(l, r), col = c0 + j * 16 + li;
+            const bool in = ks == 0 && beta != T(0) && row < m && col < n;
+            acc[i][j][r] = in ? ab * cv[i][j][r] : T(0);
+          }
+    }
+    int nti = ti, nci = ci + 1;
+    if (nci == chunks_eff) {
+      nci = 0;
+      ++nti;
+    }
+    const int cr0 = r0, cc0 = c0;
+    // prefetch the next (tile, chunk); past the last one, reload the last
+    // chunk so that every step issues the same loads
+    if (nti < my_tiles && nti != ti) setup_tile(grp + nti * groups);
+    load_chunk(IC<cur ^ 1>{}, ks + (nti < my_tiles ? nci : ci) * gk);
+    // every prefetch load is issued before the MFMAs of the current chunk
+    __builtin_amdgcn_sched_barrier(0);
+    if (!(AV && BV)) {  // k tail: cancel k >= kmax through A
+#pragma unroll
+      for (int h = 0; h < 2; ++h)
+#pragma unroll
+        for (int e = 0; e < EPL; ++e) fa[cur][h][e] = kgb[cur] + e < k ? fa[cur][h][e] : T(0);
+    }
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+      for (int j = 0; j < 2; ++j) acc[i][j] = P::mma_chunk(fa[cur][i], fb[cur][j], acc[i][j]);
+    if (ci == chunks_eff - 1 && gk == 1) {
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            const int row = cr0 + i * 16 + P::orow(l, r), col = cc0 + j * 16 + li;
+            if (row < m && col < n) gstore<T>(C + (int64_t)row * ldc + col, alpha * acc[i][j][r]);
+          }
+    }
+    ti = nti;
+    ci = nci;
+  };
+  for (int it = 0; it < total; it += 2) {
+    step(IC<0>{});
+    if (it + 1 < total) step(IC<1>{});
   }
 }
 
-// C = beta * C + alpha * op(A) op(B); C m x n.  beta == 0 never reads C.
-// `lds` must hold kLdsBytes.  Ends with a barrier.
+// C = beta * C + alpha * op(A) op(B); C m x n, op(A) m x k, op(B) k x n;
+// alpha must be +1 or -1.  With beta == 0 the old C is ignored (it is read,
+// so C must be allocated).  Uses LDS only for the split-K reduction.  Ends
+// with a barrier.
 template <typename T, int MODE>
-__device__ void gemm_block(int m, int n, int k, T alpha, T beta, T* C, int ldc, const T* A, int lda, bool aT,
-                           const T* B, int ldb, bool bT, void* lds) {
+__device__ __forceinline__ void gemm_block(int m, int n, int k, T alpha, T beta, T* C, int ldc, const T* A, int lda,
+                                           bool aT, const T* B, int ldb, bool bT) {
   using P = Mma<T, MODE>;
   using acc_t = typename P::acc_t;
+  constexpr int KW = P::KW, V = 16 / sizeof(T);
   const int nw = blockDim.x >> 6, w = wave_id(), l = lane_id();
+  const int li = l & 15;
   if (m <= 0 || n <= 0) {
     __syncthreads();
     return;
   }
-  // k == 0: C = beta * C
-  const bool small = (m <= 32 && n <= 32 && k >= 8 * P::KSTEP * nw);
-  if (small) {
-    // split K over waves, deterministic reduction through LDS
-    acc_t acc[2][2] = {{P::zero(), P::zero()}, {P::zero(), P::zero()}};
-    wave_tile<T, MODE>(acc, m, n, 0, 0, w * P::KSTEP, k, nw * P::KSTEP, A, lda, aT, B, ldb, bT);
-    T* red = reinterpret_cast<T*>(lds);  // [4 slots][64 lanes][16]
-    const int slot = w & 3;
-    for (int round = 0; round < (nw + 3) / 4; ++round) {
-      if ((w >> 2) == round) {
-        T* dst = red + ((size_t)slot * 64 + l) * 16;
+  const int tiles_n = (n + 31) >> 5, ntiles = ((m + 31) >> 5) * tiles_n;
+  if (k <= 0) {  // C = beta C
+    for (int e = threadIdx.x; e < m * n; e += blockDim.x) {
+      const int i = e / n, j = e - i * n;
+      T* c = C + (int64_t)i * ldc + j;
+      gstore<T>(c, beta == T(0) ? T(0) : beta * gload<T>(c));
+    }
+    __syncthreads();
+    return;
+  }
+  const int nch = (k + KW - 1) / KW;
+  int gk = 1;
+  while (2 * gk * ntiles <= nw && 2 * gk <= nch) gk *= 2;
+  const int groups = nw / gk, grp = w / gk, ks = w - grp * gk;
+  // this wave: tiles grp, grp + groups, ...; chunks ks, ks + gk, ...
+  const int my_tiles = grp < ntiles ? (ntiles - grp + groups - 1) / groups : 0;
+  const int my_chunks = ks < nch ? (nch - ks + gk - 1) / gk : 0;
+  const bool a_along = !aT, b_along = bT, full = (k % KW) == 0;
+  const bool a_vec = full && a_along && ((reinterpret_cast<uintptr_t>(A) & 15) == 0) && (lda % V) == 0;
+  const bool b_vec = full && b_along && ((reinterpret_cast<uintptr_t>(B) & 15) == 0) && (ldb % V) == 0;
+  acc_t acc[2][2];
+  int r0 = 0, c0 = 0;
+#define PP_GEMM_TILES(AV, BV)                                                                                      \
+  gemm_tiles<T, MODE, AV, BV>(m, n, k, alpha, beta, C, ldc, A, lda, a_along, B, ldb, b_along, tiles_n, grp, groups, \
+                              my_tiles, ks, gk, my_chunks, acc, r0, c0)
+  if (a_vec) {
+    if (b_vec) PP_GEMM_TILES(true, true);
+    else PP_GEMM_TILES(true, false);
+  } else {
+    if (b_vec) PP_GEMM_TILES(false, true);
+    else PP_GEMM_TILES(false, false);
+  }
+#undef PP_GEMM_TILES
+  if (gk > 1) {
+    // each wave holds at most one tile: partial sums of its K slice, added
+    // in wave order by the first wave of the group
+    PP_LDS T* red = (PP_LDS T*)lds_base();
+    if (my_tiles > 0) {
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+#pragma unroll
+          for (int r = 0; r < 4; ++r) red[((size_t)w * 64 + l) * 16 + (i * 2 + j) * 4 + r] = acc[i][j][r];
+    }
+    __syncthreads();
+    if (my_tiles > 0 && ks == 0) {
+      for (int q = 1; q < gk; ++q) {
+        const PP_LDS T* src = red + ((size_t)(w + q) * 64 + l) * 16;
 #pragma unroll
         for (int i = 0; i < 2; ++i)
 #pragma unroll
           for (int j = 0; j < 2; ++j)
 #pragma unroll
-            for (int r = 0; r < 4; ++r) {
-              const int q = (i * 2 + j) * 4 + r;
-              dst[q] = (round == 0) ? acc[i][j][r] : dst[q] + acc[i][j][r];
-            }
+            for (int r = 0; r < 4; ++r) acc[i][j][r] += src[(i * 2 + j) * 4 + r];
       }
-      __syncthreads();
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            const int row = r0 + i * 16 + P::orow(l, r), col = c0 + j * 16 + li;
+            if (row < m && col < n) gstore<T>(C + (int64_t)row * ldc + col, alpha * acc[i][j][r]);
+          }
     }
-    // 1024 outputs, each the fixed-order sum of up to 4 slots
-    const int slots = nw < 4 ? nw : 4;
-    for (int idx = threadIdx.x; idx < 64 * 16; idx += blockDim.x) {
-      const int ln = idx >> 4, q = idx & 15;
-      const int i = q >> 3, j = (q >> 2) & 1, r = q & 3;
-      const int row = i * 16 + P::orow(ln, r), col = j * 16 + (ln & 15);
-      if (row < m && col < n) {
-        T s = red[(size_t)ln * 16 + q];
-        for (int sl = 1; sl < slots; ++sl) s += red[((size_t)sl * 64 + ln) * 16 + q];
-        T* c = &C[(int64_t)row * ldc + col];
-        *c = (beta == T(0)) ? alpha * s : beta * *c + alpha * s;
-      }
-    }
-    __syncthreads();
-    return;
-  }
-
-  // wave grid: tall outputs stack waves vertically, wide ones horizontally
-  int wm, wn;
-  if (n <= 32) { wm = nw; wn = 1; }
-  else if (m <= 32) { wm = 1; wn = nw; }
-  else if (nw >= 16) { wm = 4; wn = nw / 4; }
-  else if (nw >= 4) { wm = 2; wn = nw / 2; }
-  else { wm = nw; wn = 1; }
-  const int tm = wm * 32, tn = wn * 32;
-  const int tiles_m = (m + tm - 1) / tm, tiles_n = (n + tn - 1) / tn;
-  const int my_m = w / wn, my_n = w % wn;
-  for (int t = 0; t < tiles_m * tiles_n; ++t) {
-    const int r0 = (t / tiles_n) * tm + my_m * 32, c0 = (t % tiles_n) * tn + my_n * 32;
-    if (r0 >= m || c0 >= n) continue;
-    // the old C tile is fetched before the K loop so its latency overlaps it
-    T cold[2][2][4];
-#pragma unroll
-    for (int i = 0; i < 2; ++i)
-#pragma unroll
-      for (int j = 0; j < 2; ++j)
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-          const int row = r0 + i * 16 + P::orow(l, r), col = c0 + j * 16 + (l & 15);
-          cold[i][j][r] = (beta != T(0) && row < m && col < n) ? C[(int64_t)row * ldc + col] : T(0);
-        }
-    acc_t acc[2][2] = {{P::zero(), P::zero()}, {P::zero(), P::zero()}};
-    wave_tile<T, MODE>(acc, m, n, r0, c0, 0, k, P::KSTEP, A, lda, aT, B, ldb, bT);
-#pragma unroll
-    for (int i = 0; i < 2; ++i)
-#pragma unroll
-      for (int j = 0; j < 2; ++j)
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-          const int row = r0 + i * 16 + P::orow(l, r), col = c0 + j * 16 + (l & 15);
-          if (row < m && col < n)
-            C[(int64_t)row * ldc + col] = beta * cold[i][j][r] + alpha * acc[i][j][r];
-        }
   }
   __syncthreads();
 }

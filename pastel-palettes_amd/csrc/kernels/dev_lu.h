@@ -14,240 +14,236 @@
 //           (blockLowRankAccum, src/kernel.cu:1255-1280).  The reference
 //           orthonormalises with Givens rotations; Householder is used here.
 //           Q Q^T depends only on range(Y), so the product U1 V1^T is the same.
-// Blocking is by panels of PB columns; the panel's updates run on MFMA via
-// gemm_block.  All routines end with a workgroup barrier.
+//
+// Organisation (MI355X): everything is blocked by NB = 32.
+//   * getrf: right-looking.  The 32-column panel is eliminated with one row
+//     per thread (pivot row published through double-buffered LDS, one
+//     barrier per column); the same row operations applied to the identity
+//     give L11^{-1} for free.  U12 = L11^{-1} A12 and the trailing update
+//     A22 -= L21 U12 run on MFMA through gemm_block.
+//   * trsm*: the inverses of the 32 x 32 diagonal blocks of the triangular
+//     factor are formed first (one wave per block, column-wise substitution),
+//     then the solve is a sequence of block products X_i = inv_i B_i and
+//     block updates, all on MFMA.
+// The diagonal-block inverses live in the worker's scratch (`tmp`, at least
+// ceil(n / 32) * 32 * 32 elements).  All routines end with a barrier.
 #pragma once
 
 #include "dev_gemm.h"
-#include "dev_leaf.h"
 
 namespace pp {
 
-constexpr int PB = 16;  // panel width
+constexpr int NB = 32;  // panel width and diagonal block size
+
+// Column c of the inverse of the nb x nb triangular block held in LDS (pitch
+// NB + 1): unit lower (UPPER = false) or non-unit upper.  Rows >= nb are 0.
+template <typename T, bool UPPER>
+__device__ __forceinline__ void tri_inverse_column(const PP_LDS T* s, int nb, int c, T y[NB]) {
+  // The empty asm statements (which consume each row's result and clobber
+  // memory) keep the uniform LDS reads of one row next to their use; hoisted
+  // across rows they would need ~NB^2/2 live registers.
+  if (!UPPER) {
+#pragma unroll
+    for (int i = 0; i < NB; ++i) {
+      T acc = (i == c) ? T(1) : T(0);
+#pragma unroll
+      for (int j = 0; j < i; ++j) acc -= s[i * (NB + 1) + j] * y[j];
+      asm volatile("" : "+v"(acc)::"memory");
+      y[i] = (i < nb) ? acc : T(0);
+    }
+  } else {
+#pragma unroll
+    for (int i = NB - 1; i >= 0; --i) {
+      T acc = (i == c) ? T(1) : T(0);
+#pragma unroll
+      for (int j = i + 1; j < NB; ++j) acc -= s[i * (NB + 1) + j] * y[j];
+      asm volatile("" : "+v"(acc)::"memory");
+      y[i] = (i < nb) ? acc / s[i * (NB + 1) + i] : T(0);
+    }
+  }
+}
+
+// Inverses of the NB x NB diagonal blocks of the n x n triangular matrix F
+// (pitch ldf); block b is written row-major (pitch NB) at out + b NB NB.
+// One wave per block; uses NB (NB + 1) elements of LDS per wave.
+template <typename T, bool UPPER>
+__device__ __forceinline__ void diag_inverses(const T* F, int ldf, int n, T* out) {
+  const int nw = blockDim.x >> 6, w = wave_id(), l = lane_id();
+  const int nblk = (n + NB - 1) / NB;
+  PP_LDS T* s = (PP_LDS T*)lds_base() + w * NB * (NB + 1);
+  for (int b0 = 0; b0 < nblk; b0 += nw) {
+    const int b = b0 + w;
+    const int r0 = b * NB;
+    const int nb = b < nblk ? ((n - r0) < NB ? (n - r0) : NB) : 0;
+    if (b < nblk) {
+      for (int e = l; e < NB * NB; e += 64) {
+        const int i = e / NB, j = e - i * NB;
+        s[i * (NB + 1) + j] = (i < nb && j < nb) ? F[(int64_t)(r0 + i) * ldf + r0 + j] : T(0);
+      }
+    }
+    __syncthreads();
+    if (b < nblk && l < nb) {
+      T y[NB];
+      tri_inverse_column<T, UPPER>(s, nb, l, y);
+      T* o = out + (int64_t)b * NB * NB;
+#pragma unroll
+      for (int i = 0; i < NB; ++i)
+        if (i < nb) o[i * NB + l] = y[i];
+    }
+    __syncthreads();
+  }
+}
 
 // ---------------------------------------------------------------- getrf
-// M is ny x nx, pitch ld; requires ny <= 2 * blockDim.x.
-template <typename T, int FMODE>
-__device__ void getrf_block(T* M, int nx, int ny, int ld, void* lds) {
-  if (getrf_lds<T>(M, nx, ny, ld, lds, kLdsBytes)) return;  // whole block fits in LDS
-  T* piv = reinterpret_cast<T*>(lds);            // [PB] current pivot row
-  T* l11 = piv + PB;                             // [PB][PB] diagonal block
-  void* glds = reinterpret_cast<char*>(lds) + 8 * 1024;  // gemm scratch after 8 KB
-  const int tid = threadIdx.x, nt = blockDim.x;
-  const int mn = nx < ny ? nx : ny;
-  for (int p = 0; p < mn; p += PB) {
-    const int pw = (mn - p) < PB ? (mn - p) : PB;
-    // ---- panel M[p:ny, p:p+pw]: thread owns rows p + tid and p + tid + nt
-    T row0[PB], row1[PB];
-    const int i0 = p + tid, i1 = p + tid + nt;
-    const bool own0 = i0 < ny, own1 = i1 < ny;
-#pragma unroll
-    for (int j = 0; j < PB; ++j) {
-      row0[j] = (own0 && j < pw) ? M[(int64_t)i0 * ld + p + j] : T(0);
-      row1[j] = (own1 && j < pw) ? M[(int64_t)i1 * ld + p + j] : T(0);
-    }
-    // fully unrolled so that row0/row1 stay in registers
-#pragma unroll
-    for (int j = 0; j < PB; ++j) {
-      if (j < pw) {
-        // owner of pivot row p + j publishes it (rows < nt + p are in row0)
-        if (tid == j) {
-#pragma unroll
-          for (int c = 0; c < PB; ++c) piv[c] = row0[c];
-        }
-        __syncthreads();
-        const T d = piv[j];
-        if (own0 && i0 > p + j) {
-          const T lv = row0[j] / d;
-          row0[j] = lv;
-#pragma unroll
-          for (int c = j + 1; c < PB; ++c) row0[c] -= lv * piv[c];
-        }
-        if (own1) {
-          const T lv = row1[j] / d;
-          row1[j] = lv;
-#pragma unroll
-          for (int c = j + 1; c < PB; ++c) row1[c] -= lv * piv[c];
-        }
-        __syncthreads();
-      }
-    }
-#pragma unroll
-    for (int j = 0; j < PB; ++j) {
-      if (j < pw) {
-        if (own0) M[(int64_t)i0 * ld + p + j] = row0[j];
-        if (own1) M[(int64_t)i1 * ld + p + j] = row1[j];
-      }
-    }
-    if (tid < pw) {
-#pragma unroll
-      for (int c = 0; c < PB; ++c) l11[tid * PB + c] = row0[c];
-    }
-    __syncthreads();
-    // ---- U12 = L11^{-1} M[p:p+pw, p+pw:nx]
-    for (int c = p + pw + tid; c < nx; c += nt) {
-      T x[PB];
-#pragma unroll
-      for (int i = 0; i < PB; ++i) x[i] = (i < pw) ? M[(int64_t)(p + i) * ld + c] : T(0);
-#pragma unroll
-      for (int i = 0; i < PB; ++i)
-#pragma unroll
-        for (int j = 0; j < PB; ++j)
-          if (j < i) x[i] -= l11[i * PB + j] * x[j];
-#pragma unroll
-      for (int i = 0; i < PB; ++i)
-        if (i < pw) M[(int64_t)(p + i) * ld + c] = x[i];
-    }
-    __syncthreads();
-    // ---- trailing update
-    const int rm = ny - p - pw, rn = nx - p - pw;
-    if (rm > 0 && rn > 0)
-      gemm_block<T, FMODE>(rm, rn, pw, T(-1), T(1), &M[(int64_t)(p + pw) * ld + p + pw], ld,
-                           &M[(int64_t)(p + pw) * ld + p], ld, false, &M[(int64_t)p * ld + p + pw], ld, false, glds);
-  }
-  __syncthreads();
-}
-
-// ---------------------------------------------------------------- triangular solves
-// Stage the PB x PB block T[r0.., c0..] (pitch ld) into lds (row-major PB).
+// Unblocked LU of the panel M[p:ny, p:p+pw] (pw <= NB) in place.  Thread t
+// owns row p + t; the pivot row of step j is published by its owner into one
+// of two LDS buffers, so one barrier per column suffices (the buffer written
+// at step j + 2 was last read before every thread passed barrier j + 1).  The
+// top pw rows also carry the identity through the elimination, which leaves
+// L11^{-1} there; it is written row-major (pitch NB) to linv.  Rows beyond
+// blockDim.x are eliminated afterwards against the final pivot rows.
 template <typename T>
-__device__ __forceinline__ void stage_block(T* dst, const T* src, int ld, int r0, int c0, int pr, int pc) {
-  for (int e = threadIdx.x; e < PB * PB; e += blockDim.x) {
-    const int i = e / PB, j = e % PB;
-    dst[e] = (i < pr && j < pc) ? src[(int64_t)(r0 + i) * ld + c0 + j] : T(0);
+__device__ __forceinline__ void panel_lu(T* M, int ld, int p, int pw, int ny, T* linv) {
+  PP_LDS T* piv = (PP_LDS T*)lds_base();  // [2][2 NB]
+  const int tid = threadIdx.x, nt = blockDim.x;
+  const int R = ny - p;
+  const bool own = tid < R;
+  T* row = M + (int64_t)(p + tid) * ld + p;
+  T x[NB], e[NB];
+#pragma unroll
+  for (int c = 0; c < NB; ++c) {
+    x[c] = (own && c < pw) ? row[c] : T(0);
+    e[c] = (c == tid) ? T(1) : T(0);
   }
-}
-
-// B (ny_b x nx_b logical) <- L^{-1} B, L unit lower ny_b x ny_b.
-template <typename T, int FMODE>
-__device__ void trsml_block(T* B, const T* L, int nx_b, int ny_b, int ld_b, int ld_l, bool b_T, void* lds) {
-  if (!b_T && trsm_rows<T, 0>(B, L, ny_b, nx_b, ld_b, ld_l, lds, kLdsBytes)) return;
-  T* lb = reinterpret_cast<T*>(lds);
-  void* glds = reinterpret_cast<char*>(lds) + 8 * 1024;
-  for (int p = 0; p < ny_b; p += PB) {
-    const int pw = (ny_b - p) < PB ? (ny_b - p) : PB;
-    stage_block(lb, L, ld_l, p, p, pw, pw);
-    __syncthreads();
-    // columns of the logical B are independent
-    for (int c = threadIdx.x; c < nx_b; c += blockDim.x) {
-      T x[PB];
 #pragma unroll
-      for (int i = 0; i < PB; ++i)
-        x[i] = (i < pw) ? (b_T ? B[(int64_t)c * ld_b + p + i] : B[(int64_t)(p + i) * ld_b + c]) : T(0);
+  for (int j = 0; j < NB; ++j) {
+    if (j < pw) {
+      PP_LDS T* pv = piv + (j & 1) * 2 * NB;
+      if (tid == j) {
 #pragma unroll
-      for (int i = 0; i < PB; ++i)
+        for (int c = j; c < NB; ++c) pv[c] = x[c];
 #pragma unroll
-        for (int j = 0; j < PB; ++j)
-          if (j < i) x[i] -= lb[i * PB + j] * x[j];
+        for (int c = 0; c <= j; ++c) pv[NB + c] = e[c];
+      }
+      __syncthreads();
+      if (own && tid > j) {
+        const T lv = x[j] / pv[j];
+        x[j] = lv;
 #pragma unroll
-      for (int i = 0; i < PB; ++i)
-        if (i < pw) {
-          if (b_T) B[(int64_t)c * ld_b + p + i] = x[i];
-          else B[(int64_t)(p + i) * ld_b + c] = x[i];
-        }
-    }
-    __syncthreads();
-    const int rm = ny_b - p - pw;
-    if (rm > 0) {
-      if (!b_T)  // B[p+pw:, :] -= L[p+pw:, p:p+pw] B[p:p+pw, :]
-        gemm_block<T, FMODE>(rm, nx_b, pw, T(-1), T(1), &B[(int64_t)(p + pw) * ld_b], ld_b,
-                             &L[(int64_t)(p + pw) * ld_l + p], ld_l, false, &B[(int64_t)p * ld_b], ld_b, false, glds);
-      else  // Bs[:, p+pw:] -= Bs[:, p:p+pw] L[p+pw:, p:p+pw]^T
-        gemm_block<T, FMODE>(nx_b, rm, pw, T(-1), T(1), &B[p + pw], ld_b, &B[p], ld_b, false,
-                             &L[(int64_t)(p + pw) * ld_l + p], ld_l, true, glds);
-    }
-  }
-  __syncthreads();
-}
-
-// B (ny_b x nx_b logical) <- B U^{-1}, U upper non-unit nx_b x nx_b.
-template <typename T, int FMODE>
-__device__ void trsmr_block(T* B, const T* U, int nx_b, int ny_b, int ld_b, int ld_u, bool b_T, void* lds) {
-  // transposed B (the V factor of a low-rank block): U^T Bs_new = Bs
-  if (b_T && trsm_rows<T, 1>(B, U, nx_b, ny_b, ld_b, ld_u, lds, kLdsBytes)) return;
-  T* ub = reinterpret_cast<T*>(lds);
-  void* glds = reinterpret_cast<char*>(lds) + 8 * 1024;
-  for (int p = 0; p < nx_b; p += PB) {
-    const int pw = (nx_b - p) < PB ? (nx_b - p) : PB;
-    stage_block(ub, U, ld_u, p, p, pw, pw);
-    __syncthreads();
-    // rows of the logical B are independent: x U11 = b
-    for (int r = threadIdx.x; r < ny_b; r += blockDim.x) {
-      T x[PB];
+        for (int c = j + 1; c < NB; ++c) x[c] -= lv * pv[c];
+        if (tid < pw) {
 #pragma unroll
-      for (int j = 0; j < PB; ++j)
-        x[j] = (j < pw) ? (b_T ? B[(int64_t)(p + j) * ld_b + r] : B[(int64_t)r * ld_b + p + j]) : T(1);
-#pragma unroll
-      for (int j = 0; j < PB; ++j) {
-        if (j < pw) {
-#pragma unroll
-          for (int i = 0; i < PB; ++i)
-            if (i < j) x[j] -= x[i] * ub[i * PB + j];
-          x[j] = x[j] / ub[j * PB + j];
+          for (int c = 0; c <= j; ++c) e[c] -= lv * pv[NB + c];
         }
       }
+    }
+  }
+  if (own) {
 #pragma unroll
-      for (int j = 0; j < PB; ++j)
-        if (j < pw) {
-          if (b_T) B[(int64_t)(p + j) * ld_b + r] = x[j];
-          else B[(int64_t)r * ld_b + p + j] = x[j];
-        }
+    for (int c = 0; c < NB; ++c)
+      if (c < pw) row[c] = x[c];
+  }
+  if (tid < pw) {
+#pragma unroll
+    for (int c = 0; c < NB; ++c)
+      if (c < pw) linv[tid * NB + c] = e[c];
+  }
+  __syncthreads();
+  for (int r = nt + tid; r < R; r += nt) {
+    T* rr = M + (int64_t)(p + r) * ld + p;
+#pragma unroll
+    for (int c = 0; c < NB; ++c) x[c] = c < pw ? rr[c] : T(0);
+#pragma unroll
+    for (int j = 0; j < NB; ++j) {
+      if (j < pw) {
+        const T* pr = M + (int64_t)(p + j) * ld + p;
+        const T lv = x[j] / pr[j];
+        x[j] = lv;
+#pragma unroll
+        for (int c = j + 1; c < NB; ++c)
+          if (c < pw) x[c] -= lv * pr[c];
+      }
     }
-    __syncthreads();
-    const int rn = nx_b - p - pw;
-    if (rn > 0) {
-      if (!b_T)  // B[:, p+pw:] -= B[:, p:p+pw] U[p:p+pw, p+pw:]
-        gemm_block<T, FMODE>(ny_b, rn, pw, T(-1), T(1), &B[p + pw], ld_b, &B[p], ld_b, false,
-                             &U[(int64_t)p * ld_u + p + pw], ld_u, false, glds);
-      else  // Bs[p+pw:, :] -= U[p:p+pw, p+pw:]^T Bs[p:p+pw, :]
-        gemm_block<T, FMODE>(rn, ny_b, pw, T(-1), T(1), &B[(int64_t)(p + pw) * ld_b], ld_b,
-                             &U[(int64_t)p * ld_u + p + pw], ld_u, true, &B[(int64_t)p * ld_b], ld_b, false, glds);
-    }
+#pragma unroll
+    for (int c = 0; c < NB; ++c)
+      if (c < pw) rr[c] = x[c];
   }
   __syncthreads();
 }
 
-// B (ny_b x nx_b) <- U^{-1} B, U upper non-unit ny_b x ny_b (backward).
-template <typename T, int FMODE>
-__device__ void trsmu_block(T* B, const T* U, int nx_b, int ny_b, int ld_b, int ld_u, void* lds) {
-  if (trsm_rows<T, 2>(B, U, ny_b, nx_b, ld_b, ld_u, lds, kLdsBytes)) return;
-  T* ub = reinterpret_cast<T*>(lds);
-  void* glds = reinterpret_cast<char*>(lds) + 8 * 1024;
-  const int nblk = (ny_b + PB - 1) / PB;
-  for (int bi = nblk - 1; bi >= 0; --bi) {
-    const int p = bi * PB;
-    const int pw = (ny_b - p) < PB ? (ny_b - p) : PB;
-    stage_block(ub, U, ld_u, p, p, pw, pw);
-    __syncthreads();
-    for (int c = threadIdx.x; c < nx_b; c += blockDim.x) {
-      T x[PB];
-#pragma unroll
-      for (int i = 0; i < PB; ++i) x[i] = (i < pw) ? B[(int64_t)(p + i) * ld_b + c] : T(0);
-#pragma unroll
-      for (int ii = PB - 1; ii >= 0; --ii) {
-        if (ii < pw) {
-#pragma unroll
-          for (int j = 0; j < PB; ++j)
-            if (j > ii && j < pw) x[ii] -= ub[ii * PB + j] * x[j];
-          x[ii] = x[ii] / ub[ii * PB + ii];
-        }
-      }
-#pragma unroll
-      for (int i = 0; i < PB; ++i)
-        if (i < pw) B[(int64_t)(p + i) * ld_b + c] = x[i];
+// ---------------------------------------------------------------- Cholesky
+// Control word in the LDS arena (past every routine's working area): set to 1
+// by chol_upper when the factorisation failed even with the shift.
+__device__ __forceinline__ volatile PP_LDS int* chol_fail_flag() {
+  return (volatile PP_LDS int*)(lds_base() + kLdsBytes - 16);
+}
+
+// G (k x k, pitch k, symmetric positive semi-definite; k <= 64) <- R, the
+// upper Cholesky factor of G.  first: the first factorisation of an accum;
+// when its plain factorisation meets a non-positive pivot, G + s I is
+// factorised instead, s = 11 (rows k + k (k + 1)) u tr(G) (the shift of
+// shifted CholeskyQR3, Fukaya et al., SIAM J. Sci. Comput. 42 (2020)); the
+// failure flag is reset there.  Later factorisations are never shifted: a
+// failure sets the flag (the caller then orthonormalises by Householder).
+// Entries below the diagonal are set to zero.
+template <typename T>
+__device__ __forceinline__ void chol_upper(T* G, int k, int rows, bool first) {
+  PP_LDS T* s = (PP_LDS T*)lds_base();  // k x (k + 1)
+  const int sp = k + 1, tid = threadIdx.x, nt = blockDim.x;
+  const T u = sizeof(T) == 8 ? T(1.1102230246251565e-16) : T(5.9604644775390625e-08);
+  bool ok = false;
+  for (int attempt = 0; attempt < (first ? 2 : 1) && !ok; ++attempt) {
+    for (int e = tid; e < k * k; e += nt) {
+      const int i = e / k, j = e - i * k;
+      s[i * sp + j] = G[e];
     }
     __syncthreads();
-    if (p > 0)  // B[0:p, :] -= U[0:p, p:p+pw] B[p:p+pw, :]
-      gemm_block<T, FMODE>(p, nx_b, pw, T(-1), T(1), B, ld_b, &U[p], ld_u, false, &B[(int64_t)p * ld_b], ld_b, false,
-                           glds);
+    if (attempt == 1) {
+      T tr = T(0);
+      for (int i = 0; i < k; ++i) tr += s[i * sp + i];
+      const T shift = T(11) * (T(rows) * T(k) + T(k) * T(k + 1)) * u * tr;
+      __syncthreads();
+      for (int i = tid; i < k; i += nt) s[i * sp + i] += shift;
+      __syncthreads();
+    }
+    ok = true;
+    for (int j = 0; j < k; ++j) {
+      const T d = s[j * sp + j];  // every thread reads the same pivot
+      if (!(d > T(0)) || !(d < T(__builtin_inf()))) {
+        ok = false;
+        break;
+      }
+      const T r = sqrt(d), ri = T(1) / r;
+      __syncthreads();
+      for (int c = j + tid; c < k; c += nt) s[j * sp + c] = c == j ? r : s[j * sp + c] * ri;
+      __syncthreads();
+      const int m = k - j - 1;  // trailing upper triangle, row-major over (a, b)
+      for (int e = tid; e < m * m; e += nt) {
+        const int a = j + 1 + e / m, b = j + 1 + e % m;
+        if (b >= a) s[a * sp + b] -= s[j * sp + a] * s[j * sp + b];
+      }
+      __syncthreads();
+    }
+    __syncthreads();
+  }
+  for (int e = tid; e < k * k; e += nt) {
+    const int i = e / k, j = e - i * k;
+    G[e] = j >= i ? s[i * sp + j] : T(0);
+  }
+  if (tid == 0) {
+    if (first) *chol_fail_flag() = ok ? 0 : 1;
+    else if (!ok) *chol_fail_flag() = 1;
   }
   __syncthreads();
 }
+
+// Scratch elements the routines above need for an n x n factor.
+__host__ __device__ inline int64_t tri_scratch(int n) { return (int64_t)((n + NB - 1) / NB) * NB * NB; }
 
 // ---------------------------------------------------------------- reductions
 // Block-wide sum of one value per thread (deterministic fixed tree order).
 template <typename T>
-__device__ T block_sum(T v, T* red) {
+__device__ T block_sum(T v, PP_LDS T* red) {
   red[threadIdx.x] = v;
   __syncthreads();
   for (int s = blockDim.x / 2; s > 0; s >>= 1) {
@@ -262,7 +258,7 @@ __device__ T block_sum(T v, T* red) {
 // w[c] = sum_{i=r0}^{ny-1} v[i] * Y[i][c] for c in [c0, kc), v = column j of Y
 // with implicit v[r0] = 1.  Partial sums over row groups, fixed-order reduce.
 template <typename T>
-__device__ void reflector_dots(const T* Y, int ld, int ny, int r0, int jcol, int c0, int kc, T* w, T* part) {
+__device__ void reflector_dots(const T* Y, int ld, int ny, int r0, int jcol, int c0, int kc, T* w, PP_LDS T* part) {
   const int nt = blockDim.x;
   const int ncol = kc - c0;
   if (ncol <= 0) {
@@ -292,9 +288,9 @@ __device__ void reflector_dots(const T* Y, int ld, int ny, int r0, int jcol, int
 // reflector tails below the diagonal (v[j] = 1 implicit), tau[k].
 // Then Q (ny x k, pitch ldq) = H_0 ... H_{k-1} [I; 0].
 template <typename T>
-__device__ void householder_qr_q(T* Y, int ld, int ny, int k, T* Q, int ldq, T* tau, T* w, T* red) {
+__device__ __forceinline__ void householder_qr_q(T* Y, int ld, int ny, int k, T* Q, int ldq, T* tau, T* w, PP_LDS T* red) {
   const int nt = blockDim.x;
-  T* part = red;  // blockDim.x scratch
+  PP_LDS T* part = red;  // blockDim.x scratch
   for (int j = 0; j < k && j < ny; ++j) {
     // norm of Y[j+1:, j]
     T s = T(0);
@@ -367,45 +363,6 @@ __device__ void householder_qr_q(T* Y, int ld, int ny, int k, T* Q, int ldq, T* 
     }
     __syncthreads();
   }
-}
-
-// ---------------------------------------------------------------- accum
-// U1 (ny x k1), V1 (nx x k1), U2 (ny x k2), V2 (nx x k2); scratch layout:
-// Y [0, off1), Vn [off1, off2), Q [off2, ...) then small panels.
-template <typename T, int GMODE>
-__device__ void accum_block(T* U1, T* V1, const T* U2, const T* V2, int nx, int ny, int k1, int k2, int ld_u1,
-                            int ld_v1, int ld_u2, int ld_v2, int off1, int off2, T* tmp, const T* rnd, void* lds) {
-  T* Y = tmp;
-  T* Vn = tmp + off1;
-  T* Q = tmp + off2;
-  const int kmax = k1 > k2 ? k1 : k2;
-  T* small = Q + (((int64_t)ny * k1 + 31) / 32) * 32;
-  T* S = small;                    // k1 x kmax
-  T* tau = small + 4 * k1 * kmax;  // kmax
-  T* w = tau + kmax;               // kmax
-  T* red = reinterpret_cast<T*>(reinterpret_cast<char*>(lds) + 40 * 1024);  // blockDim.x
-  // Y = U1 (Omega^T V1)^T + U2 (Omega^T V2)^T
-  gemm_block<T, GMODE>(k1, k1, nx, T(1), T(0), S, k1, rnd, k1, true, V1, ld_v1, false, lds);
-  gemm_block<T, GMODE>(ny, k1, k1, T(1), T(0), Y, k1, U1, ld_u1, false, S, k1, true, lds);
-  gemm_block<T, GMODE>(k1, k2, nx, T(1), T(0), S, k2, rnd, k1, true, V2, ld_v2, false, lds);
-  gemm_block<T, GMODE>(ny, k1, k2, T(1), T(1), Y, k1, U2, ld_u2, false, S, k2, true, lds);
-  // orthonormal basis of range(Y)
-  householder_qr_q<T>(Y, k1, ny, k1, Q, k1, tau, w, red);
-  // Vn = V1 (Q^T U1)^T + V2 (Q^T U2)^T
-  gemm_block<T, GMODE>(k1, k1, ny, T(1), T(0), S, k1, Q, k1, true, U1, ld_u1, false, lds);
-  gemm_block<T, GMODE>(nx, k1, k1, T(1), T(0), Vn, k1, V1, ld_v1, false, S, k1, true, lds);
-  gemm_block<T, GMODE>(k1, k2, ny, T(1), T(0), S, k2, Q, k1, true, U2, ld_u2, false, lds);
-  gemm_block<T, GMODE>(nx, k1, k2, T(1), T(1), Vn, k1, V2, ld_v2, false, S, k2, true, lds);
-  // U1 <- Q, V1 <- Vn
-  for (int e = threadIdx.x; e < ny * k1; e += blockDim.x) {
-    const int i = e / k1, c = e % k1;
-    U1[(int64_t)i * ld_u1 + c] = Q[e];
-  }
-  for (int e = threadIdx.x; e < nx * k1; e += blockDim.x) {
-    const int i = e / k1, c = e % k1;
-    V1[(int64_t)i * ld_v1 + c] = Vn[e];
-  }
-  __syncthreads();
 }
 
 }  // namespace pp

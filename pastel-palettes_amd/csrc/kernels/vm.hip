@@ -3,19 +3,11 @@
 // norms).
 #include <hip/hip_runtime.h>
 
-#include "../h_ops/h_ops.h"
-#include "dev_lu.h"
+#include "dev_steps.h"
 #include "kernels.h"
 #include "vm.h"
 
 namespace pp {
-
-__device__ __forceinline__ int32_t word(const int32_t* pc, int i) { return __builtin_amdgcn_readfirstlane(pc[i]); }
-
-template <typename T>
-__device__ __forceinline__ T* slot(void* const* ptrs, const int32_t* pc, int i) {
-  return reinterpret_cast<T*>(ptrs[word(pc, i)]);
-}
 
 __device__ void record_error(int32_t* status, int32_t code, int32_t worker, int32_t inst) {
   if (atomicCAS(&status[0], VM_OK, code) == VM_OK) {
@@ -25,123 +17,8 @@ __device__ void record_error(int32_t* status, int32_t code, int32_t worker, int3
   __hip_atomic_store(&status[3], 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// Executes one execute entry; returns its length in words or -1.
-template <typename T, int FMODE, int GMODE>
-__device__ int exec_entry(const int32_t* pc, const VmArgs& a, T* tmp, void* lds) {
-  void* const* P = a.ptrs;
-  switch (word(pc, 2)) {
-    case nop:
-      return nop_l;
-    case getrf: {
-      T* M = slot<T>(P, pc, 3) + word(pc, 4);
-      getrf_block<T, FMODE>(M, word(pc, 5), word(pc, 6), word(pc, 7), lds);
-      return getrf_l;
-    }
-    case trsml: {
-      T* B = slot<T>(P, pc, 3) + word(pc, 5);
-      const T* L = slot<T>(P, pc, 4) + word(pc, 6);
-      trsml_block<T, FMODE>(B, L, word(pc, 7), word(pc, 8), word(pc, 10), word(pc, 11), word(pc, 12) != 0, lds);
-      return trsml_l;
-    }
-    case trsmr: {
-      T* B = slot<T>(P, pc, 3) + word(pc, 5);
-      const T* U = slot<T>(P, pc, 4) + word(pc, 6);
-      trsmr_block<T, FMODE>(B, U, word(pc, 7), word(pc, 8), word(pc, 10), word(pc, 11), word(pc, 12) != 0, lds);
-      return trsmr_l;
-    }
-    case trsmu: {
-      T* B = slot<T>(P, pc, 3) + word(pc, 5);
-      const T* U = slot<T>(P, pc, 4) + word(pc, 6);
-      trsmu_block<T, FMODE>(B, U, word(pc, 7), word(pc, 8), word(pc, 10), word(pc, 11), lds);
-      return trsmu_l;
-    }
-    case gemm:
-    case gemm_plus: {
-      T* M = slot<T>(P, pc, 3) + word(pc, 6);
-      const T* A = slot<T>(P, pc, 4) + word(pc, 7);
-      const T* B = slot<T>(P, pc, 5) + word(pc, 8);
-      const T alpha = word(pc, 2) == gemm ? T(-1) : T(1);
-      gemm_block<T, GMODE>(word(pc, 9), word(pc, 10), word(pc, 11), alpha, T(1), M, word(pc, 12), A, word(pc, 13),
-                           word(pc, 15) != 0, B, word(pc, 14), word(pc, 16) != 0, lds);
-      return gemm_l;
-    }
-    case gemm_3x: {
-      T* M = slot<T>(P, pc, 3) + word(pc, 7);
-      const T* A = slot<T>(P, pc, 4) + word(pc, 8);
-      const T* B = slot<T>(P, pc, 5) + word(pc, 9);
-      const T* C = slot<T>(P, pc, 6) + word(pc, 10);
-      const int m = word(pc, 11), n = word(pc, 12), k = word(pc, 13), l = word(pc, 14);
-      const int ldm = word(pc, 15), lda = word(pc, 16), ldb = word(pc, 17), ldc = word(pc, 18);
-      const bool aT = word(pc, 19) != 0, bT = word(pc, 20) != 0, cT = word(pc, 21) != 0;
-      if (word(pc, 22)) {  // (A B) C
-        gemm_block<T, GMODE>(m, l, k, T(1), T(0), tmp, l, A, lda, aT, B, ldb, bT, lds);
-        gemm_block<T, GMODE>(m, n, l, T(-1), T(1), M, ldm, tmp, l, false, C, ldc, cT, lds);
-      } else {  // A (B C)
-        gemm_block<T, GMODE>(k, n, l, T(1), T(0), tmp, n, B, ldb, bT, C, ldc, cT, lds);
-        gemm_block<T, GMODE>(m, n, k, T(-1), T(1), M, ldm, A, lda, aT, tmp, n, false, lds);
-      }
-      return gemm_3x_l;
-    }
-    case gemm_4x: {
-      T* M = slot<T>(P, pc, 3) + word(pc, 8);
-      const T* A = slot<T>(P, pc, 4) + word(pc, 9);
-      const T* B = slot<T>(P, pc, 5) + word(pc, 10);
-      const T* C = slot<T>(P, pc, 6) + word(pc, 11);
-      const T* D = slot<T>(P, pc, 7) + word(pc, 12);
-      const int m = word(pc, 13), n = word(pc, 14), k = word(pc, 15), l = word(pc, 16), o = word(pc, 17);
-      const int ldm = word(pc, 18), lda = word(pc, 19), ldb = word(pc, 20), ldc = word(pc, 21), ldd = word(pc, 22);
-      const bool aT = word(pc, 23) != 0, bT = word(pc, 24) != 0, cT = word(pc, 25) != 0, dT = word(pc, 26) != 0;
-      T* t1 = tmp;
-      T* t2 = tmp + word(pc, 28);
-      switch (word(pc, 27)) {
-        case 0:  // ((A B) C) D
-          gemm_block<T, GMODE>(m, l, k, T(1), T(0), t1, l, A, lda, aT, B, ldb, bT, lds);
-          gemm_block<T, GMODE>(m, o, l, T(1), T(0), t2, o, t1, l, false, C, ldc, cT, lds);
-          gemm_block<T, GMODE>(m, n, o, T(-1), T(1), M, ldm, t2, o, false, D, ldd, dT, lds);
-          break;
-        case 1:  // (A (B C)) D
-          gemm_block<T, GMODE>(k, o, l, T(1), T(0), t1, o, B, ldb, bT, C, ldc, cT, lds);
-          gemm_block<T, GMODE>(m, o, k, T(1), T(0), t2, o, A, lda, aT, t1, o, false, lds);
-          gemm_block<T, GMODE>(m, n, o, T(-1), T(1), M, ldm, t2, o, false, D, ldd, dT, lds);
-          break;
-        case 2:  // A ((B C) D)
-          gemm_block<T, GMODE>(k, o, l, T(1), T(0), t1, o, B, ldb, bT, C, ldc, cT, lds);
-          gemm_block<T, GMODE>(k, n, o, T(1), T(0), t2, n, t1, o, false, D, ldd, dT, lds);
-          gemm_block<T, GMODE>(m, n, k, T(-1), T(1), M, ldm, A, lda, aT, t2, n, false, lds);
-          break;
-        case 3:  // A (B (C D))
-          gemm_block<T, GMODE>(l, n, o, T(1), T(0), t1, n, C, ldc, cT, D, ldd, dT, lds);
-          gemm_block<T, GMODE>(k, n, l, T(1), T(0), t2, n, B, ldb, bT, t1, n, false, lds);
-          gemm_block<T, GMODE>(m, n, k, T(-1), T(1), M, ldm, A, lda, aT, t2, n, false, lds);
-          break;
-        case 4:  // (A B) (C D)
-          gemm_block<T, GMODE>(m, l, k, T(1), T(0), t1, l, A, lda, aT, B, ldb, bT, lds);
-          gemm_block<T, GMODE>(l, n, o, T(1), T(0), t2, n, C, ldc, cT, D, ldd, dT, lds);
-          gemm_block<T, GMODE>(m, n, l, T(-1), T(1), M, ldm, t1, l, false, t2, n, false, lds);
-          break;
-        default:
-          return -1;
-      }
-      return gemm_4x_l;
-    }
-    case accum: {
-      T* U1 = slot<T>(P, pc, 3) + word(pc, 7);
-      T* V1 = slot<T>(P, pc, 4) + word(pc, 8);
-      const T* U2 = slot<T>(P, pc, 5) + word(pc, 9);
-      const T* V2 = slot<T>(P, pc, 6) + word(pc, 10);
-      accum_block<T, GMODE>(U1, V1, U2, V2, word(pc, 11), word(pc, 12), word(pc, 13), word(pc, 14), word(pc, 15),
-                            word(pc, 16), word(pc, 17), word(pc, 18), word(pc, 19), word(pc, 20), tmp,
-                            reinterpret_cast<const T*>(a.rnd), lds);
-      return accum_l;
-    }
-    default:
-      return -1;
-  }
-}
-
 template <typename T, int FMODE, int GMODE, int NT>
 __global__ __launch_bounds__(NT) void vm_kernel(VmArgs a) {
-  __shared__ __attribute__((aligned(16))) char lds[kLdsBytes];
   __shared__ int32_t ctl[2];
   const int w = blockIdx.x;
   const int32_t* pc = a.words + a.stream_off[w];
@@ -194,7 +71,7 @@ __global__ __launch_bounds__(NT) void vm_kernel(VmArgs a) {
       break;
     }
     const int32_t id = word(pc, 1);
-    const int len = exec_entry<T, FMODE, GMODE>(pc, a, tmp, lds);
+    const int len = exec_entry<T, FMODE, GMODE>(pc, a.ptrs, a.rnd, tmp);
     if (len < 0) {
       if (threadIdx.x == 0) record_error(a.status, VM_BAD_OPERATION, w, id);
       break;
@@ -230,7 +107,6 @@ template <typename T, int FMODE, int GMODE>
 static hipError_t launch_vm_nt(const VmArgs& a, int workers, int nt, hipStream_t s) {
   switch (nt) {
     case 256: return launch_vm_t<T, FMODE, GMODE, 256>(a, workers, s);
-    case 512: return launch_vm_t<T, FMODE, GMODE, 512>(a, workers, s);
     default: return hipErrorInvalidValue;
   }
 }
@@ -260,10 +136,6 @@ int vm_blocks_per_cu(int precision, int threads) {
     if (precision == 0) return max_blocks_t<double, MODE_FP64, MODE_FP64, 256>();
     if (precision == 1) return max_blocks_t<float, MODE_FP32, MODE_FP32, 256>();
     if (precision == 2) return max_blocks_t<float, MODE_FP32, MODE_BF16, 256>();
-  } else if (threads == 512) {
-    if (precision == 0) return max_blocks_t<double, MODE_FP64, MODE_FP64, 512>();
-    if (precision == 1) return max_blocks_t<float, MODE_FP32, MODE_FP32, 512>();
-    if (precision == 2) return max_blocks_t<float, MODE_FP32, MODE_BF16, 512>();
   }
   return 0;
 }
@@ -509,13 +381,9 @@ hipError_t launch_mfma_probe(int mode, const double* a, const double* b, double*
 template <typename T, int FMODE, int GMODE, int NT>
 __global__ __launch_bounds__(NT) void single_op_kernel(const int32_t* words, void* const* ptrs, void* scratch,
                                                        const void* rnd, int32_t* status, int reps) {
-  __shared__ __attribute__((aligned(16))) char lds[kLdsBytes];
-  VmArgs a{};
-  a.ptrs = ptrs;
-  a.rnd = rnd;
   int len = -1;
   for (int r = 0; r < reps; ++r) {
-    len = exec_entry<T, FMODE, GMODE>(words, a, reinterpret_cast<T*>(scratch), lds);
+    len = exec_entry<T, FMODE, GMODE>(words, ptrs, rnd, reinterpret_cast<T*>(scratch));
     __syncthreads();
   }
   if (threadIdx.x == 0) status[0] = len;
@@ -524,12 +392,9 @@ __global__ __launch_bounds__(NT) void single_op_kernel(const int32_t* words, voi
 template <typename T, int FMODE, int GMODE>
 static void launch_single_t(int threads, const int32_t* words, void* const* ptrs, void* scratch, const void* rnd,
                             int32_t* status, int reps, hipStream_t s) {
-  if (threads == 512)
-    hipLaunchKernelGGL((single_op_kernel<T, FMODE, GMODE, 512>), dim3(1), dim3(512), 0, s, words, ptrs, scratch, rnd,
-                       status, reps);
-  else
-    hipLaunchKernelGGL((single_op_kernel<T, FMODE, GMODE, 256>), dim3(1), dim3(256), 0, s, words, ptrs, scratch, rnd,
-                       status, reps);
+  (void)threads;  // one workgroup shape: 256 threads (4 wavefronts)
+  hipLaunchKernelGGL((single_op_kernel<T, FMODE, GMODE, 256>), dim3(1), dim3(256), 0, s, words, ptrs, scratch, rnd,
+                     status, reps);
 }
 
 hipError_t launch_single_op(int precision, int threads, int reps, const int32_t* words, void* const* ptrs,

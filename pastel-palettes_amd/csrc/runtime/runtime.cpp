@@ -42,7 +42,7 @@ int ctx_free(pp_ctx* c, void* p) {
 
 int enqueue_launch(pp_ctx* ctx, pp_stream* s, int wavefronts_per_inst, hipStream_t st, bool record_events) {
   const int threads = 64 * wavefronts_per_inst;
-  if (threads != 256 && threads != 512) return set_error(PP_ERR_INVALID, "wavefronts_per_inst must be 4 or 8");
+  if (threads != 256) return set_error(PP_ERR_INVALID, "wavefronts_per_inst must be 4");
   const int per_cu = pp::vm_blocks_per_cu(s->precision, threads);
   if (per_cu < 1 || s->workers > per_cu * ctx->cus)
     return set_error(PP_ERR_INVALID, "workers (" + std::to_string(s->workers) + ") exceed co-resident capacity " +
@@ -423,7 +423,7 @@ int pp_fill_sketch(pp_ctx* ctx, void* dptr, int64_t count, int precision, uint64
 int pp_exec_single_timed(pp_ctx* ctx, int precision, const int32_t* words, int n_words, const void* const* ptr_table,
                          int n_ptrs, uint64_t scratch_bytes, uint64_t seed, int wavefronts, int reps, double* ms) {
   if (ctx == nullptr || words == nullptr || n_words < 3 || ptr_table == nullptr || n_ptrs < 1 || reps < 1 ||
-      (wavefronts != 4 && wavefronts != 8))
+      wavefronts != 4)
     return set_error(PP_ERR_INVALID, "bad args");
   const int len = operationLength(words[2]);
   if (words[0] != execute || len <= 0 || n_words < len) return set_error(PP_ERR_INVALID, "bad instruction words");

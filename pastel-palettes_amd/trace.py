@@ -89,3 +89,39 @@ def gantt(program, trace, width=100, workers=None):
             if kind == "exec" or rows[w][c] == " ":
                 rows[w][c] = "#" if kind == "exec" else "."
     return "\n".join(f"{w:4d} |{''.join(r)}|" for w, r in enumerate(rows))
+
+
+def critical_path(program, trace):
+    """Chain of instructions that determined the traced launch's end time.
+
+    Walks back from the last instruction to finish: at each step the
+    predecessor (DAG dependency, or the previous instruction of the same
+    worker) that finished last is the one the instruction waited for.
+    Returns (path, by_op) where path is a list of (inst, op, t0_us, t1_us)
+    from first to last and by_op sums the path's execution time per
+    operation; the gaps between consecutive path entries are signalling and
+    scheduling latency."""
+    recs = [r for r in intervals(program, trace) if r[1] == "exec"]
+    if not recs:
+        return [], {}
+    info = {r[2]: r for r in recs}  # inst -> (w, kind, inst, op, t0, t1)
+    prev_on_worker = {}
+    last = {}
+    for w, kind, inst, op, t0, t1 in sorted(recs, key=lambda r: (r[0], r[4])):
+        if w in last:
+            prev_on_worker[inst] = last[w]
+        last[w] = inst
+    cur = max(recs, key=lambda r: r[5])[2]
+    path = []
+    while cur is not None:
+        w, kind, inst, op, t0, t1 = info[cur]
+        path.append((inst, op, t0, t1))
+        cands = [p for p, _ in program.preds(inst) if p in info]
+        if inst in prev_on_worker:
+            cands.append(prev_on_worker[inst])
+        cur = max(cands, key=lambda p: info[p][5]) if cands else None
+    path.reverse()
+    by_op = collections.defaultdict(float)
+    for inst, op, t0, t1 in path:
+        by_op[OP_NAMES.get(op, str(op))] += t1 - t0
+    return path, dict(by_op)

@@ -16,11 +16,12 @@
 //           Q Q^T depends only on range(Y), so the product U1 V1^T is the same.
 //
 // Organisation (MI355X): everything is blocked by NB = 32.
-//   * getrf: right-looking.  The 32-column panel is eliminated with one row
-//     per thread (pivot row published through double-buffered LDS, one
-//     barrier per column); the same row operations applied to the identity
-//     give L11^{-1} for free.  U12 = L11^{-1} A12 and the trailing update
-//     A22 -= L21 U12 run on MFMA through gemm_block.
+//   * getrf: right-looking.  In the 32-column panel one wave factors the
+//     diagonal block (pivot rows broadcast from registers), while a second
+//     wave forms L11^{-1} and two more eliminate the rows below, following
+//     the first through LDS counters (no workgroup barrier per column).
+//     U12 = L11^{-1} A12 and the trailing update A22 -= L21 U12 run on MFMA
+//     through gemm_block.
 //   * trsm*: the inverses of the 32 x 32 diagonal blocks of the triangular
 //     factor are formed first (one wave per block, column-wise substitution),
 //     then the solve is a sequence of block products X_i = inv_i B_i and
@@ -95,78 +96,176 @@ __device__ __forceinline__ void diag_inverses(const T* F, int ldf, int n, T* out
 }
 
 // ---------------------------------------------------------------- getrf
-// Unblocked LU of the panel M[p:ny, p:p+pw] (pw <= NB) in place.  Thread t
-// owns row p + t; the pivot row of step j is published by its owner into one
-// of two LDS buffers, so one barrier per column suffices (the buffer written
-// at step j + 2 was last read before every thread passed barrier j + 1).  The
-// top pw rows also carry the identity through the elimination, which leaves
-// L11^{-1} there; it is written row-major (pitch NB) to linv.  Rows beyond
-// blockDim.x are eliminated afterwards against the final pivot rows.
+// LDS ordering inside the panel.  The LDS unit executes the operations of one
+// wave in issue order, so (a) lanes of a wave see the wave's earlier LDS
+// writes, and (b) a wave that reads a counter written after some data by
+// another wave also sees that data.  Only compiler reordering has to be
+// prevented, which a signal fence does without waiting on memory counters.
+__device__ __forceinline__ void lds_order() {
+  __atomic_signal_fence(__ATOMIC_SEQ_CST);
+  __builtin_amdgcn_wave_barrier();
+}
+
+// Waits until the LDS counter reaches `target` (written only by wave 0 of the
+// panel, which never waits on the other waves, so the wait always ends).
+__device__ __forceinline__ void lds_wait_ge(volatile PP_LDS int* c, int target) {
+  while (*c < target) __builtin_amdgcn_s_sleep(1);
+  lds_order();
+}
+
+// Value v of lane `lane` on every lane (v_readlane into scalar registers,
+// which an FMA can take as an operand).
+template <typename T>
+__device__ __forceinline__ T bcast_lane(T v, int lane) {
+  if constexpr (sizeof(T) == 8) {
+    const uint64_t b = __builtin_bit_cast(uint64_t, v);
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)b, lane);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(b >> 32), lane);
+    return __builtin_bit_cast(T, ((uint64_t)hi << 32) | lo);
+  } else {
+    return __builtin_bit_cast(T, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), lane));
+  }
+}
+
+// Unblocked LU of the panel M[p:ny, p:p+pw] (pw <= NB) in place, run by three
+// roles concurrently (no workgroup barrier inside):
+//   wave 0     factors the pw x pw diagonal block.  Lane i owns row i in
+//              registers; the pivot row of step j is read straight from lane
+//              j's registers (v_readlane), so the sequential chain touches no
+//              memory.  Multiplier column j goes to LDS after step j
+//              (counter `cols`), and final rows go to LDS in groups of eight
+//              together with their pivot reciprocals (counter `rows`).
+//   wave 1     forms L11^{-1} by the same row operations applied to the
+//              identity, following the multiplier columns, and writes it
+//              row-major (pitch NB) to linv.
+//   waves 2-3  eliminate the rows below the block against each group of
+//              pivot rows as soon as it is published.
+// Multipliers are formed with the reciprocal of the pivot, which the owner of
+// the next pivot row computes right after its pivot element is updated.
 template <typename T>
 __device__ __forceinline__ void panel_lu(T* M, int ld, int p, int pw, int ny, T* linv) {
-  PP_LDS T* piv = (PP_LDS T*)lds_base();  // [2][2 NB]
-  const int tid = threadIdx.x, nt = blockDim.x;
+  constexpr int UP = NB + 2;              // pitch of U: 16-byte aligned rows
+  constexpr int GRP = 8;                  // rows per publication
+  PP_LDS T* U = (PP_LDS T*)lds_base();    // [NB][UP]: final rows, column NB = 1 / u_jj
+  PP_LDS T* Lc = U + NB * UP;              // [NB][NB]: multiplier column j at Lc[j * NB + i]
+  volatile PP_LDS int* cols = (volatile PP_LDS int*)(Lc + NB * NB);
+  volatile PP_LDS int* rows = cols + 1;
+  const int tid = threadIdx.x, l = lane_id(), w = wave_id();
   const int R = ny - p;
-  const bool own = tid < R;
-  T* row = M + (int64_t)(p + tid) * ld + p;
-  T x[NB], e[NB];
-#pragma unroll
-  for (int c = 0; c < NB; ++c) {
-    x[c] = (own && c < pw) ? row[c] : T(0);
-    e[c] = (c == tid) ? T(1) : T(0);
-  }
-#pragma unroll
-  for (int j = 0; j < NB; ++j) {
-    if (j < pw) {
-      PP_LDS T* pv = piv + (j & 1) * 2 * NB;
-      if (tid == j) {
-#pragma unroll
-        for (int c = j; c < NB; ++c) pv[c] = x[c];
-#pragma unroll
-        for (int c = 0; c <= j; ++c) pv[NB + c] = e[c];
-      }
-      __syncthreads();
-      if (own && tid > j) {
-        const T lv = x[j] / pv[j];
-        x[j] = lv;
-#pragma unroll
-        for (int c = j + 1; c < NB; ++c) x[c] -= lv * pv[c];
-        if (tid < pw) {
-#pragma unroll
-          for (int c = 0; c <= j; ++c) e[c] -= lv * pv[NB + c];
-        }
-      }
-    }
-  }
-  if (own) {
-#pragma unroll
-    for (int c = 0; c < NB; ++c)
-      if (c < pw) row[c] = x[c];
-  }
-  if (tid < pw) {
-#pragma unroll
-    for (int c = 0; c < NB; ++c)
-      if (c < pw) linv[tid * NB + c] = e[c];
+  if (tid == 0) {
+    *cols = 0;
+    *rows = 0;
   }
   __syncthreads();
-  for (int r = nt + tid; r < R; r += nt) {
-    T* rr = M + (int64_t)(p + r) * ld + p;
+  if (w == 0) {
+    const bool own = l < pw;
+    T* row = M + (int64_t)(p + (own ? l : 0)) * ld + p;
+    T x[NB];
 #pragma unroll
-    for (int c = 0; c < NB; ++c) x[c] = c < pw ? rr[c] : T(0);
+    for (int c = 0; c < NB; ++c) x[c] = (own && c < pw) ? row[c] : T(0);
+    T rinv = T(1) / x[0];  // reciprocal of the next pivot (meaningful on its owner)
+    T my_rinv = rinv;      // reciprocal of this lane's own pivot
 #pragma unroll
     for (int j = 0; j < NB; ++j) {
       if (j < pw) {
-        const T* pr = M + (int64_t)(p + j) * ld + p;
-        const T lv = x[j] / pr[j];
-        x[j] = lv;
+        const T rj = bcast_lane(rinv, j);
+        const bool act = own && l > j;
+        // inactive lanes use a zero multiplier: x - 0 u = x for the finite
+        // pivot rows, which avoids a select per element
+        const T lv = act ? x[j] * rj : T(0);
+        if (act) {
+          x[j] = lv;
+          Lc[j * NB + l] = lv;
+        }
+        if (j + 1 < NB) {
+          const T u = bcast_lane(x[j + 1], j);
+          x[j + 1] -= lv * u;
+          rinv = T(1) / x[j + 1];
+          if (l == j + 1) my_rinv = rinv;
+        }
 #pragma unroll
-        for (int c = j + 1; c < NB; ++c)
-          if (c < pw) x[c] -= lv * pr[c];
+        for (int c = j + 2; c < NB; ++c) x[c] -= lv * bcast_lane(x[c], j);
+        lds_order();
+        if (l == 0) *cols = j + 1;
+        if ((j % GRP) == GRP - 1 || j == pw - 1) {
+          if (own && l <= j && l > j - GRP) {
+#pragma unroll
+            for (int c = 0; c < NB; ++c) U[l * UP + c] = x[c];
+            U[l * UP + NB] = my_rinv;
+          }
+          lds_order();
+          if (l == 0) *rows = j + 1;
+        }
       }
     }
+    if (own) {
 #pragma unroll
-    for (int c = 0; c < NB; ++c)
-      if (c < pw) rr[c] = x[c];
+      for (int c = 0; c < NB; ++c)
+        if (c < pw) row[c] = x[c];
+    }
+  } else if (w == 1) {
+    const bool own = l < pw;
+    T e[NB];
+#pragma unroll
+    for (int c = 0; c < NB; ++c) e[c] = (c == l) ? T(1) : T(0);
+#pragma unroll
+    for (int j = 0; j < NB; ++j) {
+      if (j < pw) {
+        lds_wait_ge(cols, j + 1);  // multiplier column j is in LDS
+        const bool act = own && l > j;
+        const T lij = act ? (T)Lc[j * NB + l] : T(0);
+#pragma unroll
+        for (int c = 0; c <= j; ++c) e[c] -= lij * bcast_lane(e[c], j);  // row j of L11^{-1} is final
+      }
+    }
+    if (own) {
+#pragma unroll
+      for (int c = 0; c < NB; ++c)
+        if (c < pw) linv[l * NB + c] = e[c];
+    }
+  } else {
+    const int nb_t = blockDim.x - 128, bt = tid - 128;
+    for (int r0 = pw; r0 < R; r0 += 2 * nb_t) {
+      // two rows per thread: r0 + bt and r0 + nb_t + bt
+      const int ra = r0 + bt, rb = r0 + nb_t + bt;
+      const bool oa = ra < R, ob = rb < R;
+      T* pa = M + (int64_t)(p + (oa ? ra : 0)) * ld + p;
+      T* pb = M + (int64_t)(p + (ob ? rb : 0)) * ld + p;
+      T xa[NB], xb[NB];
+#pragma unroll
+      for (int c = 0; c < NB; ++c) {
+        xa[c] = (oa && c < pw) ? pa[c] : T(0);
+        xb[c] = (ob && c < pw) ? pb[c] : T(0);
+      }
+#pragma unroll
+      for (int j = 0; j < NB; ++j) {
+        if (j < pw) {
+          // the first row group waits for each published group of pivots
+          if (r0 == pw && (j % GRP) == 0) lds_wait_ge(rows, (j + GRP < pw) ? j + GRP : pw);
+          // the whole pivot row is requested before any of it is used (one
+          // LDS round trip per step)
+          T pr[NB + 1];
+#pragma unroll
+          for (int c = j + 1; c <= NB; ++c) pr[c] = U[j * UP + c];
+          __builtin_amdgcn_sched_barrier(0);
+          T la = xa[j] * pr[NB], lb = xb[j] * pr[NB];
+          // keeps later steps' LDS reads from being hoisted above this one
+          asm volatile("" : "+v"(la), "+v"(lb)::"memory");
+          xa[j] = la;
+          xb[j] = lb;
+#pragma unroll
+          for (int c = j + 1; c < NB; ++c) {
+            xa[c] -= la * pr[c];
+            xb[c] -= lb * pr[c];
+          }
+        }
+      }
+#pragma unroll
+      for (int c = 0; c < NB; ++c) {
+        if (oa && c < pw) pa[c] = xa[c];
+        if (ob && c < pw) pb[c] = xb[c];
+      }
+    }
   }
   __syncthreads();
 }

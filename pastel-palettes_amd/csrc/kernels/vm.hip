@@ -174,57 +174,40 @@ __global__ void copy2d_kernel(D* dst, int ldd, const S* src, int lds_, int n, in
   }
 }
 
-// H-matrix times block: phase 1, t_b = V_b^T x[cols of b] for every low-rank
-// leaf b (one workgroup per leaf; fixed-order reduction).
+// H-matrix-vector product out = y - A_H x (FP64, nrhs columns), two launches
+// of the MFMA workgroup GEMM:
+// phase 1: one workgroup per low-rank leaf: t_b = V_b^T x_b (rank x nrhs).
 __global__ __launch_bounds__(256) void hmv_lowrank_kernel(const MatvecBlock* blocks, const int* lr_list,
                                                            const double* A, const double* x, int ldx, int nrhs,
                                                            double* t) {
-  __shared__ double red[256];
   const MatvecBlock b = blocks[lr_list[blockIdx.x]];
-  const int r = b.rank;
-  for (int q = 0; q < r * nrhs; ++q) {
-    const int kk = q / nrhs, c = q % nrhs;
-    double s = 0.;
-    for (int j = threadIdx.x; j < b.nx; j += blockDim.x)
-      s += A[b.off_v + (int64_t)j * r + kk] * x[(int64_t)(b.col0 + j) * ldx + c];
-    red[threadIdx.x] = s;
-    __syncthreads();
-    for (int h = blockDim.x / 2; h > 0; h >>= 1) {
-      if ((int)threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
-      __syncthreads();
-    }
-    if (threadIdx.x == 0) t[b.t_off + q] = red[0];
-    __syncthreads();
-  }
+  gemm_block<double, MODE_FP64>(b.rank, nrhs, b.nx, 1.0, 0.0, t + b.t_off, nrhs, A + b.off_v, b.rank, true,
+                                x + (int64_t)b.col0 * ldx, ldx, false);
 }
 
-// phase 2: for every row tile, out = y - sum of contributions (dense D x and
-// low-rank U t) of the leaves covering the tile, in a fixed order.
+// phase 2: one workgroup per row tile: out = y - sum of the contributions
+// (dense D x and low-rank U t) of the leaves covering the tile, in a fixed
+// order.
 __global__ __launch_bounds__(256) void hmv_rows_kernel(const MatvecBlock* blocks, const MatvecTile* tiles,
                                                         const int* contrib, const double* A, const double* x, int ldx,
                                                         const double* t, const double* y, int ldy, double* out,
                                                         int ldo, int nrhs) {
   const MatvecTile tl = tiles[blockIdx.x];
+  double* o = out + (int64_t)tl.row0 * ldo;
   for (int e = threadIdx.x; e < tl.rows * nrhs; e += blockDim.x) {
     const int i = e / nrhs, c = e % nrhs;
-    const int row = tl.row0 + i;
-    double acc = 0.;
-    for (int q = tl.first; q < tl.first + tl.count; ++q) {
-      const MatvecBlock b = blocks[contrib[2 * q]];
-      const int li = row - b.row0;
-      if (b.type == 1) {
-        const double* Ar = A + b.off + (int64_t)li * b.nx;
-        double s = 0.;
-        for (int j = 0; j < b.nx; ++j) s += Ar[j] * x[(int64_t)(b.col0 + j) * ldx + c];
-        acc += s;
-      } else {
-        const double* Ur = A + b.off_u + (int64_t)li * b.rank;
-        double s = 0.;
-        for (int kk = 0; kk < b.rank; ++kk) s += Ur[kk] * t[b.t_off + kk * nrhs + c];
-        acc += s;
-      }
-    }
-    out[(int64_t)row * ldo + c] = (y != nullptr ? y[(int64_t)row * ldy + c] : 0.) - acc;
+    o[(int64_t)i * ldo + c] = y != nullptr ? y[(int64_t)(tl.row0 + i) * ldy + c] : 0.;
+  }
+  __syncthreads();
+  for (int q = tl.first; q < tl.first + tl.count; ++q) {
+    const MatvecBlock b = blocks[contrib[2 * q]];
+    const int li = tl.row0 - b.row0;
+    if (b.type == 1)
+      gemm_block<double, MODE_FP64>(tl.rows, nrhs, b.nx, -1.0, 1.0, o, ldo, A + b.off + (int64_t)li * b.nx, b.nx,
+                                    false, x + (int64_t)b.col0 * ldx, ldx, false);
+    else
+      gemm_block<double, MODE_FP64>(tl.rows, nrhs, b.rank, -1.0, 1.0, o, ldo, A + b.off_u + (int64_t)li * b.rank,
+                                    b.rank, false, t + b.t_off, nrhs, false);
   }
 }
 

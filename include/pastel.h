@@ -96,6 +96,23 @@ int pp_program_solve(const pp_hmatrix* h, int nrhs, int ld, int rhs_split, int w
 int pp_program_factor_batch(const pp_hmatrix* const* hs, int count, int workers, pp_program** out);
 int pp_program_solve_batch(const pp_hmatrix* const* hs, int count, const int32_t* nrhs, const int32_t* ld,
                            int rhs_split, int workers, pp_program** out);
+
+/* Solve through explicit leaf inverses (no counterpart in the reference,
+ * which stops at the factorization).  The factor program below also forms,
+ * for every dense diagonal leaf, L^{-1} of its unit lower factor and U^{-1}
+ * of its upper factor, in an inverse area that starts at element
+ * pp_hmatrix_inverse_base(h) of the matrix slot and holds
+ * pp_hmatrix_inverse_elems(h) elements; the area must hold the identity
+ * blocks of pp_hmatrix_inverse_identity before each run.  The solve program
+ * then applies each leaf with one dense product (operation trmm). */
+int64_t pp_hmatrix_inverse_base(const pp_hmatrix* h);
+int64_t pp_hmatrix_inverse_elems(const pp_hmatrix* h);
+/* Writes the identity blocks (zeros elsewhere) into a host area of
+ * pp_hmatrix_inverse_elems(h) doubles. */
+int pp_hmatrix_inverse_identity(const pp_hmatrix* h, double* area);
+int pp_program_factor_inv_batch(const pp_hmatrix* const* hs, int count, int workers, pp_program** out);
+int pp_program_solve_inv_batch(const pp_hmatrix* const* hs, int count, const int32_t* nrhs, const int32_t* ld,
+                               int rhs_split, int workers, pp_program** out);
 void pp_program_destroy(pp_program* p);
 
 typedef struct pp_program_info {

@@ -45,6 +45,7 @@ def lib():
         L.or_trsml.argtypes = [dp, dp, ip, ip, ip, ip, ip]
         L.or_trsmr.argtypes = [dp, dp, ip, ip, ip, ip, ip]
         L.or_trsmu.argtypes = [dp, dp, ip, ip, ip, ip]
+        L.or_trmm.argtypes = [dp, dp, ip, ip, ip, ip]
         L.or_gemm.argtypes = [C.c_double, C.c_double, dp, dp, dp, ip, ip, ip, ip, ip, ip, ip, ip]
         L.or_givens_qr.argtypes = [dp, ip, ip, ip, dp, ip]
         L.or_accum.argtypes = [dp, dp, dp, dp, ip, ip, ip, ip, ip, ip, ip, ip, dp]
@@ -118,6 +119,13 @@ def trsmu(b, u):
     b = np.ascontiguousarray(b, np.float64).copy()
     u = np.ascontiguousarray(u, np.float64)
     lib().or_trsmu(_d(b), _d(u), b.shape[1], b.shape[0], b.shape[1], u.shape[1])
+    return b
+
+
+def trmm(b, t):
+    b = np.ascontiguousarray(b, np.float64).copy()
+    t = np.ascontiguousarray(t, np.float64)
+    lib().or_trmm(_d(b), _d(t), b.shape[1], b.shape[0], b.shape[1], t.shape[1])
     return b
 
 

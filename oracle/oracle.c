@@ -79,6 +79,21 @@ void or_trsmu(double* B, const double* U, int nx_b, int ny_b, int ld_b, int ld_u
   }
 }
 
+/* B <- T B with a dense ny_b x ny_b T (no reference counterpart: applies the
+ * explicit leaf inverses of the inverse-based solve). */
+void or_trmm(double* B, const double* Tm, int nx_b, int ny_b, int ld_b, int ld_t) {
+  double* X = (double*)malloc(sizeof(double) * (size_t)ny_b * (size_t)(nx_b > 0 ? nx_b : 1));
+  for (int i = 0; i < ny_b; ++i)
+    for (int c = 0; c < nx_b; ++c) X[(int64_t)i * nx_b + c] = B[(int64_t)i * ld_b + c];
+  for (int i = 0; i < ny_b; ++i)
+    for (int c = 0; c < nx_b; ++c) {
+      double s = 0.;
+      for (int k = 0; k < ny_b; ++k) s += Tm[(int64_t)i * ld_t + k] * X[(int64_t)k * nx_b + c];
+      B[(int64_t)i * ld_b + c] = s;
+    }
+  free(X);
+}
+
 /* blockDenseGemm, src/kernel.cu:415-664: M = beta M + alpha op(A) op(B). */
 void or_gemm(double alpha, double beta, double* M, const double* A, const double* B, int m, int n, int k, int ld_m,
              int ld_a, int ld_b, int a_T, int b_T) {
@@ -184,6 +199,9 @@ int or_exec(const int32_t* w, double* const* P, double* tmp, const double* rnd) 
     case 9: /* trsmu */
       or_trsmu(P[w[3]] + w[5], P[w[4]] + w[6], w[7], w[8], w[10], w[11]);
       return 13;
+    case 10: /* trmm */
+      or_trmm(P[w[3]] + w[5], P[w[4]] + w[6], w[7], w[8], w[9], w[10]);
+      return 11;
     case 4: /* gemm: M -= A B */
     case 5: /* gemm_plus: M += A B */
       or_gemm(w[2] == 4 ? -1. : 1., 1., P[w[3]] + w[6], P[w[4]] + w[7], P[w[5]] + w[8], w[9], w[10], w[11], w[12],

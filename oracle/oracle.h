@@ -29,6 +29,7 @@ void or_trsml(double* B, const double* L, int nx_b, int ny_b, int ld_b, int ld_l
 void or_trsmr(double* B, const double* U, int nx_b, int ny_b, int ld_b, int ld_u, int b_T);
 /* B <- U^{-1} B, U upper non-unit ny_b x ny_b (backward substitution). */
 void or_trsmu(double* B, const double* U, int nx_b, int ny_b, int ld_b, int ld_u);
+void or_trmm(double* B, const double* Tm, int nx_b, int ny_b, int ld_b, int ld_t);
 /* M <- beta M + alpha op(A) op(B) (src/kernel.cu:288-664); beta == 0 does
  * not read M. */
 void or_gemm(double alpha, double beta, double* M, const double* A, const double* B, int m, int n, int k, int ld_m,

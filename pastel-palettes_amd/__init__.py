@@ -52,12 +52,17 @@ def lib():
         "pp_hmatrix_destroy": (None, [vp]),
         "pp_hmatrix_size": (C.c_int, [vp]),
         "pp_hmatrix_storage": (i64, [vp]),
+        "pp_hmatrix_inverse_base": (i64, [vp]),
+        "pp_hmatrix_inverse_elems": (i64, [vp]),
+        "pp_hmatrix_inverse_identity": (C.c_int, [vp, P(dp)]),
         "pp_hmatrix_num_leaves": (C.c_int, [vp]),
         "pp_hmatrix_leaf": (C.c_int, [vp, C.c_int, P(i64)]),
         "pp_program_factor": (C.c_int, [vp, C.c_int, P(vp)]),
         "pp_program_solve": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, C.c_int, P(vp)]),
         "pp_program_factor_batch": (C.c_int, [P(vp), C.c_int, C.c_int, P(vp)]),
         "pp_program_solve_batch": (C.c_int, [P(vp), C.c_int, P(i32), P(i32), C.c_int, C.c_int, P(vp)]),
+        "pp_program_factor_inv_batch": (C.c_int, [P(vp), C.c_int, C.c_int, P(vp)]),
+        "pp_program_solve_inv_batch": (C.c_int, [P(vp), C.c_int, P(i32), P(i32), C.c_int, C.c_int, P(vp)]),
         "pp_program_destroy": (None, [vp]),
         "pp_program_get_info": (C.c_int, [vp, vp]),
         "pp_program_temp_elems": (i64, [vp, C.c_int]),
@@ -220,6 +225,31 @@ class HMatrix:
     def storage(self):
         return int(lib().pp_hmatrix_storage(self.h))
 
+    @property
+    def inverse_base(self):
+        """Start (elements) of the leaf-inverse area in the matrix slot."""
+        return int(lib().pp_hmatrix_inverse_base(self.h))
+
+    @property
+    def solver_elems(self):
+        """Arena elements a Solver keeps per matrix (arena + inverse area)."""
+        return self.inverse_base + int(lib().pp_hmatrix_inverse_elems(self.h))
+
+    def inverse_identity(self):
+        """Initial contents of the leaf-inverse area (identity blocks)."""
+        a = np.empty(int(lib().pp_hmatrix_inverse_elems(self.h)))
+        check(lib().pp_hmatrix_inverse_identity(self.h, _p(a, C.c_double)))
+        return a
+
+    def with_inverse_area(self, arena):
+        """The arena extended by the leaf-inverse area, ready for
+        Program.factor_inv_batch."""
+        inv = self.inverse_identity()
+        out = np.zeros(self.inverse_base + inv.size)
+        out[:arena.size] = arena
+        out[self.inverse_base:] = inv
+        return out
+
     def leaves(self):
         L = lib()
         out = []
@@ -269,6 +299,24 @@ class Program:
         h = C.c_void_p()
         check(lib().pp_program_solve_batch(arr, len(hms), _p(nr, C.c_int32), _p(nr, C.c_int32), rhs_split, workers,
                                            C.byref(h)))
+        return cls(h, list(hms))
+
+    @classmethod
+    def factor_inv_batch(cls, hms, workers):
+        """Factorization that also forms the explicit leaf inverses."""
+        arr = (C.c_void_p * len(hms))(*[m.h for m in hms])
+        h = C.c_void_p()
+        check(lib().pp_program_factor_inv_batch(arr, len(hms), workers, C.byref(h)))
+        return cls(h, list(hms))
+
+    @classmethod
+    def solve_inv_batch(cls, hms, nrhs, workers, rhs_split=1):
+        """Solve applying the leaf inverses formed by factor_inv_batch."""
+        arr = (C.c_void_p * len(hms))(*[m.h for m in hms])
+        nr = np.ascontiguousarray(nrhs, dtype=np.int32)
+        h = C.c_void_p()
+        check(lib().pp_program_solve_inv_batch(arr, len(hms), _p(nr, C.c_int32), _p(nr, C.c_int32), rhs_split,
+                                               workers, C.byref(h)))
         return cls(h, list(hms))
 
     def info(self):

@@ -22,6 +22,10 @@ struct pp_hmatrix {
   std::vector<const h_block*> leaves;
 };
 
+// The leaf-inverse area of a matrix slot starts after the arena, 64-element
+// aligned.
+inline int64_t inverse_base(const Hierarchical* h) { return (h->getStorage() + 63) / 64 * 64; }
+
 struct pp_program {
   h_ops_tree tree;
   std::unique_ptr<h_ops_dag> dag;

@@ -1,4 +1,5 @@
 // C ABI, host half: rts queries, H-matrix structures and programs.
+#include <algorithm>
 #include <chrono>
 #include <cstdio>
 #include <cstring>
@@ -159,6 +160,27 @@ int pp_hmatrix_create_dense(int n, int leaf, pp_hmatrix** out) {
 void pp_hmatrix_destroy(pp_hmatrix* h) { delete h; }
 int pp_hmatrix_size(const pp_hmatrix* h) { return h ? h->h->getNx_abs() : PP_ERR_INVALID; }
 int64_t pp_hmatrix_storage(const pp_hmatrix* h) { return h ? h->h->getStorage() : PP_ERR_INVALID; }
+int64_t pp_hmatrix_inverse_base(const pp_hmatrix* h) { return h ? inverse_base(h->h.get()) : PP_ERR_INVALID; }
+int64_t pp_hmatrix_inverse_elems(const pp_hmatrix* h) { return h ? h->h->inverseStorage() : PP_ERR_INVALID; }
+
+int pp_hmatrix_inverse_identity(const pp_hmatrix* h, double* area) {
+  try {
+    if (h == nullptr || area == nullptr) return set_error(PP_ERR_INVALID, "null argument");
+    std::vector<Hierarchical::inv_block> lay;
+    h->h->inverseLayout(lay);
+    std::fill(area, area + h->h->inverseStorage(), 0.0);
+    for (const auto& e : lay) {
+      const int64_t second = (((int64_t)e.nb * e.nb + 31) / 32) * 32;
+      for (int i = 0; i < e.nb; ++i) {
+        area[e.inv_off + (int64_t)i * e.nb + i] = 1.0;
+        area[e.inv_off + second + (int64_t)i * e.nb + i] = 1.0;
+      }
+    }
+    return PP_OK;
+  } catch (...) {
+    return from_exception();
+  }
+}
 int pp_hmatrix_num_leaves(const pp_hmatrix* h) { return h ? (int)h->leaves.size() : PP_ERR_INVALID; }
 
 int pp_hmatrix_leaf(const pp_hmatrix* h, int i, int64_t info[8]) {
@@ -231,6 +253,34 @@ int pp_program_factor(const pp_hmatrix* h, int workers, pp_program** out) {
   return pp_program_factor_batch(&h, 1, workers, out);
 }
 
+int pp_program_factor_inv_batch(const pp_hmatrix* const* hs, int count, int workers, pp_program** out) {
+  try {
+    if (hs == nullptr || count < 1 || workers < 1 || out == nullptr) return set_error(PP_ERR_INVALID, "bad args");
+    auto p = std::make_unique<pp_program>();
+    p->n_slots = 3 * count;
+    h_ops_tree inv_ops;
+    for (int i = 0; i < count; ++i) {
+      if (hs[i] == nullptr) return set_error(PP_ERR_INVALID, "null matrix in batch");
+      dev_temp tmp(3 * i + 1);
+      const Hierarchical* h = hs[i]->h.get();
+      std::unique_ptr<const h_index> root(h->getRootIndex());
+      std::unique_ptr<h_ops_tree> t(h->generateOps_GETRF(root.get(), &tmp, 3 * i));
+      p->tree.append(*t);
+      std::unique_ptr<h_ops_tree> v(h->generateOps_Invert(root.get(), inverse_base(h), 3 * i));
+      inv_ops.append(*v);
+      p->temp_elems.push_back(tmp.size());
+    }
+    // the inversions follow every factorization in program order, so each
+    // depends only on the getrf of its own leaf
+    p->tree.append(inv_ops);
+    finish_program(p.get(), workers);
+    *out = p.release();
+    return PP_OK;
+  } catch (...) {
+    return from_exception();
+  }
+}
+
 int pp_program_solve_batch(const pp_hmatrix* const* hs, int count, const int32_t* nrhs, const int32_t* ld,
                            int rhs_split, int workers, pp_program** out) {
   try {
@@ -243,6 +293,30 @@ int pp_program_solve_batch(const pp_hmatrix* const* hs, int count, const int32_t
       std::unique_ptr<const h_index> root(hs[i]->h->getRootIndex());
       std::unique_ptr<h_ops_tree> t(
           hs[i]->h->generateOps_Solve(root.get(), 3 * i + 2, 0, nrhs[i], ld[i], rhs_split, 3 * i));
+      p->tree.append(*t);
+      p->temp_elems.push_back(0);
+    }
+    finish_program(p.get(), workers);
+    *out = p.release();
+    return PP_OK;
+  } catch (...) {
+    return from_exception();
+  }
+}
+
+int pp_program_solve_inv_batch(const pp_hmatrix* const* hs, int count, const int32_t* nrhs, const int32_t* ld,
+                               int rhs_split, int workers, pp_program** out) {
+  try {
+    if (hs == nullptr || count < 1 || workers < 1 || out == nullptr || nrhs == nullptr || ld == nullptr)
+      return set_error(PP_ERR_INVALID, "bad args");
+    auto p = std::make_unique<pp_program>();
+    p->n_slots = 3 * count;
+    for (int i = 0; i < count; ++i) {
+      if (hs[i] == nullptr) return set_error(PP_ERR_INVALID, "null matrix in batch");
+      const Hierarchical* h = hs[i]->h.get();
+      std::unique_ptr<const h_index> root(h->getRootIndex());
+      std::unique_ptr<h_ops_tree> t(
+          h->generateOps_Solve(root.get(), 3 * i + 2, 0, nrhs[i], ld[i], rhs_split, 3 * i, inverse_base(h)));
       p->tree.append(*t);
       p->temp_elems.push_back(0);
     }

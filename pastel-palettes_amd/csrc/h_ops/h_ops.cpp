@@ -63,6 +63,7 @@ int operationLength(int op) {
     case gemm_4x: return gemm_4x_l;
     case accum: return accum_l;
     case trsmu: return trsmu_l;
+    case trmm: return trmm_l;
     default: return -1;
   }
 }
@@ -170,6 +171,17 @@ void emit_trsmu(h_ops_tree* tree, const dview& b, const dview& u) {
   o.data.addData(true, u.map());
   o.flops = (long long)b.nx * b.ny * b.ny;
   o.scratch = tri_scratch_elems(b.ny);
+  tree->push(std::move(o));
+}
+
+// The product goes through a copy of B in the worker scratch.
+void emit_trmm(h_ops_tree* tree, const dview& b, const dview& t) {
+  require(t.ny == b.ny && t.nx == b.ny, "trmm: T must be square with B's row count");
+  h_op o = make_op(trmm, {b.ptr, t.ptr, off32(b.off), off32(t.off), b.nx, b.ny, b.ld, t.ld});
+  o.data.addData(false, b.map());
+  o.data.addData(true, t.map());
+  o.flops = flops_gemm(b.ny, b.nx, b.ny);
+  o.scratch = (int64_t)b.ny * b.nx;
   tree->push(std::move(o));
 }
 
@@ -538,10 +550,30 @@ public:
                piece(b, inner, cols, (int)k, (int)j, kp, cp));
   }
 
-  // ---- solve: b <- L^{-1} b (unit lower), then b <- U^{-1} b
+  // ---- solve: b <- L^{-1} b (unit lower), then b <- U^{-1} b.  With an
+  // inverse table, a dense diagonal leaf is applied as one product with the
+  // explicit inverse of its factor instead of a substitution.
+  const std::vector<Hierarchical::inv_block>* inv = nullptr;
+  int64_t inv_base = 0;
+
+  // L^{-1} (upper == false) or U^{-1} of the dense diagonal leaf at `leaf_off`
+  dview inverse_of(const dview& leaf, bool upper) const {
+    for (const auto& e : *inv)
+      if (e.leaf_off == leaf.off) {
+        const int64_t off = inv_base + e.inv_off + (upper ? align32((int64_t)e.nb * e.nb) : 0);
+        return dview{ptr, off, e.nb, e.nb, e.nb};
+      }
+    throw std::invalid_argument("solve: leaf without an inverse");
+  }
+
   void forward(const operand& l, const mview& b) {
     if (l.type == element_t::dense) {
-      emit_trsml(tree, b, l.d.v);
+      if (inv) {
+        require(!b.t, "trmm: transposed rhs");
+        emit_trmm(tree, b.v, inverse_of(l.d.v, false));
+      } else {
+        emit_trsml(tree, b, l.d.v);
+      }
       return;
     }
     const h_block* h = l.h;
@@ -557,7 +589,8 @@ public:
   void backward(const operand& u, const mview& b) {
     if (u.type == element_t::dense) {
       require(!b.t, "trsmu: transposed rhs");
-      emit_trsmu(tree, b.v, u.d.v);
+      if (inv) emit_trmm(tree, b.v, inverse_of(u.d.v, true));
+      else emit_trsmu(tree, b.v, u.d.v);
       return;
     }
     const h_block* h = u.h;
@@ -588,14 +621,21 @@ h_ops_tree* Hierarchical::generateOps_GETRF(const h_index* self, dev_temp* tmp, 
 }
 
 h_ops_tree* Hierarchical::generateOps_Solve(const h_index* self, int rhs_ptr_id, int64_t rhs_off, int nrhs, int rhs_ld,
-                                            int rhs_split, int slot) const {
+                                            int rhs_split, int slot, int64_t inv_base) const {
   const int ms = slot >= 0 ? slot : ptr_id;
   if (self == nullptr) throw std::invalid_argument("generateOps_Solve: null index");
   if (nrhs <= 0 || rhs_ld < nrhs || rhs_split <= 0) throw std::invalid_argument("generateOps_Solve: bad rhs shape");
+  if (inv_base >= 0 && self->node != root) throw std::invalid_argument("generateOps_Solve: inverses need the root");
   h_ops_tree* tree = new h_ops_tree();
   try {
     dev_temp unused(-1);
     op_builder bld(tree, &unused, ms);
+    std::vector<inv_block> lay;
+    if (inv_base >= 0) {
+      inverseLayout(lay);
+      bld.inv = &lay;
+      bld.inv_base = inv_base;
+    }
     const operand root = of_block(self->node, ms);
     const int groups = std::min(rhs_split, nrhs);
     int c0 = 0;
@@ -605,6 +645,28 @@ h_ops_tree* Hierarchical::generateOps_Solve(const h_index* self, int rhs_ptr_id,
       bld.forward(root, b);
       bld.backward(root, b);
       c0 += w;
+    }
+  } catch (...) {
+    delete tree;
+    throw;
+  }
+  return tree;
+}
+
+h_ops_tree* Hierarchical::generateOps_Invert(const h_index* self, int64_t inv_base, int slot) const {
+  if (self == nullptr || self->node != root) throw std::invalid_argument("generateOps_Invert: needs the root index");
+  if (inv_base < 0) throw std::invalid_argument("generateOps_Invert: negative base");
+  const int ms = slot >= 0 ? slot : ptr_id;
+  std::vector<inv_block> lay;
+  inverseLayout(lay);
+  h_ops_tree* tree = new h_ops_tree();
+  try {
+    for (const inv_block& e : lay) {
+      const dview leaf{ms, e.leaf_off, e.nb, e.nb, e.nb};
+      const dview linv{ms, inv_base + e.inv_off, e.nb, e.nb, e.nb};
+      const dview uinv{ms, inv_base + e.inv_off + align32((int64_t)e.nb * e.nb), e.nb, e.nb, e.nb};
+      emit_trsml(tree, mview{linv, false}, leaf);
+      emit_trsmu(tree, uinv, leaf);
     }
   } catch (...) {
     delete tree;

@@ -18,6 +18,9 @@
 //   trsmu    13  [ex, id, trsmu, P(B), P(U), off_b, off_u, nx_b, ny_b, ny_u, ld_b, ld_u, 0]
 //                (addition: left solve with the non-unit upper factor, used by
 //                 the backward substitution; the reference has no solve)
+//   trmm     11  [ex, id, trmm, P(B), P(T), off_b, off_t, nx_b, ny_b, ld_b, ld_t]
+//                (addition: B <- T B with a dense ny_b x ny_b T, used by the
+//                 solve that applies explicit inverses of the leaf factors)
 //   wait      3  [signal_wait, id, tick_hint]
 //   finish    1  [finish]
 // The numeric values of the opcodes and operations are not in the reference
@@ -45,11 +48,12 @@ enum operation_t : int {
   gemm_4x = 7,
   accum = 8,
   trsmu = 9,
+  trmm = 10,
 };
 
 // Entry lengths in words.
 constexpr int nop_l = 3, getrf_l = 8, trsml_l = 13, trsmr_l = 13, gemm_l = 17, gemm_plus_l = 17, gemm_3x_l = 23,
-              gemm_4x_l = 29, accum_l = 21, trsmu_l = 13, wait_l = 3, finish_l = 1;
+              gemm_4x_l = 29, accum_l = 21, trsmu_l = 13, trmm_l = 11, wait_l = 3, finish_l = 1;
 constexpr int _MAX_INST_LENGTH = 32;
 
 int operationLength(int op);
@@ -146,6 +150,8 @@ void emit_getrf(h_ops_tree* tree, const dview& m);
 void emit_trsml(h_ops_tree* tree, const mview& b, const dview& l);
 void emit_trsmr(h_ops_tree* tree, const mview& b, const dview& u);
 void emit_trsmu(h_ops_tree* tree, const dview& b, const dview& u);
+// B <- T B, T dense square with B's row count.
+void emit_trmm(h_ops_tree* tree, const dview& b, const dview& t);
 // M -= A B (plus == false) or M += A B (plus == true).
 void emit_gemm(h_ops_tree* tree, mview m, mview a, mview b, bool plus);
 // M -= A B C.

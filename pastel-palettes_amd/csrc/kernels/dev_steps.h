@@ -73,6 +73,7 @@ __device__ __forceinline__ int entry_length(int op) {
     case trsml: return trsml_l;
     case trsmr: return trsmr_l;
     case trsmu: return trsmu_l;
+    case trmm: return trmm_l;
     case gemm: return gemm_l;
     case gemm_plus: return gemm_plus_l;
     case gemm_3x: return gemm_3x_l;
@@ -203,6 +204,25 @@ __device__ __forceinline__ int make_step(const int32_t* pc, void* const* P, int 
       T* Bi = B + (int64_t)r0 * ld_b;
       if (r == 0) set_gemm<T>(st, true, bs, nx_b, bs, T(1), T(0), Bi, ld_b, tmp + (int64_t)i * NB * NB, NB, false, Bi, ld_b, false);
       else set_gemm<T>(st, true, r0, nx_b, bs, T(-1), T(1), B, ld_b, U + r0, ld_u, false, Bi, ld_b, false);
+      return 1;
+    }
+    case trmm: {
+      // B <- T B: copy B to the scratch, then B = T * copy
+      T* B = slot<T>(P, pc, 3) + word(pc, 5);
+      const T* Tm = slot<T>(P, pc, 4) + word(pc, 6);
+      const int nx_b = word(pc, 7), ny_b = word(pc, 8), ld_b = word(pc, 9), ld_t = word(pc, 10);
+      if (s == 0) {
+        st.kind = ST_COPY;
+        st.C = tmp;
+        st.ldc = nx_b;
+        st.A = B;
+        st.lda = ld_b;
+        st.m = ny_b;
+        st.n = nx_b;
+        return 1;
+      }
+      if (s > 1) return 0;
+      set_gemm<T>(st, true, ny_b, nx_b, ny_b, T(1), T(0), B, ld_b, Tm, ld_t, false, tmp, nx_b, false);
       return 1;
     }
     case gemm:

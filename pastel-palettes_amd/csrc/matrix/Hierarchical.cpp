@@ -126,3 +126,39 @@ int64_t Hierarchical::storedEntries() const {
   }
   return s;
 }
+
+namespace {
+
+// Dense leaves met along the diagonal, in the order the solve visits them.
+void collect_diagonal(const h_block* b, std::vector<const h_block*>& out) {
+  if (b->type == element_t::dense) {
+    if (b->nx != b->ny) throw std::invalid_argument("Hierarchical: non-square diagonal leaf");
+    out.push_back(b);
+    return;
+  }
+  if (b->type != element_t::hierarchical) throw std::invalid_argument("Hierarchical: non-dense diagonal leaf");
+  for (int i = 0; i < b->py; ++i) collect_diagonal(b->child(i, i), out);
+}
+
+int64_t round32(int64_t x) { return (x + 31) / 32 * 32; }
+
+}  // namespace
+
+void Hierarchical::inverseLayout(std::vector<inv_block>& out) const {
+  std::vector<const h_block*> diag;
+  collect_diagonal(root, diag);
+  out.clear();
+  int64_t cursor = 0;
+  for (const h_block* b : diag) {
+    out.push_back(inv_block{b->off, b->ny, cursor});
+    cursor += 2 * round32((int64_t)b->ny * b->ny);
+  }
+}
+
+int64_t Hierarchical::inverseStorage() const {
+  std::vector<inv_block> lay;
+  inverseLayout(lay);
+  int64_t s = 0;
+  for (const inv_block& e : lay) s += 2 * round32((int64_t)e.nb * e.nb);
+  return s;
+}

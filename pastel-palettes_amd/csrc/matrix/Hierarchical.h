@@ -86,8 +86,26 @@ public:
   // Flattened h-ops of the forward + backward substitution applied to the
   // nrhs-column row-major block at (rhs_ptr_id, rhs_off, ld = rhs_ld);
   // rhs_split > 1 cuts the columns into independent groups.
+  // inv_base >= 0 applies each dense diagonal leaf through the explicit
+  // inverses that generateOps_Invert leaves at inv_base of the matrix slot.
   h_ops_tree* generateOps_Solve(const h_index* self, int rhs_ptr_id, int64_t rhs_off, int nrhs, int rhs_ld,
-                                int rhs_split = 1, int slot = -1) const;
+                                int rhs_split = 1, int slot = -1, int64_t inv_base = -1) const;
+
+  // Explicit inverses of the factored dense diagonal leaves.  Leaf i of
+  // inverseLayout() keeps L^{-1} (unit lower factor) at inv_base + inv_off
+  // and U^{-1} at inv_base + inv_off + align32(nb * nb), row-major, ld = nb.
+  struct inv_block {
+    int64_t leaf_off;  // arena offset of the leaf
+    int nb;            // leaf order
+    int64_t inv_off;   // offset of L^{-1} within the inverse area
+  };
+  void inverseLayout(std::vector<inv_block>& out) const;
+  // Elements of the inverse area.
+  int64_t inverseStorage() const;
+  // Ops forming the inverses from the factored leaves (trsml / trsmu on
+  // identity blocks).  The inverse area must hold identity blocks before the
+  // ops run.
+  h_ops_tree* generateOps_Invert(const h_index* self, int64_t inv_base, int slot = -1) const;
 
   // Leaves in depth-first order.
   void leaves(std::vector<const h_block*>& out) const;

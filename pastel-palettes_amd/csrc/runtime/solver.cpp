@@ -9,6 +9,13 @@
 // Precision: the factor arena holds the storage type of the precision mode
 // (FP64, or FP32 for FP32 / mixed); the FP64 arena stays resident for the
 // residual r = b - A_H x, which is always computed in FP64.
+//
+// Each member's arena is followed by its leaf-inverse area.  The factor
+// program also forms L^{-1} and U^{-1} of every dense diagonal leaf there
+// (those ops depend only on the getrf of their own leaf and fill idle
+// workers), and the solve applies each leaf with one product instead of a
+// substitution.  The FP64 copy holds identity blocks in the inverse areas, so
+// the arena restore before every factorization re-seeds them.
 #include <chrono>
 #include <cmath>
 #include <cstring>
@@ -194,7 +201,8 @@ int solver_create(pp_ctx* ctx, const pp_hmatrix* const* hs, int count, const dou
     s->hms.push_back(hs[i]);
     s->arena_off.push_back(s->storage_total);
     s->row_off.push_back(s->n_total);
-    s->storage_total += (hier(hs[i])->getStorage() + 63) / 64 * 64;
+    const Hierarchical* h = hier(hs[i]);
+    s->storage_total += (inverse_base(h) + h->inverseStorage() + 63) / 64 * 64;
     s->n_total += hier(hs[i])->getNx_abs();
   }
   int rc = PP_OK;
@@ -202,10 +210,27 @@ int solver_create(pp_ctx* ctx, const pp_hmatrix* const* hs, int count, const dou
   do {
     if ((rc = ctx_alloc_t(ctx, s->storage_total, &s->d_A))) break;
     s->owned.push_back(s->d_A);
+    hipError_t ez = hipMemset(s->d_A, 0, 8 * s->storage_total);
+    if (ez != hipSuccess) {
+      rc = hip_fail(ez, "clear matrix arena");
+      break;
+    }
     for (int i = 0; i < count && rc == PP_OK; ++i) {
-      hipError_t e = hipMemcpy(s->d_A + s->arena_off[i], storages[i], 8 * hier(hs[i])->getStorage(),
+      const Hierarchical* h = hier(hs[i]);
+      hipError_t e = hipMemcpy(s->d_A + s->arena_off[i], storages[i], 8 * h->getStorage(),
                                on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice);
-      if (e != hipSuccess) rc = hip_fail(e, "upload matrix");
+      if (e != hipSuccess) {
+        rc = hip_fail(e, "upload matrix");
+        break;
+      }
+      std::vector<double> ident(h->inverseStorage());
+      if (pp_hmatrix_inverse_identity(hs[i], ident.data()) != PP_OK) {
+        rc = PP_ERR_INTERNAL;
+        break;
+      }
+      e = hipMemcpy(s->d_A + s->arena_off[i] + inverse_base(h), ident.data(), 8 * ident.size(),
+                    hipMemcpyHostToDevice);
+      if (e != hipSuccess) rc = hip_fail(e, "upload inverse identities");
     }
     if (rc) break;
     if ((rc = ctx_alloc(ctx, (uint64_t)s->storage_total * eb, &s->d_F))) break;
@@ -217,12 +242,23 @@ int solver_create(pp_ctx* ctx, const pp_hmatrix* const* hs, int count, const dou
     if ((rc = ctx_alloc_t(ctx, 2 * (int64_t)count * nrhs_max, &s->d_nrm))) break;
     s->owned.push_back(s->d_nrm);
     if ((rc = build_plan(s))) break;
-    if ((rc = pp_program_factor_batch(hs, count, workers, &s->fprog))) break;
+    if ((rc = pp_program_factor_inv_batch(hs, count, workers, &s->fprog))) break;
     std::vector<void*> ptrs = slot_table(s, nrhs_max);
     if ((rc = pp_stream_upload_program(ctx, s->fprog, const_cast<const void* const*>(ptrs.data()), (int)ptrs.size(),
                                        precision, &s->fstream)))
       break;
-    s->factor_flops = (double)s->fprog->dag->getFlops();
+    // H-LU FLOPs only: the leaf inversions (trsml + trsmu on identity
+    // blocks) are part of the solve's set-up
+    double inv_flops = 0;
+    for (int i = 0; i < count; ++i) {
+      std::vector<Hierarchical::inv_block> lay;
+      hier(hs[i])->inverseLayout(lay);
+      for (const auto& e : lay) {
+        const double nb = e.nb;
+        inv_flops += nb * nb * (nb - 1) + nb * nb * nb;
+      }
+    }
+    s->factor_flops = (double)s->fprog->dag->getFlops() - inv_flops;
     s->member_res.assign(count, 0.);
   } while (false);
   if (rc != PP_OK) {
@@ -238,8 +274,8 @@ int solve_prog(pp_solver* s, int nrhs, pp_solver::SolveProg** out) {
   if (it == s->sprogs.end()) {
     pp_solver::SolveProg sp;
     std::vector<int32_t> nr(s->hms.size(), nrhs), ld(s->hms.size(), nrhs);
-    int rc = pp_program_solve_batch(s->hms.data(), (int)s->hms.size(), nr.data(), ld.data(), s->rhs_split, s->workers,
-                                    &sp.prog);
+    int rc = pp_program_solve_inv_batch(s->hms.data(), (int)s->hms.size(), nr.data(), ld.data(), s->rhs_split,
+                                        s->workers, &sp.prog);
     if (rc) return rc;
     std::vector<void*> ptrs = slot_table(s, nrhs);
     rc = pp_stream_upload_program(s->ctx, sp.prog, const_cast<const void* const*>(ptrs.data()), (int)ptrs.size(),

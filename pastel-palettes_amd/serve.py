@@ -103,8 +103,10 @@ class Server:
         hm, _ = self.matrices[key]
         eb = 8 if self.precision == self.pp.FP64 else 4
         n = hm.n
-        # FP64 arena + factor arena + rhs/residual buffers + per-worker scratch (upper bound)
-        return hm.storage * (8 + eb) + n * self.nrhs_max * (eb + 8) + self.workers * 64 * 1024 * 1024 // 16
+        # FP64 arena + factor arena (both with the leaf-inverse area) + rhs/residual
+        # buffers + per-worker scratch (upper bound)
+        elems = getattr(hm, "solver_elems", hm.storage)
+        return elems * (8 + eb) + n * self.nrhs_max * (eb + 8) + self.workers * 64 * 1024 * 1024 // 16
 
     def _evict_for(self, need):
         """Evict idle resident factorizations (LRU) until `need` bytes fit."""

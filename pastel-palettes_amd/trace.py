@@ -11,13 +11,13 @@ import collections
 import numpy as np
 
 OP_NAMES = {0: "nop", 1: "getrf", 2: "trsml", 3: "trsmr", 4: "gemm", 5: "gemm_plus", 6: "gemm_3x", 7: "gemm_4x",
-            8: "accum", 9: "trsmu"}
+            8: "accum", 9: "trsmu", 10: "trmm"}
 TICK_US = 0.01  # 100 MHz
 
 
 def _entries(stream):
     out, i = [], 0
-    lens = {0: 3, 1: 8, 2: 13, 3: 13, 4: 17, 5: 17, 6: 23, 7: 29, 8: 21, 9: 13}
+    lens = {0: 3, 1: 8, 2: 13, 3: 13, 4: 17, 5: 17, 6: 23, 7: 29, 8: 21, 9: 13, 10: 11}
     while i < len(stream):
         op = int(stream[i])
         if op == 0:

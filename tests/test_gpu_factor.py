@@ -77,6 +77,28 @@ def test_hodlr_factor_parity(pp, orc, ctx, n, levels, rank, workers):
     assert worst < 1e-9
 
 
+def test_inverse_factor_parity(pp, orc, ctx):
+    """Factor program with leaf inversions: the factors and the inverse area
+    on the GPU equal the oracle running the same streams."""
+    print("[case] inverse factor hodlr n=2048 levels=3 rank=16 workers=16", flush=True)
+    hm, kern, arena = pp.build_config(1, 2048, levels=3, rank=16)
+    prog = pp.Program.factor_inv_batch([hm], 16)
+    full = hm.with_inverse_area(arena)
+    d = torch.from_numpy(full).cuda()
+    s = pp.Stream(ctx, prog, [d.data_ptr(), None, None], 0)
+    s.set_watchdog(10000)
+    s.launch(4)
+    s.wait()
+    torch.cuda.synchronize()
+    got = d.cpu().numpy()
+    want = oracle_factor(pp, orc, prog, full)
+    for g, o in zip(leaf_products(hm, got), leaf_products(hm, want)):
+        assert np.max(np.abs(g - o)) <= 1e-9 * max(np.max(np.abs(o)), 1e-300)
+    gi, oi = got[hm.inverse_base:], want[hm.inverse_base:]
+    assert np.max(np.abs(gi - oi)) <= 1e-10 * np.max(np.abs(oi))
+    s.close()
+
+
 def test_factor_repeatable(pp, ctx):
     """Two launches of the same stream (epoch-tagged flags, no reset) give
     bit-identical factors."""

@@ -117,6 +117,21 @@ def test_trsmu(ctx, orc, prec, ny, nx, wf):
     assert rel(host(db), orc.trsmu(b, u)) < TOL[prec] * 10
 
 
+@pytest.mark.parametrize("prec", [FP64, FP32, MIXED])
+@pytest.mark.parametrize("ny,nx,pad", [(256, 16, 0), (256, 1, 3), (131, 33, 0), (128, 64, 5), (100, 256, 0)])
+def test_trmm(ctx, orc, prec, ny, nx, pad):
+    rng = np.random.default_rng(ny + 11 * nx)
+    t = rng.standard_normal((ny, ny)) / np.sqrt(ny)
+    b = rng.standard_normal((ny, nx + pad))
+    dt, db = dev(t, prec), dev(b, prec)
+    run(ctx, prec, [10, 0, 1, 0, 0, nx, ny, nx + pad, ny], [db, dt])
+    got = host(db)
+    want = b.copy()
+    want[:, :nx] = orc.trmm(b[:, :nx], t)
+    # the leaf product runs at the factor precision: FP32 in FP32 and mixed
+    assert rel(got, want) < TOL[FP64 if prec == FP64 else FP32] * 10
+
+
 GEMM_SHAPES = [(256, 256, 256), (37, 53, 29), (4096, 32, 32), (32, 32, 4096), (1, 300, 17), (256, 256, 16),
                (65, 33, 100)]
 

@@ -178,3 +178,74 @@ def test_oracle_hodlr_factor_and_solve(pp, orc):
             break
         x = x + apply_inv(r)
     assert orc.dense_residual(a, x, b) < 1e-12
+
+
+def test_oracle_trmm_matches_numpy(orc):
+    """B <- T B of the oracle against numpy's product."""
+    rng = np.random.default_rng(11)
+    for ny, nx in [(1, 1), (7, 3), (64, 16), (33, 1)]:
+        t = rng.standard_normal((ny, ny))
+        b = rng.standard_normal((ny, nx))
+        assert np.allclose(orc.trmm(b, t), t @ b, rtol=1e-13, atol=1e-13)
+
+
+def test_oracle_inverse_factor_and_solve(pp, orc):
+    """The factor program with leaf inversions leaves the same factors as the
+    plain one (bit for bit), the inverse area holds L^{-1} and U^{-1} of every
+    diagonal leaf, and the solve through the inverses agrees with the
+    substitution solve and refines to 1e-12."""
+    n = 2048
+    hm, kern, arena = pp.build_config(1, n, levels=3, rank=32)
+    a = pp.densify(hm, arena)
+    rnd = pp.sketch(pp.SKETCH_SEED, 1 << 16)
+
+    prog = pp.Program.factor(hm, 4)
+    info = prog.info()
+    fac = arena.copy()
+    orc.run_streams(prog.streams(), [fac, np.zeros(max(info["temp_elems"], 1)), None], info["n_insts"],
+                    info["scratch_elems"], rnd=rnd)
+
+    iprog = pp.Program.factor_inv_batch([hm], 4)
+    iinfo = iprog.info()
+    ifac = hm.with_inverse_area(arena)
+    orc.run_streams(iprog.streams(), [ifac, np.zeros(max(iinfo["temp_elems"], 1)), None], iinfo["n_insts"],
+                    iinfo["scratch_elems"], rnd=rnd)
+    assert np.array_equal(ifac[:hm.storage], fac[:hm.storage])
+
+    base = hm.inverse_base
+    cursor = 0
+    for lf in hm.leaves():
+        if lf["type"] != "dense" or lf["row0"] != lf["col0"]:
+            continue
+        nb = lf["ny"]
+        lu = fac[lf["off"]:lf["off"] + nb * nb].reshape(nb, nb)
+        l, u = np.tril(lu, -1) + np.eye(nb), np.triu(lu)
+        second = (nb * nb + 31) // 32 * 32
+        linv = ifac[base + cursor:base + cursor + nb * nb].reshape(nb, nb)
+        uinv = ifac[base + cursor + second:base + cursor + second + nb * nb].reshape(nb, nb)
+        assert np.max(np.abs(linv @ l - np.eye(nb))) < 1e-12
+        assert np.max(np.abs(uinv @ u - np.eye(nb))) < 1e-10
+        cursor += 2 * second
+    assert cursor == ifac.size - base
+
+    nrhs = 4
+    b = pp.rhs(1, n, nrhs)
+    sprog = pp.Program.solve(hm, nrhs, 4)
+    isprog = pp.Program.solve_inv_batch([hm], [nrhs], 4)
+    kinds = {int(isprog.inst_words(i)[2]) for i in range(isprog.info()["n_insts"])}
+    assert 10 in kinds and 2 not in kinds and 9 not in kinds  # trmm replaces trsml / trsmu
+
+    def run(p, f, r):
+        x = np.ascontiguousarray(r.copy())
+        orc.run_streams(p.streams(), [f, None, x], p.info()["n_insts"], p.info()["scratch_elems"])
+        return x
+
+    x_sub, x_inv = run(sprog, fac, b), run(isprog, ifac, b)
+    assert np.linalg.norm(x_inv - x_sub) / np.linalg.norm(x_sub) < 1e-10
+    x = x_inv
+    for _ in range(12):
+        r = b - a @ x
+        if np.max(np.linalg.norm(r, axis=0) / np.linalg.norm(b, axis=0)) < 1e-12:
+            break
+        x = x + run(isprog, ifac, r)
+    assert orc.dense_residual(a, x, b) < 1e-12

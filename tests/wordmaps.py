@@ -25,6 +25,8 @@ def operand_maps(w):
         return [(False, _map(w[3], w[5], r, c, w[10])), (True, _map(w[4], w[6], k, k, w[11]))]
     if op == 9:  # trsmu
         return [(False, _map(w[3], w[5], w[8], w[7], w[10])), (True, _map(w[4], w[6], w[9], w[9], w[11]))]
+    if op == 10:  # trmm
+        return [(False, _map(w[3], w[5], w[8], w[7], w[9])), (True, _map(w[4], w[6], w[8], w[8], w[10]))]
     if op in (4, 5):  # gemm
         m, n, k = w[9], w[10], w[11]
         a = (k, m) if w[15] else (m, k)
@@ -55,7 +57,7 @@ def operand_maps(w):
 
 
 def entry_length(op):
-    return {0: 3, 1: 8, 2: 13, 3: 13, 4: 17, 5: 17, 6: 23, 7: 29, 8: 21, 9: 13}[op]
+    return {0: 3, 1: 8, 2: 13, 3: 13, 4: 17, 5: 17, 6: 23, 7: 29, 8: 21, 9: 13, 10: 11}[op]
 
 
 def split_stream(stream):

@@ -203,3 +203,19 @@ def dense_residual(a, x, b):
     x = np.ascontiguousarray(x, np.float64)
     b = np.ascontiguousarray(b, np.float64)
     return lib().or_dense_residual(_d(a), a.shape[0], _d(x), _d(b), x.shape[1], x.shape[1])
+
+
+def hmatvec(leaves, arena, x):
+    """y = A_H x from the leaf list of an H-matrix and its FP64 arena (dense
+    leaves D, low-rank leaves U V^T), in numpy."""
+    y = np.zeros_like(x)
+    for lf in leaves:
+        r0, c0, ny, nx = lf["row0"], lf["col0"], lf["ny"], lf["nx"]
+        if lf["type"] == "dense":
+            y[r0:r0 + ny] += arena[lf["off"]:lf["off"] + ny * nx].reshape(ny, nx) @ x[c0:c0 + nx]
+        else:
+            r = lf["rank"]
+            u = arena[lf["off"]:lf["off"] + ny * r].reshape(ny, r)
+            v = arena[lf["off_v"]:lf["off_v"] + nx * r].reshape(nx, r)
+            y[r0:r0 + ny] += u @ (v.T @ x[c0:c0 + nx])
+    return y

@@ -29,8 +29,9 @@ struct MatvecPlanDev {
   int n_tiles, n_lowrank;
 };
 
-hipError_t launch_vm(const VmArgs& a, int precision, int workers, int threads, uint32_t* epoch_dev, int32_t* status,
-                     hipStream_t s);
+// factor: launch under the factorization symbol (vm_factor_kernel)
+hipError_t launch_vm(const VmArgs& a, int precision, int workers, int threads, bool factor, uint32_t* epoch_dev,
+                     int32_t* status, hipStream_t s);
 int vm_blocks_per_cu(int precision, int threads);
 
 hipError_t launch_fill_zero(void* p, int64_t n, int elem_bytes, hipStream_t s);

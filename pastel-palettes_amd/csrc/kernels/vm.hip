@@ -17,8 +17,8 @@ __device__ void record_error(int32_t* status, int32_t code, int32_t worker, int3
   __hip_atomic_store(&status[3], 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-template <typename T, int FMODE, int GMODE, int NT>
-__global__ __launch_bounds__(NT) void vm_kernel(VmArgs a) {
+template <typename T, int FMODE, int GMODE>
+__device__ __forceinline__ void vm_run(const VmArgs& a) {
   __shared__ int32_t ctl[2];
   const int w = blockIdx.x;
   const int32_t* pc = a.words + a.stream_off[w];
@@ -88,6 +88,19 @@ __global__ __launch_bounds__(NT) void vm_kernel(VmArgs a) {
   }
 }
 
+template <typename T, int FMODE, int GMODE, int NT>
+__global__ __launch_bounds__(NT) void vm_kernel(VmArgs a) {
+  vm_run<T, FMODE, GMODE>(a);
+}
+
+// The same interpreter under a symbol of its own, launched for a solver's
+// factorization program, so that kernel profiles list factorization and
+// solve launches separately.
+template <typename T, int FMODE, int GMODE, int NT>
+__global__ __launch_bounds__(NT) void vm_factor_kernel(VmArgs a) {
+  vm_run<T, FMODE, GMODE>(a);
+}
+
 // Advances the launch epoch and clears the status words (a kernel rather
 // than a memset so that the sequence can be captured in a hipGraph).
 __global__ void bump_epoch_kernel(uint32_t* e, int32_t* status) {
@@ -98,28 +111,29 @@ __global__ void bump_epoch_kernel(uint32_t* e, int32_t* status) {
 // ---------------------------------------------------------------- launch wrappers
 
 template <typename T, int FMODE, int GMODE, int NT>
-static hipError_t launch_vm_t(const VmArgs& a, int workers, hipStream_t s) {
-  hipLaunchKernelGGL((vm_kernel<T, FMODE, GMODE, NT>), dim3(workers), dim3(NT), 0, s, a);
+static hipError_t launch_vm_t(const VmArgs& a, int workers, bool factor, hipStream_t s) {
+  if (factor) hipLaunchKernelGGL((vm_factor_kernel<T, FMODE, GMODE, NT>), dim3(workers), dim3(NT), 0, s, a);
+  else hipLaunchKernelGGL((vm_kernel<T, FMODE, GMODE, NT>), dim3(workers), dim3(NT), 0, s, a);
   return hipGetLastError();
 }
 
 template <typename T, int FMODE, int GMODE>
-static hipError_t launch_vm_nt(const VmArgs& a, int workers, int nt, hipStream_t s) {
+static hipError_t launch_vm_nt(const VmArgs& a, int workers, int nt, bool factor, hipStream_t s) {
   switch (nt) {
-    case 256: return launch_vm_t<T, FMODE, GMODE, 256>(a, workers, s);
+    case 256: return launch_vm_t<T, FMODE, GMODE, 256>(a, workers, factor, s);
     default: return hipErrorInvalidValue;
   }
 }
 
-hipError_t launch_vm(const VmArgs& a, int precision, int workers, int threads, uint32_t* epoch_dev, int32_t* status,
-                     hipStream_t s) {
+hipError_t launch_vm(const VmArgs& a, int precision, int workers, int threads, bool factor, uint32_t* epoch_dev,
+                     int32_t* status, hipStream_t s) {
   hipLaunchKernelGGL(bump_epoch_kernel, dim3(1), dim3(64), 0, s, epoch_dev, status);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   switch (precision) {
-    case 0: return launch_vm_nt<double, MODE_FP64, MODE_FP64>(a, workers, threads, s);
-    case 1: return launch_vm_nt<float, MODE_FP32, MODE_FP32>(a, workers, threads, s);
-    case 2: return launch_vm_nt<float, MODE_FP32, MODE_BF16>(a, workers, threads, s);
+    case 0: return launch_vm_nt<double, MODE_FP64, MODE_FP64>(a, workers, threads, factor, s);
+    case 1: return launch_vm_nt<float, MODE_FP32, MODE_FP32>(a, workers, threads, factor, s);
+    case 2: return launch_vm_nt<float, MODE_FP32, MODE_BF16>(a, workers, threads, factor, s);
     default: return hipErrorInvalidValue;
   }
 }

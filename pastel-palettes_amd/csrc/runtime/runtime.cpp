@@ -66,7 +66,7 @@ int enqueue_launch(pp_ctx* ctx, pp_stream* s, int wavefronts_per_inst, hipStream
   a.epoch = s->d_epoch;
   a.n_insts = s->n_insts;
   if (record_events) HIP_TRY(hipEventRecord(s->ev0, st));
-  hipError_t e = pp::launch_vm(a, s->precision, s->workers, threads, s->d_epoch, s->d_status, st);
+  hipError_t e = pp::launch_vm(a, s->precision, s->workers, threads, s->role == 1, s->d_epoch, s->d_status, st);
   if (e != hipSuccess) return hip_fail(e, "launch_vm");
   if (record_events) HIP_TRY(hipEventRecord(s->ev1, st));
   s->last = st;

@@ -79,6 +79,7 @@ struct pp_stream {
   int64_t spin_limit = 20LL * 100000000LL;  // 20 s of 100 MHz ticks
   hipStream_t last = nullptr;
   bool launched = false;
+  int role = 0;  // 1: a solver's factorization program (own kernel symbol)
 };
 
 namespace ppr {

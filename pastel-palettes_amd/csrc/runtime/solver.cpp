@@ -259,6 +259,7 @@ int solver_create(pp_ctx* ctx, const pp_hmatrix* const* hs, int count, const dou
       }
     }
     s->factor_flops = (double)s->fprog->dag->getFlops() - inv_flops;
+    s->fstream->role = 1;
     s->member_res.assign(count, 0.);
   } while (false);
   if (rc != PP_OK) {

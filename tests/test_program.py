@@ -249,3 +249,13 @@ def test_oracle_inverse_factor_and_solve(pp, orc):
             break
         x = x + run(isprog, ifac, r)
     assert orc.dense_residual(a, x, b) < 1e-12
+
+
+def test_oracle_hmatvec_matches_dense(pp, orc):
+    """The numpy H-matvec the CPU baseline refines with equals the densified
+    matrix times x."""
+    hm, kern, arena = pp.build_config(1, 1024, levels=2, rank=16)
+    x = pp.rhs(1, 1024, 3)
+    want = pp.densify(hm, arena) @ x
+    got = orc.hmatvec(hm.leaves(), arena, x)
+    assert np.max(np.abs(got - want)) <= 1e-12 * np.max(np.abs(want))

@@ -165,6 +165,11 @@ def main():
     ap.add_argument("--only", choices=["fp64", "mixed", "both"], default="both")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run only a plain 256 MiB device copy (PMC byte-counter calibration)")
+    ap.add_argument("--workload", choices=["hlu", "batch", "serve"], default="hlu",
+                    help="hlu: the N=16384 factor+solve step (default); batch: many matrices in one launch "
+                         "with autotuned leaves; serve: Poisson request stream")
+    ap.add_argument("--batch-count", type=int, default=512)
+    ap.add_argument("--serve-seconds", type=float, default=20.0)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path (run with --gpus 1)",
@@ -183,6 +188,15 @@ def main():
     pp = importlib.import_module("pastel-palettes_amd")
     ctx = pp.Context(0)
     workers = args.workers or min(ctx.cu_count, 256)
+    if args.workload != "hlu":
+        wl = importlib.import_module("pastel-palettes_amd.workloads")
+        if args.workload == "batch":
+            out = wl.run_batch(pp, torch, ctx, count=args.batch_count, steps=args.steps, warmup=args.warmup,
+                               log=log)
+        else:
+            out = wl.run_serve(pp, torch, ctx, duration=args.serve_seconds, log=log)
+        print(json.dumps(out), flush=True)
+        return
 
     t0 = time.perf_counter()
     hm, kern, arena = pp.build_config(1, args.n, levels=args.levels, rank=args.rank)

@@ -16,6 +16,7 @@
 // workers), and the solve applies each leaf with one product instead of a
 // substitution.  The FP64 copy holds identity blocks in the inverse areas, so
 // the arena restore before every factorization re-seeds them.
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <cstring>
@@ -270,6 +271,16 @@ int solver_create(pp_ctx* ctx, const pp_hmatrix* const* hs, int count, const dou
   return PP_OK;
 }
 
+// Solve programs are built for a few column widths only (powers of two up to
+// nrhs_max): a request with fewer columns runs the next wider program over
+// zero-padded right-hand sides (columns never mix), so a stream of requests
+// with varying column counts reuses a handful of programs.
+int program_width(const pp_solver* s, int nrhs) {
+  int w = 1;
+  while (w < nrhs) w *= 2;
+  return std::min(w, s->nrhs_max);
+}
+
 int solve_prog(pp_solver* s, int nrhs, pp_solver::SolveProg** out) {
   auto it = s->sprogs.find(nrhs);
   if (it == s->sprogs.end()) {
@@ -425,17 +436,22 @@ int pp_solver_solve(pp_solver* s, const double* b, double* x, int nrhs, int ld, 
     const int eb = elem_bytes(s->precision);
     const int M = (int)s->hms.size();
     pp_solver::SolveProg* sp = nullptr;
-    int rc = solve_prog(s, nrhs, &sp);
+    const int w = program_width(s, nrhs);
+    int rc = solve_prog(s, w, &sp);
     if (rc) return rc;
     std::vector<double> bn(M * nrhs), rn(M * nrhs);
     double* d_bn = s->d_nrm + (int64_t)M * s->nrhs_max;
     hipError_t e = pp::launch_segnorm2(b, ld, s->d_seg, M, nrhs, d_bn, st);
     if (e != hipSuccess) return hip_fail(e, "norm b");
     // x = F^{-1} b
-    e = pp::launch_copy2d(s->d_rhs, eb, nrhs, b, 8, ld, (int)s->n_total, nrhs, st);
+    if (w > nrhs) {
+      e = pp::launch_fill_zero(s->d_rhs, s->n_total * w, eb, st);
+      if (e != hipSuccess) return hip_fail(e, "clear rhs");
+    }
+    e = pp::launch_copy2d(s->d_rhs, eb, w, b, 8, ld, (int)s->n_total, nrhs, st);
     if (e != hipSuccess) return hip_fail(e, "copy rhs");
     if ((rc = enqueue_launch(s->ctx, sp->stream, s->wpi, st, true))) return rc;
-    e = pp::launch_copy2d(x, 8, ld, s->d_rhs, eb, nrhs, (int)s->n_total, nrhs, st);
+    e = pp::launch_copy2d(x, 8, ld, s->d_rhs, eb, w, (int)s->n_total, nrhs, st);
     if (e != hipSuccess) return hip_fail(e, "copy x");
     HIP_TRY(hipMemcpyAsync(bn.data(), d_bn, 8 * bn.size(), hipMemcpyDeviceToHost, st));
     HIP_TRY(hipStreamSynchronize(st));
@@ -467,10 +483,10 @@ int pp_solver_solve(pp_solver* s, const double* b, double* x, int nrhs, int ld, 
       }
       if (it >= max_iter) break;
       ++it;
-      e = pp::launch_copy2d(s->d_rhs, eb, nrhs, s->d_r, 8, nrhs, (int)s->n_total, nrhs, st);
+      e = pp::launch_copy2d(s->d_rhs, eb, w, s->d_r, 8, nrhs, (int)s->n_total, nrhs, st);
       if (e != hipSuccess) return hip_fail(e, "copy residual");
       if ((rc = enqueue_launch(s->ctx, sp->stream, s->wpi, st, true))) return rc;
-      e = pp::launch_add_correction(x, ld, s->d_rhs, eb, nrhs, (int)s->n_total, nrhs, st);
+      e = pp::launch_add_correction(x, ld, s->d_rhs, eb, w, (int)s->n_total, nrhs, st);
       if (e != hipSuccess) return hip_fail(e, "update x");
     }
     if ((rc = check_status(sp->stream))) return rc;

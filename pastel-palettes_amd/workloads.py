@@ -1,0 +1,194 @@
+"""Batched and serving benchmark drivers (the many-matrix workloads beside the
+single N = 16384 H-LU step of bench.py).
+
+batch: `count` independent HODLR matrices, N log-uniform in [512, 8192]
+    rounded to a multiple of 256, rank drawn from {16, 24, 32}, 4 right-hand
+    sides each, leaf order per size chosen by the autotuner; one batched
+    solver (one persistent factor launch replayed from a captured hipGraph,
+    one solve launch per refinement sweep), FP32 factorization with FP64
+    refinement to the tolerance.
+serve: a Poisson stream of factor + solve requests over a pool of resident
+    matrices with N log-uniform in [1024, 32768], 1-64 right-hand sides, 30 %
+    of requests reusing a resident factorization; mixed precision with FP64
+    refinement; HBM-budgeted admission.  The open-loop rate is a fraction of
+    the peak rate measured closed-loop on the same pool.
+
+Matrices are built on the host (not timed) by a pool of spawned processes.
+"""
+import math
+import multiprocessing as mp
+import os
+import time
+
+import numpy as np
+
+
+def _build_arena(job):
+    """(n, levels, rank, point seed) -> FP64 arena; runs in a worker process."""
+    import importlib
+    import sys
+    n, levels, rank, seed = job
+    sys.path.insert(0, os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+    pp = importlib.import_module("pastel-palettes_amd")
+    hm = pp.HMatrix("hodlr", n, levels=levels, rank=rank)
+    return pp.fill_arena(hm, pp.Kernel(pp.points_2d_morton(seed, n), "inv2d", n / 4.0))
+
+
+def build_arenas(jobs, procs):
+    if procs <= 1 or len(jobs) == 1:
+        return [_build_arena(j) for j in jobs]
+    # one BLAS thread per worker process: spinning BLAS thread pools in every
+    # worker oversubscribe the cores many times over
+    keys = ("OMP_NUM_THREADS", "OPENBLAS_NUM_THREADS", "MKL_NUM_THREADS")
+    saved = {k: os.environ.get(k) for k in keys}
+    for k in keys:
+        os.environ[k] = "1"
+    try:
+        with mp.get_context("spawn").Pool(procs) as pool:
+            # largest first, so the long builds do not trail at the end
+            order = sorted(range(len(jobs)), key=lambda i: -jobs[i][0])
+            res = pool.map(_build_arena, [jobs[i] for i in order], chunksize=1)
+    finally:
+        for k, v in saved.items():
+            if v is None:
+                os.environ.pop(k, None)
+            else:
+                os.environ[k] = v
+    out = [None] * len(jobs)
+    for i, a in zip(order, res):
+        out[i] = a
+    return out
+
+
+def log_uniform_sizes(rng, count, lo, hi, multiple):
+    s = np.exp(rng.uniform(math.log(lo), math.log(hi), count))
+    return [int(min(hi, max(lo, multiple * round(v / multiple)))) for v in s]
+
+
+def host_procs():
+    return max(1, min(len(os.sched_getaffinity(0)), 16))
+
+
+def run_batch(pp, torch, ctx, count=512, nrhs=4, steps=3, warmup=1, tol=1e-10, precision=None, log=print):
+    from .tune import Autotuner
+    precision = pp.FP32 if precision is None else precision
+    rng = np.random.default_rng(3000)
+    sizes = log_uniform_sizes(rng, count, 512, 8192, 256)
+    ranks = [int(r) for r in rng.choice([16, 24, 32], count)]
+    workers = min(ctx.cu_count, 256)
+    t0 = time.perf_counter()
+    tuner = Autotuner(pp, torch, ctx, precision, rank=32, nrhs=nrhs, tol=tol, workers=workers, log=log)
+    choices = [tuner.choose(n) for n in sizes]
+    tune_s = time.perf_counter() - t0
+    jobs = [(n, c["levels"], r, 100 + i) for i, (n, c, r) in enumerate(zip(sizes, choices, ranks))]
+    log(f"[batch] building {count} matrices on {host_procs()} host processes")
+    t0 = time.perf_counter()
+    arenas = build_arenas(jobs, host_procs())
+    build_s = time.perf_counter() - t0
+    log(f"[batch] built in {build_s:.1f} s; creating the batched solver")
+    hms = [pp.HMatrix("hodlr", n, levels=lv, rank=r) for n, lv, r, _ in jobs]
+    t0 = time.perf_counter()
+    solver = pp.Solver.batch(ctx, hms, arenas, precision=precision, workers=workers, nrhs_max=nrhs)
+    solver.use_graph(True)
+    setup_s = time.perf_counter() - t0
+    log(f"[batch] solver ready in {setup_s:.1f} s ({int(sum(sizes))} rows)")
+    del arenas
+    b = np.concatenate([pp.rhs(4 + (i % 64), n, nrhs) for i, n in enumerate(sizes)], axis=0)
+    db = torch.from_numpy(b).cuda()
+    dx = torch.zeros_like(db)
+    stream = torch.cuda.Stream()
+    for _ in range(warmup):
+        solver.factor(stream.cuda_stream)
+        solver.solve(db.data_ptr(), dx.data_ptr(), nrhs, tol=tol, max_iter=60, hip_stream=stream.cuda_stream)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    fms, reps = [], []
+    for _ in range(steps):
+        solver.factor(stream.cuda_stream)
+        fms.append(solver.last_factor_ms())
+        reps.append(solver.solve(db.data_ptr(), dx.data_ptr(), nrhs, tol=tol, max_iter=60,
+                                 hip_stream=stream.cuda_stream))
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    flops = solver.factor_flops
+    leaf_hist = {}
+    for c in choices:
+        leaf_hist[c["leaf"]] = leaf_hist.get(c["leaf"], 0) + 1
+    out = {
+        "workload": f"batch of {count} HODLR matrices, N log-uniform [512, 8192] (multiple of 256), "
+                    f"rank 16/24/32, {nrhs} rhs, FP32 factor + FP64 refinement to {tol:g}, hipGraph factor launch",
+        "matrices_per_s": count * steps / dt,
+        "ms_per_batch": 1e3 * dt / steps,
+        "factor_ms": float(np.mean(fms)),
+        "factor_tflops": flops / (float(np.mean(fms)) * 1e-3) / 1e12,
+        "total_rows": int(sum(sizes)),
+        "residual": max(r["residual"] for r in reps),
+        "iterations": max(r["iterations"] for r in reps),
+        "converged": all(r["converged"] for r in reps),
+        "leaf_choices": leaf_hist,
+        "tuned_buckets": {k: {"leaf": v["leaf"], "ms_per_matrix": v["ms_per_matrix"],
+                              "candidates": v["candidates"]} for k, v in tuner.cache.items()},
+        "tune_s": tune_s, "build_s": build_s, "setup_s": setup_s, "workers": workers,
+    }
+    solver.close()
+    return out
+
+
+def run_serve(pp, torch, ctx, keys=12, duration=20.0, load=0.8, burst=48, slots=4, tol=1e-10, budget_gb=200,
+              log=print):
+    from . import serve
+    rng = np.random.default_rng(5000)
+    sizes = log_uniform_sizes(rng, keys, 1024, 32768, 256)
+    jobs = [(n, max(1, int(round(math.log2(n / 256)))), 32, 200 + i) for i, n in enumerate(sizes)]
+    log(f"[serve] building {keys} matrices (N = {sizes})")
+    t0 = time.perf_counter()
+    arenas = build_arenas(jobs, host_procs())
+    build_s = time.perf_counter() - t0
+    log(f"[serve] built in {build_s:.1f} s")
+    matrices = {}
+    for i, ((n, lv, r, _), a) in enumerate(zip(jobs, arenas)):
+        matrices[f"m{i}"] = (pp.HMatrix("hodlr", n, levels=lv, rank=r), torch.from_numpy(a).cuda())
+    del arenas
+    names = sorted(matrices)
+    workers = min(ctx.cu_count, 256) // slots
+    server = serve.Server(pp, torch, ctx, matrices, pp.MIXED, int(budget_gb * 1e9), slots=slots,
+                          workers_per_slot=workers, tol=tol)
+
+    def make_rhs(key, nrhs, k):
+        return pp.rhs(7 + k % 97, matrices[key][0].n, nrhs)
+
+    try:
+        # warm: every matrix factored once and resident
+        serve.replay(server, [(0.0, k, 1, False) for k in names], make_rhs, realtime=False)
+        log("[serve] all matrices resident")
+        # peak: a closed burst (all requests queued at once) of the same mix
+        tr = serve.poisson_trace(1e9, burst / 1e9, names, seed=11)[:burst]
+        reqs, wall = serve.replay(server, [(0.0, k, nr, ru) for _, k, nr, ru in tr], make_rhs, realtime=False)
+        peak = serve.summarize(reqs, wall)
+        rate = load * peak["throughput_rps"]
+        log(f"[serve] peak {peak['throughput_rps']:.1f} req/s; open loop at {rate:.1f} req/s for {duration:.0f} s")
+        trace = serve.poisson_trace(rate, duration, names, seed=12)
+        reqs, wall = serve.replay(server, trace, make_rhs, realtime=True)
+        s = serve.summarize(reqs, wall)
+        by_size = {}
+        for q in reqs:
+            if q.error is None:
+                by_size.setdefault(matrices[q.key][0].n, []).append(q.latency)
+        per_n = {str(n): {"count": len(v), "p50_ms": 1e3 * float(np.percentile(v, 50)),
+                          "p99_ms": 1e3 * float(np.percentile(v, 99))} for n, v in sorted(by_size.items())}
+    finally:
+        server.close()
+    return {
+        "workload": f"serving: Poisson requests over {keys} resident HODLR matrices, N log-uniform [1024, 32768], "
+                    f"rank 32, leaf ~256, 1-64 rhs, 30% reuse, mixed precision + FP64 refinement to {tol:g}, "
+                    f"{slots} slots x {workers} workers, HBM budget {budget_gb} GB",
+        "sizes": sizes,
+        "peak_rps": peak["throughput_rps"],
+        "rate_rps": rate,
+        "throughput_rps": s["throughput_rps"],
+        "p50_ms": s["p50_ms"], "p99_ms": s["p99_ms"],
+        "p99_over_p50": s["p99_ms"] / s["p50_ms"] if s["p50_ms"] else None,
+        "requests": s["requests"], "errors": s["errors"], "factored": s["factored"],
+        "max_residual": s["max_residual"], "converged": s["converged"],
+        "per_size": per_n, "evictions": server.evictions, "build_s": build_s,
+    }

@@ -145,6 +145,28 @@ def test_solve_refined(pp, orc, ctx, precision, cfg, n):
     assert err < 1e-8
 
 
+def test_padded_solve_widths(pp, ctx):
+    """Requests with 3 and 11 columns run the 4- and 16-wide solve programs
+    over zero-padded right-hand sides; each column equals the same column
+    solved in a full 16-column request."""
+    print("[case] padded solve widths n=2048", flush=True)
+    hm, kern, arena = pp.build_config(1, 2048, levels=3, rank=16)
+    solver = pp.Solver(ctx, hm, arena, precision=2, workers=16, nrhs_max=16)
+    solver.factor()
+    b = pp.rhs(1, 2048, 16)
+    full = torch.zeros(2048, 16, dtype=torch.float64, device="cuda")
+    rep = solver.solve(torch.from_numpy(b).cuda().data_ptr(), full.data_ptr(), 16, tol=1e-12, max_iter=40)
+    assert rep["converged"]
+    want = full.cpu().numpy()
+    for k in (3, 11):
+        db = torch.from_numpy(np.ascontiguousarray(b[:, :k])).cuda()
+        dx = torch.zeros_like(db)
+        r = solver.solve(db.data_ptr(), dx.data_ptr(), k, tol=1e-12, max_iter=40)
+        assert r["converged"]
+        got = dx.cpu().numpy()
+        assert np.linalg.norm(got - want[:, :k]) <= 1e-10 * np.linalg.norm(want[:, :k])
+
+
 def test_matvec_matches_dense(pp, ctx):
     hm, kern, arena = pp.build_config(1, 2048, levels=3, rank=16)
     solver = pp.Solver(ctx, hm, arena, precision=0, workers=8, nrhs_max=4)

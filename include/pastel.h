@@ -97,10 +97,11 @@ int pp_program_factor_batch(const pp_hmatrix* const* hs, int count, int workers,
 int pp_program_solve_batch(const pp_hmatrix* const* hs, int count, const int32_t* nrhs, const int32_t* ld,
                            int rhs_split, int workers, pp_program** out);
 
-/* Solve through explicit leaf inverses (no counterpart in the reference,
- * which stops at the factorization).  The factor program below also forms,
- * for every dense diagonal leaf, L^{-1} of its unit lower factor and U^{-1}
- * of its upper factor, in an inverse area that starts at element
+/* Solve through explicit inverses (no counterpart in the reference, which
+ * stops at the factorization).  The factor program below also forms, for
+ * every dense diagonal leaf, L^{-1} of its unit lower factor and U^{-1} of its
+ * upper factor, and for HODLR-shaped matrices the low-rank off-diagonal
+ * blocks of the full L^{-1} and U^{-1}, in an inverse area that starts at element
  * pp_hmatrix_inverse_base(h) of the matrix slot and holds
  * pp_hmatrix_inverse_elems(h) elements; the area must hold the identity
  * blocks of pp_hmatrix_inverse_identity before each run.  The solve program
@@ -110,6 +111,17 @@ int64_t pp_hmatrix_inverse_elems(const pp_hmatrix* h);
 /* Writes the identity blocks (zeros elsewhere) into a host area of
  * pp_hmatrix_inverse_elems(h) doubles. */
 int pp_hmatrix_inverse_identity(const pp_hmatrix* h, double* area);
+/* Layout of the inverse area.  Leaf i (solve order): info = {row0, nb,
+ * offset of L^{-1} (U^{-1} follows at + nb*nb rounded up to 32), arena offset
+ * of the leaf}.  For HODLR-shaped matrices the area also holds, per internal
+ * node i (children before parents), the low-rank factors of the off-diagonal
+ * blocks of L^{-1} (-l_u l_v^T, rows of the second child x columns of the
+ * first) and U^{-1} (-u_u u_v^T, rows of the first child x columns of the
+ * second): info = {row0, na, nb, rl, ru, l_u, l_v, u_u, u_v} (offsets within
+ * the area, ld = rank).  Non-HODLR-shaped matrices have no nodes. */
+int pp_hmatrix_inverse_count(const pp_hmatrix* h, int* leaves, int* nodes);
+int pp_hmatrix_inverse_leaf(const pp_hmatrix* h, int i, int64_t info[4]);
+int pp_hmatrix_inverse_node(const pp_hmatrix* h, int i, int64_t info[9]);
 int pp_program_factor_inv_batch(const pp_hmatrix* const* hs, int count, int workers, pp_program** out);
 int pp_program_solve_inv_batch(const pp_hmatrix* const* hs, int count, const int32_t* nrhs, const int32_t* ld,
                                int rhs_split, int workers, pp_program** out);

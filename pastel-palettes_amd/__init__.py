@@ -55,6 +55,9 @@ def lib():
         "pp_hmatrix_inverse_base": (i64, [vp]),
         "pp_hmatrix_inverse_elems": (i64, [vp]),
         "pp_hmatrix_inverse_identity": (C.c_int, [vp, P(dp)]),
+        "pp_hmatrix_inverse_count": (C.c_int, [vp, P(C.c_int), P(C.c_int)]),
+        "pp_hmatrix_inverse_leaf": (C.c_int, [vp, C.c_int, P(i64)]),
+        "pp_hmatrix_inverse_node": (C.c_int, [vp, C.c_int, P(i64)]),
         "pp_hmatrix_num_leaves": (C.c_int, [vp]),
         "pp_hmatrix_leaf": (C.c_int, [vp, C.c_int, P(i64)]),
         "pp_program_factor": (C.c_int, [vp, C.c_int, P(vp)]),
@@ -234,6 +237,22 @@ class HMatrix:
     def solver_elems(self):
         """Arena elements a Solver keeps per matrix (arena + inverse area)."""
         return self.inverse_base + int(lib().pp_hmatrix_inverse_elems(self.h))
+
+    def inverse_layout(self):
+        """(leaves, nodes) of the inverse area: leaves as dicts row0, nb, inv_off,
+        leaf_off; nodes as dicts row0, na, nb, rl, ru, l_u, l_v, u_u, u_v."""
+        L = lib()
+        nl, nn = C.c_int(), C.c_int()
+        check(L.pp_hmatrix_inverse_count(self.h, C.byref(nl), C.byref(nn)))
+        a4, a9 = (C.c_int64 * 4)(), (C.c_int64 * 9)()
+        leaves, nodes = [], []
+        for i in range(nl.value):
+            check(L.pp_hmatrix_inverse_leaf(self.h, i, a4))
+            leaves.append(dict(zip(("row0", "nb", "inv_off", "leaf_off"), list(a4))))
+        for i in range(nn.value):
+            check(L.pp_hmatrix_inverse_node(self.h, i, a9))
+            nodes.append(dict(zip(("row0", "na", "nb", "rl", "ru", "l_u", "l_v", "u_u", "u_v"), list(a9))))
+        return leaves, nodes
 
     def inverse_identity(self):
         """Initial contents of the leaf-inverse area (identity blocks)."""

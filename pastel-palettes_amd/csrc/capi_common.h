@@ -35,6 +35,7 @@ struct pp_program {
   int n_slots = 0;
   std::vector<int64_t> temp_elems;  // per batch member (slot 3*i+1)
   int64_t scratch = 0;              // per-worker scratch elements
+  long long primary_flops = -1;     // FLOPs of the factorization ops alone (factor + inverse programs)
   double dag_ms = 0, schedule_ms = 0, encode_ms = 0;
   ~pp_program();
 };

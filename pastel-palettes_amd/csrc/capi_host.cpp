@@ -163,6 +163,52 @@ int64_t pp_hmatrix_storage(const pp_hmatrix* h) { return h ? h->h->getStorage() 
 int64_t pp_hmatrix_inverse_base(const pp_hmatrix* h) { return h ? inverse_base(h->h.get()) : PP_ERR_INVALID; }
 int64_t pp_hmatrix_inverse_elems(const pp_hmatrix* h) { return h ? h->h->inverseStorage() : PP_ERR_INVALID; }
 
+int pp_hmatrix_inverse_count(const pp_hmatrix* h, int* leaves, int* nodes) {
+  try {
+    if (h == nullptr) return set_error(PP_ERR_INVALID, "null matrix");
+    std::vector<Hierarchical::inv_block> lay;
+    std::vector<Hierarchical::inv_node> nd;
+    h->h->inverseLayout(lay);
+    h->h->inverseNodes(nd);
+    if (leaves) *leaves = (int)lay.size();
+    if (nodes) *nodes = (int)nd.size();
+    return PP_OK;
+  } catch (...) {
+    return from_exception();
+  }
+}
+
+int pp_hmatrix_inverse_leaf(const pp_hmatrix* h, int i, int64_t info[4]) {
+  try {
+    if (h == nullptr || info == nullptr) return set_error(PP_ERR_INVALID, "null argument");
+    std::vector<Hierarchical::inv_block> lay;
+    h->h->inverseLayout(lay);
+    if (i < 0 || i >= (int)lay.size()) return set_error(PP_ERR_INVALID, "leaf index out of range");
+    info[0] = lay[i].row0;
+    info[1] = lay[i].nb;
+    info[2] = lay[i].inv_off;
+    info[3] = lay[i].leaf_off;
+    return PP_OK;
+  } catch (...) {
+    return from_exception();
+  }
+}
+
+int pp_hmatrix_inverse_node(const pp_hmatrix* h, int i, int64_t info[9]) {
+  try {
+    if (h == nullptr || info == nullptr) return set_error(PP_ERR_INVALID, "null argument");
+    std::vector<Hierarchical::inv_node> nd;
+    h->h->inverseNodes(nd);
+    if (i < 0 || i >= (int)nd.size()) return set_error(PP_ERR_INVALID, "node index out of range");
+    const auto& n = nd[i];
+    const int64_t v[9] = {n.row0, n.na, n.nb, n.rl, n.ru, n.l_u, n.l_v, n.u_u, n.u_v};
+    for (int k = 0; k < 9; ++k) info[k] = v[k];
+    return PP_OK;
+  } catch (...) {
+    return from_exception();
+  }
+}
+
 int pp_hmatrix_inverse_identity(const pp_hmatrix* h, double* area) {
   try {
     if (h == nullptr || area == nullptr) return set_error(PP_ERR_INVALID, "null argument");
@@ -258,21 +304,28 @@ int pp_program_factor_inv_batch(const pp_hmatrix* const* hs, int count, int work
     if (hs == nullptr || count < 1 || workers < 1 || out == nullptr) return set_error(PP_ERR_INVALID, "bad args");
     auto p = std::make_unique<pp_program>();
     p->n_slots = 3 * count;
-    h_ops_tree inv_ops;
+    std::vector<dev_temp> tmps;
     for (int i = 0; i < count; ++i) {
       if (hs[i] == nullptr) return set_error(PP_ERR_INVALID, "null matrix in batch");
-      dev_temp tmp(3 * i + 1);
+      tmps.emplace_back(3 * i + 1);
       const Hierarchical* h = hs[i]->h.get();
       std::unique_ptr<const h_index> root(h->getRootIndex());
-      std::unique_ptr<h_ops_tree> t(h->generateOps_GETRF(root.get(), &tmp, 3 * i));
+      std::unique_ptr<h_ops_tree> t(h->generateOps_GETRF(root.get(), &tmps[i], 3 * i));
       p->tree.append(*t);
-      std::unique_ptr<h_ops_tree> v(h->generateOps_Invert(root.get(), inverse_base(h), 3 * i));
-      inv_ops.append(*v);
-      p->temp_elems.push_back(tmp.size());
     }
+    p->primary_flops = p->tree.getFlops();
     // the inversions follow every factorization in program order, so each
-    // depends only on the getrf of its own leaf
-    p->tree.append(inv_ops);
+    // depends only on the factor blocks it reads; HODLR-shaped matrices get
+    // the full hierarchical inverses, others the leaf inverses only
+    for (int i = 0; i < count; ++i) {
+      const Hierarchical* h = hs[i]->h.get();
+      std::unique_ptr<const h_index> root(h->getRootIndex());
+      std::unique_ptr<h_ops_tree> v(h->hodlrShaped()
+                                        ? h->generateOps_InvertH(root.get(), inverse_base(h), &tmps[i], 3 * i)
+                                        : h->generateOps_Invert(root.get(), inverse_base(h), 3 * i));
+      p->tree.append(*v);
+      p->temp_elems.push_back(tmps[i].size());
+    }
     finish_program(p.get(), workers);
     *out = p.release();
     return PP_OK;

@@ -674,3 +674,122 @@ h_ops_tree* Hierarchical::generateOps_Invert(const h_index* self, int64_t inv_ba
   }
   return tree;
 }
+
+namespace {
+
+// Emits R += op(Inv(X)) P for the triangular factor inverse of subtree X
+// (lower: L^{-1}, upper: U^{-1}; op = transpose when `trans`), P and R with
+// X's rows.  Diagonal leaves use the explicit leaf inverses; every internal
+// node y inside X contributes t = F_in^T P[src]; R[dst] -= F_out t, split by
+// the diagonal leaves of dst so that the row blocks proceed independently.
+class inverse_emitter {
+public:
+  h_ops_tree* tree;
+  dev_temp* tmp;
+  int ms;
+  int64_t base;
+  std::vector<Hierarchical::inv_block> leaves;
+  std::vector<Hierarchical::inv_node> nodes;
+
+  const Hierarchical::inv_block& leaf_of(const h_block* b) const {
+    for (const auto& e : leaves)
+      if (e.leaf_off == b->off) return e;
+    throw std::invalid_argument("inverse: unknown leaf");
+  }
+  const Hierarchical::inv_node& node_of(const h_block* b) const {
+    for (const auto& n : nodes)
+      if (n.node == b) return n;
+    throw std::invalid_argument("inverse: unknown node");
+  }
+  dview leaf_inverse(const h_block* b, bool upper) const {
+    const auto& e = leaf_of(b);
+    return dview{ms, base + e.inv_off + (upper ? align32((int64_t)e.nb * e.nb) : 0), e.nb, e.nb, e.nb};
+  }
+  static void diagonal(const h_block* x, std::vector<const h_block*>& out) {
+    if (x->type == element_t::dense) {
+      out.push_back(x);
+      return;
+    }
+    diagonal(x->child(0, 0), out);
+    diagonal(x->child(1, 1), out);
+  }
+  static void internal(const h_block* x, std::vector<const h_block*>& out) {
+    if (x->type != element_t::hierarchical) return;
+    internal(x->child(0, 0), out);
+    internal(x->child(1, 1), out);
+    out.push_back(x);
+  }
+
+  void apply(const h_block* x, bool upper, bool trans, const dview& p, const dview& r) {
+    const int cols = p.nx;
+    std::vector<const h_block*> diag;
+    diagonal(x, diag);
+    for (const h_block* e : diag) {
+      const int o = e->row0 - x->row0;
+      emit_gemm(tree, mview{r.sub(o, 0, e->ny, cols), false}, mview{leaf_inverse(e, upper), trans},
+                mview{p.sub(o, 0, e->ny, cols), false}, true);
+    }
+    std::vector<const h_block*> inner;
+    internal(x, inner);
+    for (const h_block* y : inner) {
+      const auto& n = node_of(y);
+      const int oa = (int)(n.row0 - x->row0), ob = oa + n.na;
+      const int rank = upper ? n.ru : n.rl;
+      const bool src_is_a = upper == trans;  // lower: A -> B; upper: B -> A; transposed swaps
+      const int src_o = src_is_a ? oa : ob, src_n = src_is_a ? n.na : n.nb;
+      const int dst_o = src_is_a ? ob : oa, dst_n = src_is_a ? n.nb : n.na;
+      // the factor with the source rows and the one with the destination rows
+      int64_t f_a, f_b;  // factor with A's rows, factor with B's rows
+      if (upper) {
+        f_a = n.u_u;
+        f_b = n.u_v;
+      } else {
+        f_a = n.l_v;
+        f_b = n.l_u;
+      }
+      const dview fin{ms, base + (src_is_a ? f_a : f_b), src_n, rank, rank};
+      const dview fout{ms, base + (src_is_a ? f_b : f_a), dst_n, rank, rank};
+      const dview t = tmp->request(rank, cols);
+      emit_gemm(tree, mview{t, false}, mview{fin, true}, mview{p.sub(src_o, 0, src_n, cols), false}, true);
+      std::vector<const h_block*> dleaves;
+      diagonal(src_is_a ? y->child(1, 1) : y->child(0, 0), dleaves);
+      for (const h_block* e : dleaves) {
+        const int o = e->row0 - x->row0;
+        emit_gemm(tree, mview{r.sub(o, 0, e->ny, cols), false}, mview{fout.sub(o - dst_o, 0, e->ny, rank), false},
+                  mview{t, false}, false);
+      }
+    }
+  }
+};
+
+}  // namespace
+
+h_ops_tree* Hierarchical::generateOps_InvertH(const h_index* self, int64_t inv_base, dev_temp* tmp, int slot) const {
+  if (tmp == nullptr) throw std::invalid_argument("generateOps_InvertH: null temp");
+  h_ops_tree* tree = generateOps_Invert(self, inv_base, slot);
+  try {
+    inverse_emitter em;
+    em.tree = tree;
+    em.tmp = tmp;
+    em.ms = slot >= 0 ? slot : ptr_id;
+    em.base = inv_base;
+    inverseLayout(em.leaves);
+    inverseNodes(em.nodes);
+    if (em.nodes.empty() && root->type == element_t::hierarchical)
+      throw std::invalid_argument("generateOps_InvertH: the tree is not HODLR-shaped");
+    for (const inv_node& n : em.nodes) {
+      const h_block *a = n.node->child(0, 0), *b = n.node->child(1, 1);
+      const h_block *lo = n.node->child(1, 0), *up = n.node->child(0, 1);
+      const dview ul{em.ms, lo->off_u, n.nb, n.rl, n.rl}, vl{em.ms, lo->off_v, n.na, n.rl, n.rl};
+      const dview uc{em.ms, up->off_u, n.na, n.ru, n.ru}, vc{em.ms, up->off_v, n.nb, n.ru, n.ru};
+      em.apply(b, false, false, ul, dview{em.ms, inv_base + n.l_u, n.nb, n.rl, n.rl});
+      em.apply(a, false, true, vl, dview{em.ms, inv_base + n.l_v, n.na, n.rl, n.rl});
+      em.apply(a, true, false, uc, dview{em.ms, inv_base + n.u_u, n.na, n.ru, n.ru});
+      em.apply(b, true, true, vc, dview{em.ms, inv_base + n.u_v, n.nb, n.ru, n.ru});
+    }
+  } catch (...) {
+    delete tree;
+    throw;
+  }
+  return tree;
+}

@@ -12,7 +12,7 @@ namespace pp {
 struct MatvecBlock {
   int32_t type;  // 1 dense, 2 low rank
   int32_t row0, col0, ny, nx, rank;
-  int64_t off, off_u, off_v;  // element offsets in the FP64 matrix arena
+  int64_t off, off_u, off_v;  // element offsets in the arena the plan is applied to
   int64_t t_off;              // low rank: offset of its rank x nrhs V^T x block
 };
 
@@ -42,6 +42,11 @@ hipError_t launch_add_correction(double* x, int ldx, const void* d, int d_bytes,
                                  hipStream_t s);
 hipError_t launch_hmatvec(const MatvecPlanDev& p, const double* A, const double* x, int ldx, const double* y, int ldy,
                           double* out, int ldo, int nrhs, double* t, hipStream_t s);
+// out = sum(D x) - sum(U (V^T x)) over the plan (a hierarchical inverse whose
+// off-diagonal blocks are stored as -U V^T), in the arena's element type
+// (8: FP64, 4: FP32).
+hipError_t launch_happly(const MatvecPlanDev& p, int elem_bytes, const void* A, const void* x, int ldx, void* out,
+                         int ldo, int nrhs, void* t, hipStream_t s);
 hipError_t launch_segnorm2(const double* r, int ldr, const int* seg, int nseg, int nrhs, double* out, hipStream_t s);
 hipError_t launch_colnorm2(const double* r, int ldr, int n, int nrhs, double* out, hipStream_t s);
 hipError_t launch_mfma_probe(int mode, const double* a, const double* b, double* out, hipStream_t s);

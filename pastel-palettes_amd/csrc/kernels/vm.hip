@@ -188,40 +188,43 @@ __global__ void copy2d_kernel(D* dst, int ldd, const S* src, int lds_, int n, in
   }
 }
 
-// H-matrix-vector product out = y - A_H x (FP64, nrhs columns), two launches
-// of the MFMA workgroup GEMM:
-// phase 1: one workgroup per low-rank leaf: t_b = V_b^T x_b (rank x nrhs).
-__global__ __launch_bounds__(256) void hmv_lowrank_kernel(const MatvecBlock* blocks, const int* lr_list,
-                                                           const double* A, const double* x, int ldx, int nrhs,
-                                                           double* t) {
+// H-matrix-vector product over a block list (nrhs columns), two launches of
+// the MFMA workgroup GEMM:
+//   out = y + sd * sum(D x) - sum(U (V^T x))
+// with sd = -1 and y = b for the FP64 residual r = b - A_H x, and sd = +1,
+// y = 0 for applying a hierarchical inverse (whose off-diagonal blocks are
+// stored as -U V^T).
+// phase 1: one workgroup per low-rank block: t_b = V_b^T x_b (rank x nrhs).
+template <typename T, int MODE>
+__global__ __launch_bounds__(256) void hmv_lowrank_kernel(const MatvecBlock* blocks, const int* lr_list, const T* A,
+                                                           const T* x, int ldx, int nrhs, T* t) {
   const MatvecBlock b = blocks[lr_list[blockIdx.x]];
-  gemm_block<double, MODE_FP64>(b.rank, nrhs, b.nx, 1.0, 0.0, t + b.t_off, nrhs, A + b.off_v, b.rank, true,
-                                x + (int64_t)b.col0 * ldx, ldx, false);
+  gemm_block<T, MODE>(b.rank, nrhs, b.nx, T(1), T(0), t + b.t_off, nrhs, A + b.off_v, b.rank, true,
+                      x + (int64_t)b.col0 * ldx, ldx, false);
 }
 
-// phase 2: one workgroup per row tile: out = y - sum of the contributions
-// (dense D x and low-rank U t) of the leaves covering the tile, in a fixed
-// order.
+// phase 2: one workgroup per row tile: out = y + contributions of the blocks
+// covering the tile, in a fixed order.
+template <typename T, int MODE>
 __global__ __launch_bounds__(256) void hmv_rows_kernel(const MatvecBlock* blocks, const MatvecTile* tiles,
-                                                        const int* contrib, const double* A, const double* x, int ldx,
-                                                        const double* t, const double* y, int ldy, double* out,
-                                                        int ldo, int nrhs) {
+                                                        const int* contrib, const T* A, const T* x, int ldx, const T* t,
+                                                        const T* y, int ldy, T* out, int ldo, int nrhs, T sd) {
   const MatvecTile tl = tiles[blockIdx.x];
-  double* o = out + (int64_t)tl.row0 * ldo;
+  T* o = out + (int64_t)tl.row0 * ldo;
   for (int e = threadIdx.x; e < tl.rows * nrhs; e += blockDim.x) {
     const int i = e / nrhs, c = e % nrhs;
-    o[(int64_t)i * ldo + c] = y != nullptr ? y[(int64_t)(tl.row0 + i) * ldy + c] : 0.;
+    o[(int64_t)i * ldo + c] = y != nullptr ? y[(int64_t)(tl.row0 + i) * ldy + c] : T(0);
   }
   __syncthreads();
   for (int q = tl.first; q < tl.first + tl.count; ++q) {
     const MatvecBlock b = blocks[contrib[2 * q]];
     const int li = tl.row0 - b.row0;
     if (b.type == 1)
-      gemm_block<double, MODE_FP64>(tl.rows, nrhs, b.nx, -1.0, 1.0, o, ldo, A + b.off + (int64_t)li * b.nx, b.nx,
-                                    false, x + (int64_t)b.col0 * ldx, ldx, false);
+      gemm_block<T, MODE>(tl.rows, nrhs, b.nx, sd, T(1), o, ldo, A + b.off + (int64_t)li * b.nx, b.nx, false,
+                          x + (int64_t)b.col0 * ldx, ldx, false);
     else
-      gemm_block<double, MODE_FP64>(tl.rows, nrhs, b.rank, -1.0, 1.0, o, ldo, A + b.off_u + (int64_t)li * b.rank,
-                                    b.rank, false, t + b.t_off, nrhs, false);
+      gemm_block<T, MODE>(tl.rows, nrhs, b.rank, T(-1), T(1), o, ldo, A + b.off_u + (int64_t)li * b.rank, b.rank,
+                          false, t + b.t_off, nrhs, false);
   }
 }
 
@@ -323,14 +326,31 @@ hipError_t launch_add_correction(double* x, int ldx, const void* d, int d_bytes,
   return hipGetLastError();
 }
 
+template <typename T, int MODE>
+static hipError_t launch_hmv_t(const MatvecPlanDev& p, const T* A, const T* x, int ldx, const T* y, int ldy, T* out,
+                               int ldo, int nrhs, T* t, T sd, hipStream_t s) {
+  if (p.n_lowrank > 0)
+    hipLaunchKernelGGL((hmv_lowrank_kernel<T, MODE>), dim3(p.n_lowrank), dim3(256), 0, s, p.blocks, p.lr_list, A, x,
+                       ldx, nrhs, t);
+  hipLaunchKernelGGL((hmv_rows_kernel<T, MODE>), dim3(p.n_tiles), dim3(256), 0, s, p.blocks, p.tiles, p.contrib, A, x,
+                     ldx, (const T*)t, y, ldy, out, ldo, nrhs, sd);
+  return hipGetLastError();
+}
+
 hipError_t launch_hmatvec(const MatvecPlanDev& p, const double* A, const double* x, int ldx, const double* y, int ldy,
                           double* out, int ldo, int nrhs, double* t, hipStream_t s) {
-  if (p.n_lowrank > 0)
-    hipLaunchKernelGGL(hmv_lowrank_kernel, dim3(p.n_lowrank), dim3(256), 0, s, p.blocks, p.lr_list, A, x, ldx, nrhs,
-                       t);
-  hipLaunchKernelGGL(hmv_rows_kernel, dim3(p.n_tiles), dim3(256), 0, s, p.blocks, p.tiles, p.contrib, A, x, ldx, t, y,
-                     ldy, out, ldo, nrhs);
-  return hipGetLastError();
+  return launch_hmv_t<double, MODE_FP64>(p, A, x, ldx, y, ldy, out, ldo, nrhs, t, -1.0, s);
+}
+
+hipError_t launch_happly(const MatvecPlanDev& p, int elem_bytes, const void* A, const void* x, int ldx, void* out,
+                         int ldo, int nrhs, void* t, hipStream_t s) {
+  if (elem_bytes == 8)
+    return launch_hmv_t<double, MODE_FP64>(p, (const double*)A, (const double*)x, ldx, nullptr, 0, (double*)out, ldo,
+                                           nrhs, (double*)t, 1.0, s);
+  if (elem_bytes == 4)
+    return launch_hmv_t<float, MODE_FP32>(p, (const float*)A, (const float*)x, ldx, nullptr, 0, (float*)out, ldo, nrhs,
+                                          (float*)t, 1.0f, s);
+  return hipErrorInvalidValue;
 }
 
 hipError_t launch_segnorm2(const double* r, int ldr, const int* seg, int nseg, int nrhs, double* out, hipStream_t s) {

@@ -150,9 +150,56 @@ void Hierarchical::inverseLayout(std::vector<inv_block>& out) const {
   out.clear();
   int64_t cursor = 0;
   for (const h_block* b : diag) {
-    out.push_back(inv_block{b->off, b->ny, cursor});
+    out.push_back(inv_block{b->off, b->ny, cursor, b->row0});
     cursor += 2 * round32((int64_t)b->ny * b->ny);
   }
+}
+
+namespace {
+
+bool hodlr_shaped(const h_block* b) {
+  if (b->type == element_t::dense) return b->nx == b->ny;
+  if (b->type != element_t::hierarchical || b->py != 2 || b->px != 2) return false;
+  const h_block *l = b->child(1, 0), *u = b->child(0, 1);
+  if (l->type != element_t::low_rank || u->type != element_t::low_rank) return false;
+  return hodlr_shaped(b->child(0, 0)) && hodlr_shaped(b->child(1, 1));
+}
+
+void collect_nodes(const h_block* b, int64_t row0, std::vector<Hierarchical::inv_node>& out, int64_t& cursor) {
+  if (b->type != element_t::hierarchical) return;
+  const h_block *a = b->child(0, 0), *c = b->child(1, 1);
+  collect_nodes(a, row0, out, cursor);
+  collect_nodes(c, row0 + a->ny, out, cursor);
+  Hierarchical::inv_node n{};
+  n.node = b;
+  n.row0 = row0;
+  n.na = a->ny;
+  n.nb = c->ny;
+  n.rl = b->child(1, 0)->rank;
+  n.ru = b->child(0, 1)->rank;
+  n.l_u = cursor;
+  cursor += round32((int64_t)n.nb * n.rl);
+  n.l_v = cursor;
+  cursor += round32((int64_t)n.na * n.rl);
+  n.u_u = cursor;
+  cursor += round32((int64_t)n.na * n.ru);
+  n.u_v = cursor;
+  cursor += round32((int64_t)n.nb * n.ru);
+  out.push_back(n);
+}
+
+}  // namespace
+
+bool Hierarchical::hodlrShaped() const { return hodlr_shaped(root); }
+
+void Hierarchical::inverseNodes(std::vector<inv_node>& out) const {
+  out.clear();
+  if (!hodlrShaped()) return;
+  std::vector<inv_block> lay;
+  inverseLayout(lay);
+  int64_t cursor = 0;
+  for (const inv_block& e : lay) cursor += 2 * round32((int64_t)e.nb * e.nb);
+  collect_nodes(root, 0, out, cursor);
 }
 
 int64_t Hierarchical::inverseStorage() const {
@@ -160,5 +207,10 @@ int64_t Hierarchical::inverseStorage() const {
   inverseLayout(lay);
   int64_t s = 0;
   for (const inv_block& e : lay) s += 2 * round32((int64_t)e.nb * e.nb);
+  std::vector<inv_node> nodes;
+  inverseNodes(nodes);
+  for (const inv_node& n : nodes)
+    s += round32((int64_t)n.nb * n.rl) + round32((int64_t)n.na * n.rl) + round32((int64_t)n.na * n.ru) +
+         round32((int64_t)n.nb * n.ru);
   return s;
 }

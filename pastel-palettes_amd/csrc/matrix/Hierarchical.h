@@ -98,14 +98,42 @@ public:
     int64_t leaf_off;  // arena offset of the leaf
     int nb;            // leaf order
     int64_t inv_off;   // offset of L^{-1} within the inverse area
+    int64_t row0;      // first row of the leaf
   };
   void inverseLayout(std::vector<inv_block>& out) const;
-  // Elements of the inverse area.
+  // Elements of the inverse area (leaf inverses, then, for HODLR-shaped
+  // trees, the low-rank factors of inverseNodes()).
   int64_t inverseStorage() const;
+
+  // Hierarchical inverse of the factors of a HODLR-shaped tree (every
+  // hierarchical diagonal node is 2 x 2 with low-rank off-diagonal children).
+  // For a node with diagonal children A, B, the unit-lower factor is
+  // [A 0; U_l V_l^T B] and the upper factor [A U_c V_c^T; 0 B], so
+  //   L^{-1} has the off-diagonal block -l_u l_v^T, rows of B x columns of A,
+  //          l_u = L_B^{-1} U_l (nb x rl), l_v = L_A^{-T} V_l (na x rl);
+  //   U^{-1} has the off-diagonal block -u_u u_v^T, rows of A x columns of B,
+  //          u_u = U_A^{-1} U_c (na x ru), u_v = U_B^{-T} V_c (nb x ru);
+  // both exact (no truncation).  Offsets are within the inverse area, ld =
+  // rank; nodes in post-order (children before parents).
+  struct inv_node {
+    const h_block* node;
+    int64_t row0;  // first row of A (B follows)
+    int na, nb, rl, ru;
+    int64_t l_u, l_v, u_u, u_v;
+  };
+  bool hodlrShaped() const;
+  // Empty when the tree is not HODLR-shaped.
+  void inverseNodes(std::vector<inv_node>& out) const;
   // Ops forming the inverses from the factored leaves (trsml / trsmu on
   // identity blocks).  The inverse area must hold identity blocks before the
   // ops run.
   h_ops_tree* generateOps_Invert(const h_index* self, int64_t inv_base, int slot = -1) const;
+  // generateOps_Invert plus the low-rank factors of inverseNodes(): products
+  // of the children's inverses (already formed, post-order) with the
+  // off-diagonal factors, as gemm / gemm_plus ops.  The node factor areas must
+  // be zero and the leaf areas identity before the ops run; `tmp` hands out
+  // the r x r intermediates (zero-filled temp slot).
+  h_ops_tree* generateOps_InvertH(const h_index* self, int64_t inv_base, dev_temp* tmp, int slot = -1) const;
 
   // Leaves in depth-first order.
   void leaves(std::vector<const h_block*>& out) const;

@@ -53,11 +53,13 @@ struct pp_solver {
   double* d_t = nullptr;    // matvec low-rank products
   double* d_nrm = nullptr;  // [members * nrhs_max] r norms, then the same for b
   int* d_seg = nullptr;     // member row ranges (row0, rows)
-  pp::MatvecBlock* d_blocks = nullptr;
-  pp::MatvecTile* d_tiles = nullptr;
-  int* d_contrib = nullptr;
-  int* d_lr = nullptr;
-  pp::MatvecPlanDev plan{};
+  pp::MatvecPlanDev plan{};  // A_H (FP64 arena): residual
+  // Hierarchical-inverse solve (every member HODLR-shaped): x = U^{-1} L^{-1} b
+  // as two H-matrix products over the inverse areas of the factor arena.
+  bool hinv = false;
+  pp::MatvecPlanDev lplan{}, uplan{};
+  void* d_rhs2 = nullptr;  // n_total x nrhs_max, storage precision
+  void* d_tinv = nullptr;  // V^T x products of lplan / uplan, storage precision
   double factor_flops = 0;
   double last_factor_ms = 0;
   std::vector<double> member_res;
@@ -71,38 +73,18 @@ namespace {
 
 const Hierarchical* hier(const pp_hmatrix* h) { return h->h.get(); }
 
-int build_plan(pp_solver* s) {
-  std::vector<pp::MatvecBlock> blocks;
+// Uploads a block list as a matvec plan: one row tile per dense diagonal
+// block (they must tile the stacked rows), each with the blocks covering it.
+int upload_plan(pp_solver* s, const std::vector<pp::MatvecBlock>& blocks, pp::MatvecPlanDev* plan) {
   std::vector<int> lr;
   std::vector<std::pair<int64_t, int>> ranges;  // global row0, rows
-  int64_t t_off = 0;
-  for (size_t m = 0; m < s->hms.size(); ++m) {
-    std::vector<const h_block*> lv;
-    hier(s->hms[m])->leaves(lv);
-    const int64_t ro = s->row_off[m], ao = s->arena_off[m];
-    for (const h_block* b : lv) {
-      pp::MatvecBlock mb{};
-      mb.type = (int)b->type;
-      mb.row0 = (int32_t)(b->row0 + ro);
-      mb.col0 = (int32_t)(b->col0 + ro);
-      mb.ny = b->ny;
-      mb.nx = b->nx;
-      mb.rank = b->rank;
-      mb.off = b->off + ao;
-      mb.off_u = b->off_u + ao;
-      mb.off_v = b->off_v + ao;
-      if (b->type == element_t::low_rank) {
-        mb.t_off = t_off;
-        t_off += (int64_t)b->rank * s->nrhs_max;
-        lr.push_back((int)blocks.size());
-      }
-      if (b->row0 == b->col0 && b->type == element_t::dense) ranges.emplace_back(b->row0 + ro, b->ny);
-      blocks.push_back(mb);
-    }
+  for (size_t i = 0; i < blocks.size(); ++i) {
+    const auto& b = blocks[i];
+    if (b.type == (int)element_t::low_rank) lr.push_back((int)i);
+    if (b.type == (int)element_t::dense && b.row0 == b.col0) ranges.emplace_back(b.row0, b.ny);
   }
   if (s->n_total > 0x7fffffffLL) return set_error(PP_ERR_INVALID, "stacked rows exceed int32");
   std::sort(ranges.begin(), ranges.end());
-  // tiles: each diagonal leaf's rows with the leaves of its member covering them
   std::vector<pp::MatvecTile> tiles;
   std::vector<int> contrib;
   for (const auto& r : ranges) {
@@ -122,37 +104,135 @@ int build_plan(pp_solver* s) {
   }
   int64_t covered = 0;
   for (const auto& t : tiles) covered += t.rows;
-  if (covered != s->n_total) return set_error(PP_ERR_INVALID, "matvec plan: diagonal leaves do not tile the rows");
+  if (covered != s->n_total) return set_error(PP_ERR_INVALID, "matvec plan: diagonal blocks do not tile the rows");
+  pp::MatvecBlock* d_blocks = nullptr;
+  pp::MatvecTile* d_tiles = nullptr;
+  int *d_contrib = nullptr, *d_lr = nullptr;
+  int rc;
+  if ((rc = ctx_alloc_t(s->ctx, blocks.size(), &d_blocks))) return rc;
+  s->owned.push_back(d_blocks);
+  if ((rc = ctx_alloc_t(s->ctx, tiles.size(), &d_tiles))) return rc;
+  s->owned.push_back(d_tiles);
+  if ((rc = ctx_alloc_t(s->ctx, std::max<size_t>(contrib.size(), 2), &d_contrib))) return rc;
+  s->owned.push_back(d_contrib);
+  if ((rc = ctx_alloc_t(s->ctx, std::max<size_t>(lr.size(), 1), &d_lr))) return rc;
+  s->owned.push_back(d_lr);
+  HIP_TRY(hipMemcpy(d_blocks, blocks.data(), sizeof(pp::MatvecBlock) * blocks.size(), hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(d_tiles, tiles.data(), sizeof(pp::MatvecTile) * tiles.size(), hipMemcpyHostToDevice));
+  if (!contrib.empty())
+    HIP_TRY(hipMemcpy(d_contrib, contrib.data(), sizeof(int) * contrib.size(), hipMemcpyHostToDevice));
+  if (!lr.empty()) HIP_TRY(hipMemcpy(d_lr, lr.data(), sizeof(int) * lr.size(), hipMemcpyHostToDevice));
+  plan->blocks = d_blocks;
+  plan->tiles = d_tiles;
+  plan->contrib = d_contrib;
+  plan->lr_list = d_lr;
+  plan->n_tiles = (int)tiles.size();
+  plan->n_lowrank = (int)lr.size();
+  return PP_OK;
+}
+
+// Residual plan over the FP64 arenas: every leaf of every member.
+int build_plan(pp_solver* s) {
+  std::vector<pp::MatvecBlock> blocks;
+  int64_t t_off = 0;
+  for (size_t m = 0; m < s->hms.size(); ++m) {
+    std::vector<const h_block*> lv;
+    hier(s->hms[m])->leaves(lv);
+    const int64_t ro = s->row_off[m], ao = s->arena_off[m];
+    for (const h_block* b : lv) {
+      pp::MatvecBlock mb{};
+      mb.type = (int)b->type;
+      mb.row0 = (int32_t)(b->row0 + ro);
+      mb.col0 = (int32_t)(b->col0 + ro);
+      mb.ny = b->ny;
+      mb.nx = b->nx;
+      mb.rank = b->rank;
+      mb.off = b->off + ao;
+      mb.off_u = b->off_u + ao;
+      mb.off_v = b->off_v + ao;
+      if (b->type == element_t::low_rank) {
+        mb.t_off = t_off;
+        t_off += (int64_t)b->rank * s->nrhs_max;
+      }
+      blocks.push_back(mb);
+    }
+  }
+  int rc;
+  if ((rc = upload_plan(s, blocks, &s->plan))) return rc;
+  if ((rc = ctx_alloc_t(s->ctx, std::max<int64_t>(t_off, 1), &s->d_t))) return rc;
+  s->owned.push_back(s->d_t);
   std::vector<int> seg;
   for (size_t m = 0; m < s->hms.size(); ++m) {
     seg.push_back((int)s->row_off[m]);
     seg.push_back(hier(s->hms[m])->getNx_abs());
   }
-  int rc;
-  if ((rc = ctx_alloc_t(s->ctx, blocks.size(), &s->d_blocks))) return rc;
-  s->owned.push_back(s->d_blocks);
-  if ((rc = ctx_alloc_t(s->ctx, tiles.size(), &s->d_tiles))) return rc;
-  s->owned.push_back(s->d_tiles);
-  if ((rc = ctx_alloc_t(s->ctx, std::max<size_t>(contrib.size(), 2), &s->d_contrib))) return rc;
-  s->owned.push_back(s->d_contrib);
-  if ((rc = ctx_alloc_t(s->ctx, std::max<size_t>(lr.size(), 1), &s->d_lr))) return rc;
-  s->owned.push_back(s->d_lr);
-  if ((rc = ctx_alloc_t(s->ctx, std::max<int64_t>(t_off, 1), &s->d_t))) return rc;
-  s->owned.push_back(s->d_t);
   if ((rc = ctx_alloc_t(s->ctx, seg.size(), &s->d_seg))) return rc;
   s->owned.push_back(s->d_seg);
-  HIP_TRY(hipMemcpy(s->d_blocks, blocks.data(), sizeof(pp::MatvecBlock) * blocks.size(), hipMemcpyHostToDevice));
-  HIP_TRY(hipMemcpy(s->d_tiles, tiles.data(), sizeof(pp::MatvecTile) * tiles.size(), hipMemcpyHostToDevice));
-  if (!contrib.empty())
-    HIP_TRY(hipMemcpy(s->d_contrib, contrib.data(), sizeof(int) * contrib.size(), hipMemcpyHostToDevice));
-  if (!lr.empty()) HIP_TRY(hipMemcpy(s->d_lr, lr.data(), sizeof(int) * lr.size(), hipMemcpyHostToDevice));
   HIP_TRY(hipMemcpy(s->d_seg, seg.data(), sizeof(int) * seg.size(), hipMemcpyHostToDevice));
-  s->plan.blocks = s->d_blocks;
-  s->plan.tiles = s->d_tiles;
-  s->plan.contrib = s->d_contrib;
-  s->plan.lr_list = s->d_lr;
-  s->plan.n_tiles = (int)tiles.size();
-  s->plan.n_lowrank = (int)lr.size();
+  return PP_OK;
+}
+
+// Plans applying L^{-1} and U^{-1} from the inverse areas of the factor
+// arena, when every member is HODLR-shaped; otherwise the solve runs the VM
+// solve program.
+int build_inverse_plans(pp_solver* s) {
+  for (const pp_hmatrix* h : s->hms)
+    if (!hier(h)->hodlrShaped()) return PP_OK;
+  std::vector<pp::MatvecBlock> lb, ub;
+  int64_t tl = 0, tu = 0;
+  for (size_t m = 0; m < s->hms.size(); ++m) {
+    const Hierarchical* h = hier(s->hms[m]);
+    const int64_t base = s->arena_off[m] + inverse_base(h), ro = s->row_off[m];
+    std::vector<Hierarchical::inv_block> leaves;
+    std::vector<Hierarchical::inv_node> nodes;
+    h->inverseLayout(leaves);
+    h->inverseNodes(nodes);
+    for (const auto& e : leaves) {
+      pp::MatvecBlock d{};
+      d.type = (int)element_t::dense;
+      d.row0 = d.col0 = (int32_t)(ro + e.row0);
+      d.ny = d.nx = e.nb;
+      d.off = base + e.inv_off;
+      lb.push_back(d);
+      d.off = base + e.inv_off + ((int64_t)e.nb * e.nb + 31) / 32 * 32;
+      ub.push_back(d);
+    }
+    for (const auto& n : nodes) {
+      pp::MatvecBlock l{};
+      l.type = (int)element_t::low_rank;
+      l.row0 = (int32_t)(ro + n.row0 + n.na);
+      l.col0 = (int32_t)(ro + n.row0);
+      l.ny = n.nb;
+      l.nx = n.na;
+      l.rank = n.rl;
+      l.off_u = base + n.l_u;
+      l.off_v = base + n.l_v;
+      l.t_off = tl;
+      tl += (int64_t)n.rl * s->nrhs_max;
+      lb.push_back(l);
+      pp::MatvecBlock u{};
+      u.type = (int)element_t::low_rank;
+      u.row0 = (int32_t)(ro + n.row0);
+      u.col0 = (int32_t)(ro + n.row0 + n.na);
+      u.ny = n.na;
+      u.nx = n.nb;
+      u.rank = n.ru;
+      u.off_u = base + n.u_u;
+      u.off_v = base + n.u_v;
+      u.t_off = tu;
+      tu += (int64_t)n.ru * s->nrhs_max;
+      ub.push_back(u);
+    }
+  }
+  int rc;
+  if ((rc = upload_plan(s, lb, &s->lplan))) return rc;
+  if ((rc = upload_plan(s, ub, &s->uplan))) return rc;
+  const int eb = elem_bytes(s->precision);
+  if ((rc = ctx_alloc(s->ctx, (uint64_t)std::max<int64_t>(std::max(tl, tu), 1) * eb, &s->d_tinv))) return rc;
+  s->owned.push_back(s->d_tinv);
+  if ((rc = ctx_alloc(s->ctx, (uint64_t)s->n_total * s->nrhs_max * eb, &s->d_rhs2))) return rc;
+  s->owned.push_back(s->d_rhs2);
+  s->hinv = true;
   return PP_OK;
 }
 
@@ -243,23 +323,14 @@ int solver_create(pp_ctx* ctx, const pp_hmatrix* const* hs, int count, const dou
     if ((rc = ctx_alloc_t(ctx, 2 * (int64_t)count * nrhs_max, &s->d_nrm))) break;
     s->owned.push_back(s->d_nrm);
     if ((rc = build_plan(s))) break;
+    if ((rc = build_inverse_plans(s))) break;
     if ((rc = pp_program_factor_inv_batch(hs, count, workers, &s->fprog))) break;
     std::vector<void*> ptrs = slot_table(s, nrhs_max);
     if ((rc = pp_stream_upload_program(ctx, s->fprog, const_cast<const void* const*>(ptrs.data()), (int)ptrs.size(),
                                        precision, &s->fstream)))
       break;
-    // H-LU FLOPs only: the leaf inversions (trsml + trsmu on identity
-    // blocks) are part of the solve's set-up
-    double inv_flops = 0;
-    for (int i = 0; i < count; ++i) {
-      std::vector<Hierarchical::inv_block> lay;
-      hier(hs[i])->inverseLayout(lay);
-      for (const auto& e : lay) {
-        const double nb = e.nb;
-        inv_flops += nb * nb * (nb - 1) + nb * nb * nb;
-      }
-    }
-    s->factor_flops = (double)s->fprog->dag->getFlops() - inv_flops;
+    // H-LU FLOPs only: the inversions are part of the solve's set-up
+    s->factor_flops = (double)s->fprog->primary_flops;
     s->fstream->role = 1;
     s->member_res.assign(count, 0.);
   } while (false);
@@ -436,9 +507,19 @@ int pp_solver_solve(pp_solver* s, const double* b, double* x, int nrhs, int ld, 
     const int eb = elem_bytes(s->precision);
     const int M = (int)s->hms.size();
     pp_solver::SolveProg* sp = nullptr;
-    const int w = program_width(s, nrhs);
-    int rc = solve_prog(s, w, &sp);
-    if (rc) return rc;
+    // hierarchical-inverse solves work at the request's width; VM solve
+    // programs at a power-of-two width over zero-padded columns
+    const int w = s->hinv ? nrhs : program_width(s, nrhs);
+    int rc = PP_OK;
+    if (!s->hinv && (rc = solve_prog(s, w, &sp))) return rc;
+    // d_rhs <- F^{-1} d_rhs (pitch w)
+    auto apply_inverse = [&]() -> int {
+      if (!s->hinv) return enqueue_launch(s->ctx, sp->stream, s->wpi, st, true);
+      hipError_t e2 = pp::launch_happly(s->lplan, eb, s->d_F, s->d_rhs, w, s->d_rhs2, w, nrhs, s->d_tinv, st);
+      if (e2 == hipSuccess)
+        e2 = pp::launch_happly(s->uplan, eb, s->d_F, s->d_rhs2, w, s->d_rhs, w, nrhs, s->d_tinv, st);
+      return e2 == hipSuccess ? PP_OK : hip_fail(e2, "apply hierarchical inverse");
+    };
     std::vector<double> bn(M * nrhs), rn(M * nrhs);
     double* d_bn = s->d_nrm + (int64_t)M * s->nrhs_max;
     hipError_t e = pp::launch_segnorm2(b, ld, s->d_seg, M, nrhs, d_bn, st);
@@ -450,12 +531,12 @@ int pp_solver_solve(pp_solver* s, const double* b, double* x, int nrhs, int ld, 
     }
     e = pp::launch_copy2d(s->d_rhs, eb, w, b, 8, ld, (int)s->n_total, nrhs, st);
     if (e != hipSuccess) return hip_fail(e, "copy rhs");
-    if ((rc = enqueue_launch(s->ctx, sp->stream, s->wpi, st, true))) return rc;
+    if ((rc = apply_inverse())) return rc;
     e = pp::launch_copy2d(x, 8, ld, s->d_rhs, eb, w, (int)s->n_total, nrhs, st);
     if (e != hipSuccess) return hip_fail(e, "copy x");
     HIP_TRY(hipMemcpyAsync(bn.data(), d_bn, 8 * bn.size(), hipMemcpyDeviceToHost, st));
     HIP_TRY(hipStreamSynchronize(st));
-    if ((rc = check_status(sp->stream))) return rc;
+    if (sp != nullptr && (rc = check_status(sp->stream))) return rc;
     double res = 0.;
     int it = 0;
     bool conv = false;
@@ -485,11 +566,11 @@ int pp_solver_solve(pp_solver* s, const double* b, double* x, int nrhs, int ld, 
       ++it;
       e = pp::launch_copy2d(s->d_rhs, eb, w, s->d_r, 8, nrhs, (int)s->n_total, nrhs, st);
       if (e != hipSuccess) return hip_fail(e, "copy residual");
-      if ((rc = enqueue_launch(s->ctx, sp->stream, s->wpi, st, true))) return rc;
+      if ((rc = apply_inverse())) return rc;
       e = pp::launch_add_correction(x, ld, s->d_rhs, eb, w, (int)s->n_total, nrhs, st);
       if (e != hipSuccess) return hip_fail(e, "update x");
     }
-    if ((rc = check_status(sp->stream))) return rc;
+    if (sp != nullptr && (rc = check_status(sp->stream))) return rc;
     if (rep) {
       rep->iterations = it;
       rep->converged = conv ? 1 : 0;

@@ -226,7 +226,8 @@ def test_oracle_inverse_factor_and_solve(pp, orc):
         assert np.max(np.abs(linv @ l - np.eye(nb))) < 1e-12
         assert np.max(np.abs(uinv @ u - np.eye(nb))) < 1e-10
         cursor += 2 * second
-    assert cursor == ifac.size - base
+    leaves, nodes = hm.inverse_layout()
+    assert cursor == (nodes[0]["l_u"] if nodes else ifac.size - base)
 
     nrhs = 4
     b = pp.rhs(1, n, nrhs)
@@ -259,3 +260,50 @@ def test_oracle_hmatvec_matches_dense(pp, orc):
     want = pp.densify(hm, arena) @ x
     got = orc.hmatvec(hm.leaves(), arena, x)
     assert np.max(np.abs(got - want)) <= 1e-12 * np.max(np.abs(want))
+
+
+def dense_inverses(hm, area):
+    """Dense L^{-1}, U^{-1} assembled from a filled inverse area."""
+    leaves, nodes = hm.inverse_layout()
+    n = hm.n
+    li, ui = np.zeros((n, n)), np.zeros((n, n))
+    for e in leaves:
+        r, nb, o = e["row0"], e["nb"], e["inv_off"]
+        second = (nb * nb + 31) // 32 * 32
+        li[r:r + nb, r:r + nb] = area[o:o + nb * nb].reshape(nb, nb)
+        ui[r:r + nb, r:r + nb] = area[o + second:o + second + nb * nb].reshape(nb, nb)
+    for d in nodes:
+        a0, na, nb, rl, ru = d["row0"], d["na"], d["nb"], d["rl"], d["ru"]
+        b0 = a0 + na
+        lu = area[d["l_u"]:d["l_u"] + nb * rl].reshape(nb, rl)
+        lv = area[d["l_v"]:d["l_v"] + na * rl].reshape(na, rl)
+        uu = area[d["u_u"]:d["u_u"] + na * ru].reshape(na, ru)
+        uv = area[d["u_v"]:d["u_v"] + nb * ru].reshape(nb, ru)
+        li[b0:b0 + nb, a0:a0 + na] = -lu @ lv.T
+        ui[a0:a0 + na, b0:b0 + nb] = -uu @ uv.T
+    return li, ui
+
+
+@pytest.mark.parametrize("n,levels,rank", [(1024, 2, 16), (2048, 3, 32)])
+def test_oracle_hierarchical_inverse(pp, orc, n, levels, rank):
+    """The factor program with the hierarchical inverse leaves factors equal
+    (bit for bit) to the plain program's, and the inverse area assembles to
+    numpy's inverses of the dense unit-lower and upper factors."""
+    hm, kern, arena = pp.build_config(1, n, levels=levels, rank=rank)
+    rnd = pp.sketch(pp.SKETCH_SEED, 1 << 16)
+    prog = pp.Program.factor(hm, 4)
+    info = prog.info()
+    fac = arena.copy()
+    orc.run_streams(prog.streams(), [fac, np.zeros(max(info["temp_elems"], 1)), None], info["n_insts"],
+                    info["scratch_elems"], rnd=rnd)
+    iprog = pp.Program.factor_inv_batch([hm], 4)
+    iinfo = iprog.info()
+    assert iinfo["n_insts"] > info["n_insts"] + 2 * len(hm.inverse_layout()[0])
+    ifac = hm.with_inverse_area(arena)
+    orc.run_streams(iprog.streams(), [ifac, np.zeros(max(iinfo["temp_elems"], 1)), None], iinfo["n_insts"],
+                    iinfo["scratch_elems"], rnd=rnd)
+    assert np.array_equal(ifac[:hm.storage], fac[:hm.storage])
+    l, u = pp.lu_factors(pp.densify(hm, fac))
+    li, ui = dense_inverses(hm, ifac[hm.inverse_base:])
+    assert np.max(np.abs(li - np.linalg.inv(l))) < 1e-11 * np.max(np.abs(np.linalg.inv(l)))
+    assert np.max(np.abs(ui - np.linalg.inv(u))) < 1e-11 * np.max(np.abs(np.linalg.inv(u)))

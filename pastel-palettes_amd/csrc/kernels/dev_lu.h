@@ -53,43 +53,48 @@ __device__ __forceinline__ void tri_inverse_column(const PP_LDS T* s, int nb, in
       y[i] = (i < nb) ? acc : T(0);
     }
   } else {
+    // column NB of each row holds 1 / u_ii
 #pragma unroll
     for (int i = NB - 1; i >= 0; --i) {
       T acc = (i == c) ? T(1) : T(0);
 #pragma unroll
       for (int j = i + 1; j < NB; ++j) acc -= s[i * (NB + 1) + j] * y[j];
       asm volatile("" : "+v"(acc)::"memory");
-      y[i] = (i < nb) ? acc / s[i * (NB + 1) + i] : T(0);
+      y[i] = (i < nb) ? acc * s[i * (NB + 1) + NB] : T(0);
     }
   }
 }
 
 // Inverses of the NB x NB diagonal blocks of the n x n triangular matrix F
 // (pitch ldf); block b is written row-major (pitch NB) at out + b NB NB.
-// One wave per block; uses NB (NB + 1) elements of LDS per wave.
+// Two blocks per wave (lanes 0-31 and 32-63, one inverse column per lane);
+// uses NB (NB + 1) elements of LDS per block, the last column holding the
+// reciprocals of the diagonal (upper case).
 template <typename T, bool UPPER>
 __device__ __forceinline__ void diag_inverses(const T* F, int ldf, int n, T* out) {
   const int nw = blockDim.x >> 6, w = wave_id(), l = lane_id();
+  const int half = l >> 5, lc = l & 31;
   const int nblk = (n + NB - 1) / NB;
-  PP_LDS T* s = (PP_LDS T*)lds_base() + w * NB * (NB + 1);
-  for (int b0 = 0; b0 < nblk; b0 += nw) {
-    const int b = b0 + w;
+  PP_LDS T* s = (PP_LDS T*)lds_base() + (2 * w + half) * NB * (NB + 1);
+  for (int b0 = 0; b0 < nblk; b0 += 2 * nw) {
+    const int b = b0 + 2 * w + half;
     const int r0 = b * NB;
     const int nb = b < nblk ? ((n - r0) < NB ? (n - r0) : NB) : 0;
     if (b < nblk) {
-      for (int e = l; e < NB * NB; e += 64) {
+      for (int e = lc; e < NB * NB; e += 32) {
         const int i = e / NB, j = e - i * NB;
         s[i * (NB + 1) + j] = (i < nb && j < nb) ? F[(int64_t)(r0 + i) * ldf + r0 + j] : T(0);
       }
+      if (UPPER && lc < nb) s[lc * (NB + 1) + NB] = T(1) / F[(int64_t)(r0 + lc) * ldf + r0 + lc];
     }
     __syncthreads();
-    if (b < nblk && l < nb) {
+    if (b < nblk && lc < nb) {
       T y[NB];
-      tri_inverse_column<T, UPPER>(s, nb, l, y);
+      tri_inverse_column<T, UPPER>(s, nb, lc, y);
       T* o = out + (int64_t)b * NB * NB;
 #pragma unroll
       for (int i = 0; i < NB; ++i)
-        if (i < nb) o[i * NB + l] = y[i];
+        if (i < nb) o[i * NB + lc] = y[i];
     }
     __syncthreads();
   }

@@ -100,6 +100,141 @@ __device__ __forceinline__ void diag_inverses(const T* F, int ldf, int n, T* out
   }
 }
 
+// ---------------------------------------------------------------- narrow solve
+// In-place triangular solve with few right-hand-side columns, the right-hand
+// side resident in LDS for the whole solve:
+//   forward:  for i = 0, 1, ...:  X_i = D_i W_i;  W_{>i} -= T_{>i,i} X_i
+//   backward: for i = last .. 0:  X_i = D_i W_i;  W_{<i} -= T_{<i,i} X_i
+// W is m x w (w <= 64), W(r, c) = Wg[r ldw + c] or, when w_trans,
+// Wg[c ldw + r].  T(r, q) = F[r ldf + q] or, when f_trans, F[q ldf + r];
+// D_i (inverse of T's i-th 32 x 32 diagonal block) is dinv[i NB NB + r NB + q]
+// or, when f_trans, its transpose.  One step's tiles are 32 x 32 MFMA tiles,
+// T and D fragments loaded from global memory, W fragments from LDS.
+
+// acc += A (32 rows by la, k < kmax) x W[kbase + k][c0 .. c0 + 31] from LDS
+template <typename T, int MODE>
+__device__ __forceinline__ void tile_from_lds(const OperandLane<T, Mma<T, MODE>::EPL> (&la)[2], int kmax,
+                                              const PP_LDS T* Wk, int wp, int c0,
+                                              typename Mma<T, MODE>::acc_t (&acc)[2][2]) {
+  using P = Mma<T, MODE>;
+  constexpr int KW = P::KW, EPL = P::EPL, CH = NB / KW;
+  const int l = lane_id(), g = l >> 4, li = l & 15;
+  T fa[CH][2][EPL], fb[CH][2][EPL];
+#pragma unroll
+  for (int c = 0; c < CH; ++c) {
+    const int kg = c * KW + EPL * g;
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      la[h].template load<false>(fa[c][h], kg, kmax);
+#pragma unroll
+      for (int e = 0; e < EPL; ++e) fb[c][h][e] = Wk[(kg + e) * wp + c0 + 16 * h + li];
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < CH; ++c)
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+      for (int j = 0; j < 2; ++j) acc[i][j] = P::mma_chunk(fa[c][i], fb[c][j], acc[i][j]);
+}
+
+// LDS bytes of narrow_solve for m rows and w columns.
+template <typename T>
+__device__ __forceinline__ int narrow_solve_bytes(int m, int w) {
+  const int ntc = (w + 31) >> 5, wp = ntc * 32 + 16 / (int)sizeof(T);
+  return ((m + NB - 1) / NB) * NB * wp * (int)sizeof(T);
+}
+
+template <typename T, int MODE>
+__device__ __forceinline__ void narrow_solve(T* Wg, int ldw, bool w_trans, int m, int w, const T* F, int ldf,
+                                             bool f_trans, const T* dinv, bool forward) {
+  using P = Mma<T, MODE>;
+  using acc_t = typename P::acc_t;
+  const int ntc = (w + 31) >> 5, wc = ntc * 32, wp = wc + 16 / (int)sizeof(T);
+  const int nblk = (m + NB - 1) / NB, mr = nblk * NB;
+  PP_LDS T* Ws = (PP_LDS T*)lds_base();
+  const int tid = threadIdx.x, nt = blockDim.x, wv = wave_id(), nw = nt >> 6, l = lane_id(), li = l & 15;
+  // stage W (in the order that keeps global accesses coalesced); rows >= m
+  // and columns >= w are zero and stay zero
+  for (int e = tid; e < mr * wc; e += nt) {
+    int r, c;
+    if (w_trans) {
+      c = e / mr;
+      r = e - c * mr;
+    } else {
+      r = e / wc;
+      c = e - r * wc;
+    }
+    T v = T(0);
+    if (r < m && c < w) v = w_trans ? Wg[(int64_t)c * ldw + r] : Wg[(int64_t)r * ldw + c];
+    Ws[r * wp + c] = v;
+  }
+  __syncthreads();
+  for (int s = 0; s < nblk; ++s) {
+    const int i = forward ? s : nblk - 1 - s;
+    const int r0 = i * NB, bs = (m - r0) < NB ? (m - r0) : NB;
+    // X_i = D_i W_i, one column tile per wave, in place
+    if (wv < ntc) {
+      const int c0 = wv * 32;
+      OperandLane<T, P::EPL> la[2];
+#pragma unroll
+      for (int h = 0; h < 2; ++h) la[h].init(dinv + (int64_t)i * NB * NB, NB, !f_trans, 16 * h + li, bs);
+      acc_t acc[2][2];
+#pragma unroll
+      for (int a = 0; a < 2; ++a) acc[a][0] = acc[a][1] = P::zero();
+      tile_from_lds<T, MODE>(la, bs, Ws + r0 * wp, wp, c0, acc);
+#pragma unroll
+      for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            const int row = a * 16 + P::orow(l, r);
+            Ws[(r0 + row) * wp + c0 + j * 16 + li] = row < bs ? acc[a][j][r] : T(0);
+          }
+    }
+    __syncthreads();
+    // W_rest -= T_{rest,i} X_i
+    const int q_lo = forward ? r0 + NB : 0, q_hi = forward ? mr : r0;
+    const int nrt = (q_hi - q_lo) >> 5, ntiles = nrt * ntc;
+    for (int t = wv; t < ntiles; t += nw) {
+      const int q0 = q_lo + (t / ntc) * 32, c0 = (t % ntc) * 32;
+      OperandLane<T, P::EPL> la[2];
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        if (f_trans) la[h].init(F + (int64_t)r0 * ldf, ldf, false, q0 + 16 * h + li, m);
+        else la[h].init(F + r0, ldf, true, q0 + 16 * h + li, m);
+      }
+      acc_t acc[2][2];
+#pragma unroll
+      for (int a = 0; a < 2; ++a) acc[a][0] = acc[a][1] = P::zero();
+      tile_from_lds<T, MODE>(la, bs, Ws + r0 * wp, wp, c0, acc);
+#pragma unroll
+      for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            const int row = q0 + a * 16 + P::orow(l, r);
+            PP_LDS T* wpx = Ws + row * wp + c0 + j * 16 + li;
+            if (row < m) *wpx = *wpx - acc[a][j][r];
+          }
+    }
+    __syncthreads();
+  }
+  // write W back
+  for (int e = tid; e < m * w; e += nt) {
+    if (w_trans) {
+      const int c = e / m, r = e - c * m;
+      Wg[(int64_t)c * ldw + r] = Ws[r * wp + c];
+    } else {
+      const int r = e / w, c = e - r * w;
+      Wg[(int64_t)r * ldw + c] = Ws[r * wp + c];
+    }
+  }
+  __syncthreads();
+}
+
 // ---------------------------------------------------------------- getrf
 // LDS ordering inside the panel.  The LDS unit executes the operations of one
 // wave in issue order, so (a) lanes of a wave see the wave's earlier LDS

@@ -23,7 +23,7 @@ __device__ __forceinline__ T* slot(void* const* ptrs, const int32_t* pc, int i) 
   return reinterpret_cast<T*>(ptrs[word(pc, i)]);
 }
 
-enum : int { ST_NONE = 0, ST_GEMM = 1, ST_PANEL = 2, ST_DIAGINV = 3, ST_QR = 4, ST_COPY = 5, ST_CHOL = 6 };
+enum : int { ST_NONE = 0, ST_GEMM = 1, ST_PANEL = 2, ST_DIAGINV = 3, ST_QR = 4, ST_COPY = 5, ST_CHOL = 6, ST_TSOLVE = 7 };
 
 // One primitive step.  Field use per kind:
 //   GEMM     C = beta C + alpha op(A) op(B), m x n x k; leaf selects the
@@ -34,6 +34,8 @@ enum : int { ST_NONE = 0, ST_GEMM = 1, ST_PANEL = 2, ST_DIAGINV = 3, ST_QR = 4, 
 //            tau = A, w = A + k)
 //   COPY     C[i][j] = A[i][j] for i < m, j < n (pitches ldc, lda)
 //   CHOL     chol_upper(G = C, k = m, rows = k, first = aT)
+//   TSOLVE   narrow_solve(W = C, ldw = ldc, w_trans = bT, m, w = n, F = A,
+//            ldf = lda, f_trans = aT, dinv = B, forward = k != 0)
 template <typename T>
 struct Step {
   int kind;
@@ -63,6 +65,30 @@ __device__ __forceinline__ void set_gemm(Step<T>& st, bool leaf, int m, int n, i
   st.B = B;
   st.ldb = ldb;
   st.bT = bT;
+}
+
+// A trsm with at most 64 right-hand-side columns whose right-hand side fits
+// in LDS runs as DIAGINV + one TSOLVE step.
+template <typename T>
+__device__ __forceinline__ bool narrow_ok(int m, int w) {
+  return w <= 64 && narrow_solve_bytes<T>(m, w) <= kLdsBytes - 1024;
+}
+
+template <typename T>
+__device__ __forceinline__ void set_tsolve(Step<T>& st, T* W, int ldw, bool w_trans, int m, int w, const T* F, int ldf,
+                                           bool f_trans, const T* dinv, bool forward) {
+  st.kind = ST_TSOLVE;
+  st.leaf = true;
+  st.C = W;
+  st.ldc = ldw;
+  st.bT = w_trans;
+  st.m = m;
+  st.n = w;
+  st.A = F;
+  st.lda = ldf;
+  st.aT = f_trans;
+  st.B = dinv;
+  st.k = forward ? 1 : 0;
 }
 
 // Entry length in words, or -1 for an unknown operation.
@@ -133,6 +159,11 @@ __device__ __forceinline__ int make_step(const int32_t* pc, void* const* P, int 
         st.aT = false;
         return 1;
       }
+      if (narrow_ok<T>(n, nx_b)) {
+        if (s > 1) return 0;
+        set_tsolve<T>(st, B, ld_b, b_T, n, nx_b, L, ld_l, false, tmp, true);
+        return 1;
+      }
       if (s > 2 * nblk) return 0;
       const int i = (s - 1) >> 1, r = (s - 1) & 1;
       const int r0 = i * NB, bs = (n - r0) < NB ? (n - r0) : NB, rest = n - r0 - bs;
@@ -166,6 +197,12 @@ __device__ __forceinline__ int make_step(const int32_t* pc, void* const* P, int 
         st.aT = true;
         return 1;
       }
+      if (narrow_ok<T>(n, ny_b)) {
+        // W = B^T: a forward solve with U^T
+        if (s > 1) return 0;
+        set_tsolve<T>(st, B, ld_b, !b_T, n, ny_b, U, ld_u, true, tmp, true);
+        return 1;
+      }
       if (s > 2 * nblk) return 0;
       const int j = (s - 1) >> 1, r = (s - 1) & 1;
       const int c0 = j * NB, bs = (n - c0) < NB ? (n - c0) : NB, rest = n - c0 - bs;
@@ -196,6 +233,11 @@ __device__ __forceinline__ int make_step(const int32_t* pc, void* const* P, int 
         st.m = n;
         st.C = tmp;
         st.aT = true;
+        return 1;
+      }
+      if (narrow_ok<T>(n, nx_b)) {
+        if (s > 1) return 0;
+        set_tsolve<T>(st, B, ld_b, false, n, nx_b, U, ld_u, false, tmp, false);
         return 1;
       }
       if (s > 2 * nblk) return 0;
@@ -454,6 +496,9 @@ __device__ __forceinline__ void exec_step(const Step<T>& st) {
       break;
     case ST_CHOL:
       chol_upper<T>(st.C, st.m, st.k, st.aT);
+      break;
+    case ST_TSOLVE:
+      narrow_solve<T, FMODE>(st.C, st.ldc, st.bT, st.m, st.n, st.A, st.lda, st.aT, st.B, st.k != 0);
       break;
     default:
       break;

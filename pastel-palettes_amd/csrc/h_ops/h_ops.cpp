@@ -661,12 +661,20 @@ h_ops_tree* Hierarchical::generateOps_Invert(const h_index* self, int64_t inv_ba
   inverseLayout(lay);
   h_ops_tree* tree = new h_ops_tree();
   try {
+    // column slices of kInvSlice: L^{-1}'s slice at s0 is zero above row s0
+    // and U^{-1}'s below row s0 + w, so each slice is a narrow solve with the
+    // trailing (leading) part of the factor only, and the slices run in
+    // parallel
+    constexpr int kInvSlice = 64;
     for (const inv_block& e : lay) {
       const dview leaf{ms, e.leaf_off, e.nb, e.nb, e.nb};
       const dview linv{ms, inv_base + e.inv_off, e.nb, e.nb, e.nb};
       const dview uinv{ms, inv_base + e.inv_off + align32((int64_t)e.nb * e.nb), e.nb, e.nb, e.nb};
-      emit_trsml(tree, mview{linv, false}, leaf);
-      emit_trsmu(tree, uinv, leaf);
+      for (int s0 = 0; s0 < e.nb; s0 += kInvSlice) {
+        const int w = std::min(kInvSlice, e.nb - s0), hi = s0 + w;
+        emit_trsml(tree, mview{linv.sub(s0, s0, e.nb - s0, w), false}, leaf.sub(s0, s0, e.nb - s0, e.nb - s0));
+        emit_trsmu(tree, uinv.sub(0, s0, hi, w), leaf.sub(0, 0, hi, hi));
+      }
     }
   } catch (...) {
     delete tree;

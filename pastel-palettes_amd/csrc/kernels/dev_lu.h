@@ -111,24 +111,35 @@ __device__ __forceinline__ void diag_inverses(const T* F, int ldf, int n, T* out
 // or, when f_trans, its transpose.  One step's tiles are 32 x 32 MFMA tiles,
 // T and D fragments loaded from global memory, W fragments from LDS.
 
-// acc += A (32 rows by la, k < kmax) x W[kbase + k][c0 .. c0 + 31] from LDS
+// A fragments of a 32-row tile (rows by la, k < kmax, both 16-wide chunks)
 template <typename T, int MODE>
-__device__ __forceinline__ void tile_from_lds(const OperandLane<T, Mma<T, MODE>::EPL> (&la)[2], int kmax,
+__device__ __forceinline__ void load_a_frags(const OperandLane<T, Mma<T, MODE>::EPL> (&la)[2], int kmax,
+                                             T (&fa)[NB / Mma<T, MODE>::KW][2][Mma<T, MODE>::EPL]) {
+  using P = Mma<T, MODE>;
+  constexpr int KW = P::KW, EPL = P::EPL, CH = NB / KW;
+  const int g = lane_id() >> 4;
+#pragma unroll
+  for (int c = 0; c < CH; ++c)
+#pragma unroll
+    for (int h = 0; h < 2; ++h) la[h].template load<false>(fa[c][h], c * KW + EPL * g, kmax);
+}
+
+// acc += A (fragments fa) x W[k][c0 .. c0 + 31] for k < 32, W rows from LDS
+template <typename T, int MODE>
+__device__ __forceinline__ void tile_from_lds(const T (&fa)[NB / Mma<T, MODE>::KW][2][Mma<T, MODE>::EPL],
                                               const PP_LDS T* Wk, int wp, int c0,
                                               typename Mma<T, MODE>::acc_t (&acc)[2][2]) {
   using P = Mma<T, MODE>;
   constexpr int KW = P::KW, EPL = P::EPL, CH = NB / KW;
   const int l = lane_id(), g = l >> 4, li = l & 15;
-  T fa[CH][2][EPL], fb[CH][2][EPL];
+  T fb[CH][2][EPL];
 #pragma unroll
   for (int c = 0; c < CH; ++c) {
     const int kg = c * KW + EPL * g;
 #pragma unroll
-    for (int h = 0; h < 2; ++h) {
-      la[h].template load<false>(fa[c][h], kg, kmax);
+    for (int h = 0; h < 2; ++h)
 #pragma unroll
       for (int e = 0; e < EPL; ++e) fb[c][h][e] = Wk[(kg + e) * wp + c0 + 16 * h + li];
-    }
   }
 #pragma unroll
   for (int c = 0; c < CH; ++c)
@@ -170,19 +181,44 @@ __device__ __forceinline__ void narrow_solve(T* Wg, int ldw, bool w_trans, int m
     Ws[r * wp + c] = v;
   }
   __syncthreads();
+  constexpr int CH = NB / P::KW, PRE = 2;  // T fragments of up to PRE tiles are requested early
   for (int s = 0; s < nblk; ++s) {
     const int i = forward ? s : nblk - 1 - s;
     const int r0 = i * NB, bs = (m - r0) < NB ? (m - r0) : NB;
+    const int q_lo = forward ? r0 + NB : 0, q_hi = forward ? mr : r0;
+    const int nrt = (q_hi - q_lo) >> 5, ntiles = nrt * ntc;
+    // the rest update's T fragments do not depend on X_i: request them
+    // before X_i is formed
+    auto t_lanes = [&](int t, OperandLane<T, P::EPL> (&la)[2]) {
+      const int q0 = q_lo + (t / ntc) * 32;
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        if (f_trans) la[h].init(F + (int64_t)r0 * ldf, ldf, false, q0 + 16 * h + li, m);
+        else la[h].init(F + r0, ldf, true, q0 + 16 * h + li, m);
+      }
+    };
+    T pre[PRE][CH][2][P::EPL];
+#pragma unroll
+    for (int q = 0; q < PRE; ++q) {
+      const int t = wv + q * nw;
+      if (t < ntiles) {
+        OperandLane<T, P::EPL> la[2];
+        t_lanes(t, la);
+        load_a_frags<T, MODE>(la, bs, pre[q]);
+      }
+    }
     // X_i = D_i W_i, one column tile per wave, in place
     if (wv < ntc) {
       const int c0 = wv * 32;
       OperandLane<T, P::EPL> la[2];
 #pragma unroll
       for (int h = 0; h < 2; ++h) la[h].init(dinv + (int64_t)i * NB * NB, NB, !f_trans, 16 * h + li, bs);
+      T fa[CH][2][P::EPL];
+      load_a_frags<T, MODE>(la, bs, fa);
       acc_t acc[2][2];
 #pragma unroll
       for (int a = 0; a < 2; ++a) acc[a][0] = acc[a][1] = P::zero();
-      tile_from_lds<T, MODE>(la, bs, Ws + r0 * wp, wp, c0, acc);
+      tile_from_lds<T, MODE>(fa, Ws + r0 * wp, wp, c0, acc);
 #pragma unroll
       for (int a = 0; a < 2; ++a)
 #pragma unroll
@@ -195,20 +231,12 @@ __device__ __forceinline__ void narrow_solve(T* Wg, int ldw, bool w_trans, int m
     }
     __syncthreads();
     // W_rest -= T_{rest,i} X_i
-    const int q_lo = forward ? r0 + NB : 0, q_hi = forward ? mr : r0;
-    const int nrt = (q_hi - q_lo) >> 5, ntiles = nrt * ntc;
-    for (int t = wv; t < ntiles; t += nw) {
+    auto update = [&](int t, const T (&fa)[CH][2][P::EPL]) {
       const int q0 = q_lo + (t / ntc) * 32, c0 = (t % ntc) * 32;
-      OperandLane<T, P::EPL> la[2];
-#pragma unroll
-      for (int h = 0; h < 2; ++h) {
-        if (f_trans) la[h].init(F + (int64_t)r0 * ldf, ldf, false, q0 + 16 * h + li, m);
-        else la[h].init(F + r0, ldf, true, q0 + 16 * h + li, m);
-      }
       acc_t acc[2][2];
 #pragma unroll
       for (int a = 0; a < 2; ++a) acc[a][0] = acc[a][1] = P::zero();
-      tile_from_lds<T, MODE>(la, bs, Ws + r0 * wp, wp, c0, acc);
+      tile_from_lds<T, MODE>(fa, Ws + r0 * wp, wp, c0, acc);
 #pragma unroll
       for (int a = 0; a < 2; ++a)
 #pragma unroll
@@ -219,6 +247,18 @@ __device__ __forceinline__ void narrow_solve(T* Wg, int ldw, bool w_trans, int m
             PP_LDS T* wpx = Ws + row * wp + c0 + j * 16 + li;
             if (row < m) *wpx = *wpx - acc[a][j][r];
           }
+    };
+#pragma unroll
+    for (int q = 0; q < PRE; ++q) {
+      const int t = wv + q * nw;
+      if (t < ntiles) update(t, pre[q]);
+    }
+    for (int t = wv + PRE * nw; t < ntiles; t += nw) {
+      OperandLane<T, P::EPL> la[2];
+      t_lanes(t, la);
+      T fa[CH][2][P::EPL];
+      load_a_frags<T, MODE>(la, bs, fa);
+      update(t, fa);
     }
     __syncthreads();
   }

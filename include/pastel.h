@@ -239,6 +239,20 @@ int pp_solver_create_device(pp_ctx* ctx, const pp_hmatrix* h, const double* stor
 int pp_solver_create_batch(pp_ctx* ctx, const pp_hmatrix* const* hs, int count, const double* const* storages,
                            int on_device, int precision, int workers, int wavefronts_per_inst, int nrhs_max,
                            int rhs_split, pp_solver** out);
+/* Solve modes (no counterpart in the reference).  PP_SOLVE_HINV (the
+ * default of the functions above): the factorization launch also forms the
+ * full hierarchical inverses of HODLR-shaped members and each solve is two
+ * H-matrix-vector products; PP_SOLVE_LEAFINV: leaf inverses only, solve
+ * programs apply them (trmm); PP_SOLVE_SUBST: plain factorization, solve
+ * programs by substitution (trsml / trsmu).  The first moves work from every
+ * solve into each factorization, which pays off when solves are latency
+ * bound, the last is cheapest when one launch carries many matrices. */
+#define PP_SOLVE_HINV 0
+#define PP_SOLVE_LEAFINV 1
+#define PP_SOLVE_SUBST 2
+int pp_solver_create_batch_mode(pp_ctx* ctx, const pp_hmatrix* const* hs, int count, const double* const* storages,
+                                int on_device, int precision, int workers, int wavefronts_per_inst, int nrhs_max,
+                                int rhs_split, int solve_mode, pp_solver** out);
 int pp_solver_num_members(const pp_solver* s);
 int64_t pp_solver_rows(const pp_solver* s);
 /* Row offset, size and last refined residual of member i. */

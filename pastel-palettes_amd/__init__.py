@@ -15,6 +15,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libpastel.so")
 
 FP64, FP32, MIXED = 0, 1, 2
+SOLVE_HINV, SOLVE_LEAFINV, SOLVE_SUBST = 0, 1, 2  # solver modes (pp_solver_create_batch_mode)
 PRECISIONS = {"fp64": FP64, "fp32": FP32, "mixed": MIXED}
 
 _lib = None
@@ -112,6 +113,8 @@ def lib():
         "pp_solver_solve_flops": (dp, [vp, C.c_int]),
         "pp_solver_create_batch": (C.c_int, [vp, P(vp), C.c_int, P(vp), C.c_int, C.c_int, C.c_int, C.c_int,
                                              C.c_int, C.c_int, P(vp)]),
+        "pp_solver_create_batch_mode": (C.c_int, [vp, P(vp), C.c_int, P(vp), C.c_int, C.c_int, C.c_int, C.c_int,
+                                                  C.c_int, C.c_int, C.c_int, P(vp)]),
         "pp_solver_num_members": (C.c_int, [vp]),
         "pp_solver_rows": (i64, [vp]),
         "pp_solver_member": (C.c_int, [vp, C.c_int, P(i64), P(C.c_int), P(dp)]),
@@ -488,10 +491,17 @@ class Stream:
 class Solver:
     """Factor + solve + FP64 iterative refinement on one H-matrix."""
 
-    def __init__(self, ctx, hm, storage, precision=FP64, workers=None, wavefronts=4, nrhs_max=16, rhs_split=1):
+    def __init__(self, ctx, hm, storage, precision=FP64, workers=None, wavefronts=4, nrhs_max=16, rhs_split=1,
+                 solve_mode=None):
         if workers is None:
             workers = min(ctx.cu_count, 256)
         h = C.c_void_p()
+        if solve_mode not in (None, SOLVE_HINV):
+            b = Solver.batch(ctx, [hm], [storage], precision=precision, workers=workers, wavefronts=wavefronts,
+                             nrhs_max=nrhs_max, rhs_split=rhs_split, solve_mode=solve_mode)
+            self.ctx, self.hm, self.h, self.precision, self.workers = ctx, hm, b.h, precision, workers
+            b.h = None
+            return
         if isinstance(storage, np.ndarray):
             st = np.ascontiguousarray(storage, np.float64)
             assert st.size == hm.storage
@@ -503,7 +513,8 @@ class Solver:
         self.ctx, self.hm, self.h, self.precision, self.workers = ctx, hm, h, precision, workers
 
     @classmethod
-    def batch(cls, ctx, hms, storages, precision=FP64, workers=None, wavefronts=4, nrhs_max=4, rhs_split=1):
+    def batch(cls, ctx, hms, storages, precision=FP64, workers=None, wavefronts=4, nrhs_max=4, rhs_split=1,
+              solve_mode=None):
         """One solver over many H-matrices (block-diagonal batch): one launch
         factors all, one launch solves all.  storages: host arrays or device
         pointers (ints)."""
@@ -519,8 +530,12 @@ class Solver:
             arr = (C.c_void_p * len(keep))(*[a.ctypes.data for a in keep])
         hs = (C.c_void_p * len(hms))(*[m.h for m in hms])
         h = C.c_void_p()
-        check(lib().pp_solver_create_batch(ctx.h, hs, len(hms), arr, 1 if on_dev else 0, precision, workers,
-                                           wavefronts, nrhs_max, rhs_split, C.byref(h)))
+        if solve_mode is None:
+            check(lib().pp_solver_create_batch(ctx.h, hs, len(hms), arr, 1 if on_dev else 0, precision, workers,
+                                               wavefronts, nrhs_max, rhs_split, C.byref(h)))
+        else:
+            check(lib().pp_solver_create_batch_mode(ctx.h, hs, len(hms), arr, 1 if on_dev else 0, precision, workers,
+                                                    wavefronts, nrhs_max, rhs_split, solve_mode, C.byref(h)))
         del keep
         self.ctx, self.hm, self.h, self.precision, self.workers = ctx, list(hms), h, precision, workers
         return self

@@ -11,8 +11,15 @@
 #include "rts/h_ops_dag.h"
 #include "rts/scheduler.h"
 
+struct pp_program;
+struct pp_hmatrix;
+
 namespace ppc {
 int set_error(int code, const std::string& msg);
+// Factorization program followed by the inverse construction: the full
+// hierarchical inverses of HODLR-shaped members when `hier`, else the leaf
+// inverses only.
+int factor_inverse_program(const pp_hmatrix* const* hs, int count, int workers, bool hier, pp_program** out);
 // Convert the in-flight exception to an error code (call inside catch).
 int from_exception();
 }  // namespace ppc

@@ -300,6 +300,12 @@ int pp_program_factor(const pp_hmatrix* h, int workers, pp_program** out) {
 }
 
 int pp_program_factor_inv_batch(const pp_hmatrix* const* hs, int count, int workers, pp_program** out) {
+  return ppc::factor_inverse_program(hs, count, workers, true, out);
+}
+
+}  // extern "C"
+
+int ppc::factor_inverse_program(const pp_hmatrix* const* hs, int count, int workers, bool hier, pp_program** out) {
   try {
     if (hs == nullptr || count < 1 || workers < 1 || out == nullptr) return set_error(PP_ERR_INVALID, "bad args");
     auto p = std::make_unique<pp_program>();
@@ -320,7 +326,7 @@ int pp_program_factor_inv_batch(const pp_hmatrix* const* hs, int count, int work
     for (int i = 0; i < count; ++i) {
       const Hierarchical* h = hs[i]->h.get();
       std::unique_ptr<const h_index> root(h->getRootIndex());
-      std::unique_ptr<h_ops_tree> v(h->hodlrShaped()
+      std::unique_ptr<h_ops_tree> v(hier && h->hodlrShaped()
                                         ? h->generateOps_InvertH(root.get(), inverse_base(h), &tmps[i], 3 * i)
                                         : h->generateOps_Invert(root.get(), inverse_base(h), 3 * i));
       p->tree.append(*v);
@@ -333,6 +339,8 @@ int pp_program_factor_inv_batch(const pp_hmatrix* const* hs, int count, int work
     return from_exception();
   }
 }
+
+extern "C" {
 
 int pp_program_solve_batch(const pp_hmatrix* const* hs, int count, const int32_t* nrhs, const int32_t* ld,
                            int rhs_split, int workers, pp_program** out) {

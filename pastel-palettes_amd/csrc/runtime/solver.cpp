@@ -57,6 +57,7 @@ struct pp_solver {
   // Hierarchical-inverse solve (every member HODLR-shaped): x = U^{-1} L^{-1} b
   // as two H-matrix products over the inverse areas of the factor arena.
   bool hinv = false;
+  int solve_mode = PP_SOLVE_HINV;
   pp::MatvecPlanDev lplan{}, uplan{};
   void* d_rhs2 = nullptr;  // n_total x nrhs_max, storage precision
   void* d_tinv = nullptr;  // V^T x products of lplan / uplan, storage precision
@@ -263,11 +264,13 @@ std::vector<void*> slot_table(pp_solver* s, int nrhs) {
 }
 
 int solver_create(pp_ctx* ctx, const pp_hmatrix* const* hs, int count, const double* const* storages, bool on_device,
-                  int precision, int workers, int wpi, int nrhs_max, int rhs_split, pp_solver** out) {
+                  int precision, int workers, int wpi, int nrhs_max, int rhs_split, int mode, pp_solver** out) {
   if (ctx == nullptr || hs == nullptr || count < 1 || storages == nullptr || out == nullptr || precision < 0 ||
-      precision > 2 || workers < 1 || nrhs_max < 1 || rhs_split < 1 || (wpi != 4 && wpi != 8))
+      precision > 2 || workers < 1 || nrhs_max < 1 || rhs_split < 1 || (wpi != 4 && wpi != 8) ||
+      mode < PP_SOLVE_HINV || mode > PP_SOLVE_SUBST)
     return set_error(PP_ERR_INVALID, "bad solver args");
   auto s = new pp_solver();
+  s->solve_mode = mode;
   s->ctx = ctx;
   s->precision = precision;
   s->workers = workers;
@@ -283,7 +286,8 @@ int solver_create(pp_ctx* ctx, const pp_hmatrix* const* hs, int count, const dou
     s->arena_off.push_back(s->storage_total);
     s->row_off.push_back(s->n_total);
     const Hierarchical* h = hier(hs[i]);
-    s->storage_total += (inverse_base(h) + h->inverseStorage() + 63) / 64 * 64;
+    s->storage_total += mode == PP_SOLVE_SUBST ? (h->getStorage() + 63) / 64 * 64
+                                               : (inverse_base(h) + h->inverseStorage() + 63) / 64 * 64;
     s->n_total += hier(hs[i])->getNx_abs();
   }
   int rc = PP_OK;
@@ -304,6 +308,7 @@ int solver_create(pp_ctx* ctx, const pp_hmatrix* const* hs, int count, const dou
         rc = hip_fail(e, "upload matrix");
         break;
       }
+      if (mode == PP_SOLVE_SUBST) continue;
       std::vector<double> ident(h->inverseStorage());
       if (pp_hmatrix_inverse_identity(hs[i], ident.data()) != PP_OK) {
         rc = PP_ERR_INTERNAL;
@@ -323,14 +328,16 @@ int solver_create(pp_ctx* ctx, const pp_hmatrix* const* hs, int count, const dou
     if ((rc = ctx_alloc_t(ctx, 2 * (int64_t)count * nrhs_max, &s->d_nrm))) break;
     s->owned.push_back(s->d_nrm);
     if ((rc = build_plan(s))) break;
-    if ((rc = build_inverse_plans(s))) break;
-    if ((rc = pp_program_factor_inv_batch(hs, count, workers, &s->fprog))) break;
+    if (mode == PP_SOLVE_HINV && (rc = build_inverse_plans(s))) break;
+    if (mode == PP_SOLVE_SUBST) rc = pp_program_factor_batch(hs, count, workers, &s->fprog);
+    else rc = ppc::factor_inverse_program(hs, count, workers, mode == PP_SOLVE_HINV, &s->fprog);
+    if (rc) break;
     std::vector<void*> ptrs = slot_table(s, nrhs_max);
     if ((rc = pp_stream_upload_program(ctx, s->fprog, const_cast<const void* const*>(ptrs.data()), (int)ptrs.size(),
                                        precision, &s->fstream)))
       break;
     // H-LU FLOPs only: the inversions are part of the solve's set-up
-    s->factor_flops = (double)s->fprog->primary_flops;
+    s->factor_flops = s->fprog->primary_flops >= 0 ? (double)s->fprog->primary_flops : (double)s->fprog->dag->getFlops();
     s->fstream->role = 1;
     s->member_res.assign(count, 0.);
   } while (false);
@@ -357,8 +364,11 @@ int solve_prog(pp_solver* s, int nrhs, pp_solver::SolveProg** out) {
   if (it == s->sprogs.end()) {
     pp_solver::SolveProg sp;
     std::vector<int32_t> nr(s->hms.size(), nrhs), ld(s->hms.size(), nrhs);
-    int rc = pp_program_solve_inv_batch(s->hms.data(), (int)s->hms.size(), nr.data(), ld.data(), s->rhs_split,
-                                        s->workers, &sp.prog);
+    int rc = s->solve_mode == PP_SOLVE_SUBST
+                 ? pp_program_solve_batch(s->hms.data(), (int)s->hms.size(), nr.data(), ld.data(), s->rhs_split,
+                                          s->workers, &sp.prog)
+                 : pp_program_solve_inv_batch(s->hms.data(), (int)s->hms.size(), nr.data(), ld.data(),
+                                              s->rhs_split, s->workers, &sp.prog);
     if (rc) return rc;
     std::vector<void*> ptrs = slot_table(s, nrhs);
     rc = pp_stream_upload_program(s->ctx, sp.prog, const_cast<const void* const*>(ptrs.data()), (int)ptrs.size(),
@@ -397,7 +407,7 @@ int pp_solver_create(pp_ctx* ctx, const pp_hmatrix* h, const double* storage, in
                      int wavefronts_per_inst, int nrhs_max, int rhs_split, pp_solver** out) {
   try {
     return solver_create(ctx, &h, 1, &storage, false, precision, workers, wavefronts_per_inst, nrhs_max, rhs_split,
-                         out);
+                         PP_SOLVE_HINV, out);
   } catch (...) {
     return from_exception();
   }
@@ -407,7 +417,7 @@ int pp_solver_create_device(pp_ctx* ctx, const pp_hmatrix* h, const double* stor
                             int wavefronts_per_inst, int nrhs_max, int rhs_split, pp_solver** out) {
   try {
     return solver_create(ctx, &h, 1, &storage_dev, true, precision, workers, wavefronts_per_inst, nrhs_max, rhs_split,
-                         out);
+                         PP_SOLVE_HINV, out);
   } catch (...) {
     return from_exception();
   }
@@ -418,7 +428,18 @@ int pp_solver_create_batch(pp_ctx* ctx, const pp_hmatrix* const* hs, int count, 
                            int rhs_split, pp_solver** out) {
   try {
     return solver_create(ctx, hs, count, storages, on_device != 0, precision, workers, wavefronts_per_inst, nrhs_max,
-                         rhs_split, out);
+                         rhs_split, PP_SOLVE_HINV, out);
+  } catch (...) {
+    return from_exception();
+  }
+}
+
+int pp_solver_create_batch_mode(pp_ctx* ctx, const pp_hmatrix* const* hs, int count, const double* const* storages,
+                                int on_device, int precision, int workers, int wavefronts_per_inst, int nrhs_max,
+                                int rhs_split, int solve_mode, pp_solver** out) {
+  try {
+    return solver_create(ctx, hs, count, storages, on_device != 0, precision, workers, wavefronts_per_inst, nrhs_max,
+                         rhs_split, solve_mode, out);
   } catch (...) {
     return from_exception();
   }

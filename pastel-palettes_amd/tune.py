@@ -12,6 +12,10 @@ cache file between runs.
 Wavefronts per instruction: the VM kernel is instantiated for 4 wavefronts
 (one 256-thread workgroup per worker) only, so that knob has one candidate
 here; the leaf order is the one that moves the result.
+
+The solve mode (hierarchical inverses formed in the factorization launch,
+leaf inverses only, or substitution) is chosen for a whole batch by timing a
+sample of its members in each mode.
 """
 import json
 import math
@@ -100,3 +104,35 @@ class Autotuner:
         c = dict(self.cache[key])
         c["levels"] = hodlr_levels(n, c["leaf"])
         return c
+
+
+def choose_solve_mode(pp, torch, ctx, hms, arenas, precision, nrhs=4, tol=1e-10, workers=None, sample=160,
+                      modes=None, reps=2, log=None):
+    """Solve mode with the least time per matrix for a sample (every k-th
+    member) of a batch: returns (mode, {mode: ms per matrix})."""
+    log = log or (lambda *a: None)
+    workers = workers or min(ctx.cu_count, 256)
+    modes = modes or (pp.SOLVE_HINV, pp.SOLVE_LEAFINV, pp.SOLVE_SUBST)
+    step = max(1, len(hms) // sample)
+    idx = list(range(0, len(hms), step))[:sample]
+    sh, sa = [hms[i] for i in idx], [arenas[i] for i in idx]
+    b = np.concatenate([pp.rhs(4 + (i % 64), sh[q].n, nrhs) for q, i in enumerate(idx)], axis=0)
+    db = torch.from_numpy(b).cuda()
+    dx = torch.zeros_like(db)
+    cost = {}
+    for mode in modes:
+        s = pp.Solver.batch(ctx, sh, sa, precision=precision, workers=workers, nrhs_max=nrhs, solve_mode=mode)
+        try:
+            s.factor()
+            s.solve(db.data_ptr(), dx.data_ptr(), nrhs, tol=tol, max_iter=60)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(reps):
+                s.factor()
+                rep = s.solve(db.data_ptr(), dx.data_ptr(), nrhs, tol=tol, max_iter=60)
+            torch.cuda.synchronize()
+            cost[mode] = 1e3 * (time.perf_counter() - t0) / (reps * len(sh)) if rep["converged"] else float("inf")
+        finally:
+            s.close()
+        log(f"[tune] solve mode {mode}: {cost[mode]:.3f} ms/matrix over {len(sh)} sampled matrices")
+    return min(cost, key=cost.get), cost

@@ -88,7 +88,15 @@ def run_batch(pp, torch, ctx, count=512, nrhs=4, steps=3, warmup=1, tol=1e-10, p
     log(f"[batch] built in {build_s:.1f} s; creating the batched solver")
     hms = [pp.HMatrix("hodlr", n, levels=lv, rank=r) for n, lv, r, _ in jobs]
     t0 = time.perf_counter()
-    solver = pp.Solver.batch(ctx, hms, arenas, precision=precision, workers=workers, nrhs_max=nrhs)
+    from .tune import choose_solve_mode
+    # the mode is timed on the whole batch: a sample of it is less throughput
+    # bound than the batch and favours the mode with the cheapest solves
+    mode, mode_cost = choose_solve_mode(pp, torch, ctx, hms, arenas, precision, nrhs=nrhs, tol=tol, workers=workers,
+                                        sample=len(hms), reps=1, log=log)
+    tune_s += time.perf_counter() - t0
+    log(f"[batch] solve mode {mode}")
+    t0 = time.perf_counter()
+    solver = pp.Solver.batch(ctx, hms, arenas, precision=precision, workers=workers, nrhs_max=nrhs, solve_mode=mode)
     solver.use_graph(True)
     setup_s = time.perf_counter() - t0
     log(f"[batch] solver ready in {setup_s:.1f} s ({int(sum(sizes))} rows)")
@@ -126,6 +134,8 @@ def run_batch(pp, torch, ctx, count=512, nrhs=4, steps=3, warmup=1, tol=1e-10, p
         "iterations": max(r["iterations"] for r in reps),
         "converged": all(r["converged"] for r in reps),
         "leaf_choices": leaf_hist,
+        "solve_mode": {0: "hierarchical inverse", 1: "leaf inverses", 2: "substitution"}[mode],
+        "solve_mode_ms_per_matrix": {str(k): v for k, v in mode_cost.items()},
         "tuned_buckets": {k: {"leaf": v["leaf"], "ms_per_matrix": v["ms_per_matrix"],
                               "candidates": v["candidates"]} for k, v in tuner.cache.items()},
         "tune_s": tune_s, "build_s": build_s, "setup_s": setup_s, "workers": workers,

@@ -114,3 +114,26 @@ def test_serving_alone_equals_mix(pp, ctx):
         assert a.report["converged"] and m.report["converged"]
         assert a.report["residual"] <= 1e-10
         assert np.array_equal(a.x, m.x), f"request {a.rid} differs alone vs in the mix"
+
+
+@pytest.mark.parametrize("mode", [1, 2])
+def test_solve_modes_agree(pp, ctx, mode):
+    """Leaf-inverse and substitution solves converge to the solution of the
+    hierarchical-inverse solve (tolerance 1e-10 relative after refinement to
+    1e-12)."""
+    print(f"[case] solve mode {mode}", flush=True)
+    mem = _members(pp, [512, 1024, 768])
+    nrhs = 3
+    B = np.concatenate([pp.rhs(3, hm.n, nrhs) for hm, _ in mem], axis=0)
+    out = {}
+    for m in (0, mode):
+        s = pp.Solver.batch(ctx, [x[0] for x in mem], [x[1] for x in mem], precision=2, workers=16, nrhs_max=4,
+                            solve_mode=m)
+        s.factor()
+        db = torch.from_numpy(B).cuda()
+        dx = torch.zeros_like(db)
+        rep = s.solve(db.data_ptr(), dx.data_ptr(), nrhs, tol=1e-12, max_iter=60)
+        assert rep["converged"]
+        out[m] = dx.cpu().numpy()
+        s.close()
+    assert np.linalg.norm(out[mode] - out[0]) <= 1e-10 * np.linalg.norm(out[0])

@@ -100,6 +100,137 @@ __device__ __forceinline__ void diag_inverses(const T* F, int ldf, int n, T* out
   }
 }
 
+// ---------------------------------------------------------------- panel update
+// After the panel at p (width pw, multipliers L21 below, L11^{-1} in linv):
+// U12 = L11^{-1} A12 and A22 -= L21 U12 in one step.  Each wave owns column
+// tiles of U12: it forms the 32-column tile (stored to A12), keeps it as the
+// B operand in registers (through a wave-private LDS tile) and applies it to
+// every row tile of A22 in that column, so the two products need no barrier
+// or global round trip between them.  A22's L21 fragments and C tile are
+// requested one row tile ahead.
+template <typename T, int MODE>
+__device__ __forceinline__ void lu_update(T* M, int ld, int p, int pw, int nx, int ny, const T* linv) {
+  using P = Mma<T, MODE>;
+  using acc_t = typename P::acc_t;
+  constexpr int KW = P::KW, EPL = P::EPL, CH = NB / KW, V = 16 / sizeof(T), UT = NB + 2;
+  const int rn = nx - p - pw, rm = ny - p - pw;
+  const int nw = blockDim.x >> 6, w = wave_id(), l = lane_id(), g = l >> 4, li = l & 15;
+  const int ctn = (rn + 31) >> 5, rtn = (rm + 31) >> 5;
+  T* A12 = M + (int64_t)p * ld + p + pw;
+  const T* L21 = M + (int64_t)(p + pw) * ld + p;
+  T* A22 = M + (int64_t)(p + pw) * ld + p + pw;
+  PP_LDS T* ut = (PP_LDS T*)lds_base() + w * NB * UT;
+  const bool lvec = pw == NB && (ld % V) == 0 && ((reinterpret_cast<uintptr_t>(M) & 15) == 0) && (p % V) == 0;
+  for (int ct = w; ct < ctn; ct += nw) {
+    const int c0 = ct * 32;
+    // U12 tile
+    {
+      OperandLane<T, EPL> la[2], lb[2];
+      T fa[CH][2][EPL], fb[CH][2][EPL];
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        la[h].init(linv, NB, true, 16 * h + li, pw);
+        lb[h].init(A12, ld, false, c0 + 16 * h + li, rn);
+      }
+#pragma unroll
+      for (int c = 0; c < CH; ++c)
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+          la[h].template load<false>(fa[c][h], c * KW + EPL * g, pw);
+          lb[h].template load<false>(fb[c][h], c * KW + EPL * g, pw);
+#pragma unroll
+          for (int e = 0; e < EPL; ++e)  // k >= pw: the clamped loads cancel through A
+            if (c * KW + EPL * g + e >= pw) fa[c][h][e] = T(0);
+        }
+      acc_t acc[2][2];
+#pragma unroll
+      for (int a = 0; a < 2; ++a) acc[a][0] = acc[a][1] = P::zero();
+#pragma unroll
+      for (int c = 0; c < CH; ++c)
+#pragma unroll
+        for (int a = 0; a < 2; ++a)
+#pragma unroll
+          for (int j = 0; j < 2; ++j) acc[a][j] = P::mma_chunk(fa[c][a], fb[c][j], acc[a][j]);
+#pragma unroll
+      for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            const int row = a * 16 + P::orow(l, r), col = j * 16 + li;
+            const T v = acc[a][j][r];
+            ut[row * UT + col] = row < pw ? v : T(0);
+            if (row < pw && c0 + col < rn) gstore<T>(A12 + (int64_t)row * ld + c0 + col, v);
+          }
+    }
+    if (rtn == 0) continue;
+    // the U12 tile as B fragments (the wave's own LDS writes, in order)
+    T ub[CH][2][EPL];
+#pragma unroll
+    for (int c = 0; c < CH; ++c)
+#pragma unroll
+      for (int h = 0; h < 2; ++h)
+#pragma unroll
+        for (int e = 0; e < EPL; ++e) ub[c][h][e] = ut[(c * KW + EPL * g + e) * UT + 16 * h + li];
+    auto load_tile = [&](int rt, T (&fa)[CH][2][EPL], T (&cv)[2][2][4]) {
+      const int r0 = rt * 32;
+      OperandLane<T, EPL> la[2];
+#pragma unroll
+      for (int h = 0; h < 2; ++h) la[h].init(L21, ld, true, r0 + 16 * h + li, rm);
+#pragma unroll
+      for (int c = 0; c < CH; ++c)
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+          if (lvec) la[h].template load<true>(fa[c][h], c * KW + EPL * g, pw);
+          else la[h].template load<false>(fa[c][h], c * KW + EPL * g, pw);
+        }
+#pragma unroll
+      for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            const int row = r0 + a * 16 + P::orow(l, r), col = c0 + j * 16 + li;
+            cv[a][j][r] = gload<T>(A22 + (int64_t)(row < rm ? row : rm - 1) * ld + (col < rn ? col : rn - 1));
+          }
+    };
+    auto apply_tile = [&](int rt, const T (&fa)[CH][2][EPL], const T (&cv)[2][2][4]) {
+      const int r0 = rt * 32;
+      acc_t acc[2][2];
+#pragma unroll
+      for (int a = 0; a < 2; ++a) acc[a][0] = acc[a][1] = P::zero();
+#pragma unroll
+      for (int c = 0; c < CH; ++c)
+#pragma unroll
+        for (int a = 0; a < 2; ++a)
+#pragma unroll
+          for (int j = 0; j < 2; ++j) acc[a][j] = P::mma_chunk(fa[c][a], ub[c][j], acc[a][j]);
+#pragma unroll
+      for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            const int row = r0 + a * 16 + P::orow(l, r), col = c0 + j * 16 + li;
+            if (row < rm && col < rn) gstore<T>(A22 + (int64_t)row * ld + col, cv[a][j][r] - acc[a][j][r]);
+          }
+    };
+    T fa0[CH][2][EPL], fa1[CH][2][EPL], cv0[2][2][4], cv1[2][2][4];
+    load_tile(0, fa0, cv0);
+    for (int rt = 0; rt < rtn; rt += 2) {
+      // past the last tile the last one is requested again, so every
+      // iteration issues the same loads
+      load_tile(rt + 1 < rtn ? rt + 1 : rt, fa1, cv1);
+      apply_tile(rt, fa0, cv0);
+      if (rt + 1 < rtn) {
+        load_tile(rt + 2 < rtn ? rt + 2 : rt + 1, fa0, cv0);
+        apply_tile(rt + 1, fa1, cv1);
+      }
+    }
+  }
+  __syncthreads();
+}
+
 // ---------------------------------------------------------------- narrow solve
 // In-place triangular solve with few right-hand-side columns, the right-hand
 // side resident in LDS for the whole solve:

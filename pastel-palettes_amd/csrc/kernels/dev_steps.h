@@ -23,7 +23,17 @@ __device__ __forceinline__ T* slot(void* const* ptrs, const int32_t* pc, int i) 
   return reinterpret_cast<T*>(ptrs[word(pc, i)]);
 }
 
-enum : int { ST_NONE = 0, ST_GEMM = 1, ST_PANEL = 2, ST_DIAGINV = 3, ST_QR = 4, ST_COPY = 5, ST_CHOL = 6, ST_TSOLVE = 7 };
+enum : int {
+  ST_NONE = 0,
+  ST_GEMM = 1,
+  ST_PANEL = 2,
+  ST_DIAGINV = 3,
+  ST_QR = 4,
+  ST_COPY = 5,
+  ST_CHOL = 6,
+  ST_TSOLVE = 7,
+  ST_LUUPD = 8
+};
 
 // One primitive step.  Field use per kind:
 //   GEMM     C = beta C + alpha op(A) op(B), m x n x k; leaf selects the
@@ -34,6 +44,7 @@ enum : int { ST_NONE = 0, ST_GEMM = 1, ST_PANEL = 2, ST_DIAGINV = 3, ST_QR = 4, 
 //            tau = A, w = A + k)
 //   COPY     C[i][j] = A[i][j] for i < m, j < n (pitches ldc, lda)
 //   CHOL     chol_upper(G = C, k = m, rows = k, first = aT)
+//   LUUPD    lu_update(M = C, ld = ldc, p = m, pw = n, nx = lda, ny = k, linv = A)
 //   TSOLVE   narrow_solve(W = C, ldw = ldc, w_trans = bT, m, w = n, F = A,
 //            ldf = lda, f_trans = aT, dinv = B, forward = k != 0)
 template <typename T>
@@ -118,28 +129,23 @@ __device__ __forceinline__ int make_step(const int32_t* pc, void* const* P, int 
     case nop:
       return 0;
     case getrf: {
-      // right-looking blocked LU: per 32-column panel: PANEL, U12, trailing
+      // right-looking blocked LU: per 32-column panel, PANEL then LUUPD
+      // (U12 = L11^{-1} A12 and A22 -= L21 U12)
       T* M = slot<T>(P, pc, 3) + word(pc, 4);
       const int nx = word(pc, 5), ny = word(pc, 6), ld = word(pc, 7);
       const int mn = nx < ny ? nx : ny, np = (mn + NB - 1) / NB;
-      if (s >= 3 * np) return 0;
-      const int q = s / 3, r = s - 3 * q, p = q * NB;
-      const int pw = (mn - p) < NB ? (mn - p) : NB, rn = nx - p - pw, rm = ny - p - pw;
-      T* A12 = M + (int64_t)p * ld + p + pw;
-      if (r == 0) {
-        st.kind = ST_PANEL;
-        st.C = M;
-        st.ldc = ld;
-        st.m = p;
-        st.n = pw;
-        st.k = ny;
-        st.A = tmp;
-      } else if (r == 1) {  // U12 = L11^{-1} A12 (in place)
-        set_gemm<T>(st, true, pw, rn, pw, T(1), T(0), A12, ld, tmp, NB, false, A12, ld, false);
-      } else {  // A22 -= L21 U12
-        set_gemm<T>(st, true, rm, rn, pw, T(-1), T(1), M + (int64_t)(p + pw) * ld + p + pw, ld,
-                    M + (int64_t)(p + pw) * ld + p, ld, false, A12, ld, false);
-      }
+      if (s >= 2 * np) return 0;
+      const int q = s >> 1, r = s & 1, p = q * NB;
+      const int pw = (mn - p) < NB ? (mn - p) : NB;
+      st.kind = r == 0 ? ST_PANEL : ST_LUUPD;
+      st.leaf = true;
+      st.C = M;
+      st.ldc = ld;
+      st.m = p;
+      st.n = pw;
+      st.k = ny;
+      st.lda = nx;
+      st.A = tmp;
       return 1;
     }
     case trsml: {
@@ -496,6 +502,9 @@ __device__ __forceinline__ void exec_step(const Step<T>& st) {
       break;
     case ST_CHOL:
       chol_upper<T>(st.C, st.m, st.k, st.aT);
+      break;
+    case ST_LUUPD:
+      lu_update<T, FMODE>(st.C, st.ldc, st.m, st.n, st.lda, st.k, st.A);
       break;
     case ST_TSOLVE:
       narrow_solve<T, FMODE>(st.C, st.ldc, st.bT, st.m, st.n, st.A, st.lda, st.aT, st.B, st.k != 0);

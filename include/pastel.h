@@ -277,6 +277,40 @@ double pp_solver_factor_flops(const pp_solver* s);
 double pp_solver_last_factor_ms(const pp_solver* s);
 double pp_solver_solve_flops(const pp_solver* s, int nrhs);
 
+/* ------------------------------------------------------------------ byte-string format
+ * H-matrices, instruction programs and factorizations as byte strings, for
+ * files, caches and the wire.  No counterpart in the reference, which has no
+ * persistent format (its docs/CONTRIBUTING.md:27-31 only mentions an external
+ * compressor); layout: magic "PPHL", u32 version 1, u32 section count, u32 0,
+ * then sections (u32 tag, u32 0, u64 length, payload padded to 8 bytes, u64
+ * FNV-1a 64 of the payload), little-endian.  Encoders return the string's
+ * length in bytes (negative = error) and write it only when buf is non-NULL
+ * and cap is at least that length: call once to size, once to fill.
+ * Decoders check the header, every length and checksum and the structure
+ * before they build anything, and fail with PP_ERR_INVALID otherwise. */
+
+/* Structure (TREE) and, when arena is non-NULL, the FP64 arena (ARNA). */
+int64_t pp_hmatrix_encode(const pp_hmatrix* h, const double* arena, uint8_t* buf, uint64_t cap);
+/* Rebuilds the structure; copies the arena into `arena` (capacity
+ * arena_cap elements) when non-NULL, which fails if the string holds none.
+ * *has_arena (optional) tells whether it does. */
+int pp_hmatrix_decode(const uint8_t* buf, uint64_t len, double* arena, uint64_t arena_cap, int* has_arena,
+                      pp_hmatrix** out);
+/* Instructions, dependencies, schedule and per-worker streams (PROG).  A
+ * decoded program answers every pp_program_* query as the encoded one did and
+ * uploads with pp_stream_upload_program without rebuilding its DAG and
+ * schedule.  Its streams are checked against its instructions; operand
+ * offsets are trusted as words given to pp_stream_upload are. */
+int64_t pp_program_encode(const pp_program* p, uint8_t* buf, uint64_t cap);
+int pp_program_decode(const uint8_t* buf, uint64_t len, pp_program** out);
+/* Factor arenas of a solver (FACT, in its storage precision, including the
+ * inverse areas): a factorization computed once and loaded later into a
+ * solver made over the same matrices, precision and solve mode, which can then
+ * solve without factoring.  The decoder rejects a string whose layout,
+ * structures or FP64 matrix entries (by checksum) differ from the solver's. */
+int64_t pp_solver_encode_factors(pp_solver* s, uint8_t* buf, uint64_t cap);
+int pp_solver_decode_factors(pp_solver* s, const uint8_t* buf, uint64_t len);
+
 #ifdef __cplusplus
 }
 #endif

@@ -120,6 +120,12 @@ def lib():
         "pp_solver_member": (C.c_int, [vp, C.c_int, P(i64), P(C.c_int), P(dp)]),
         "pp_solver_use_graph": (C.c_int, [vp, C.c_int]),
         "pp_solver_last_factor_ms": (dp, [vp]),
+        "pp_hmatrix_encode": (i64, [vp, P(dp), vp, u64]),
+        "pp_hmatrix_decode": (C.c_int, [vp, u64, P(dp), u64, P(C.c_int), P(vp)]),
+        "pp_program_encode": (i64, [vp, vp, u64]),
+        "pp_program_decode": (C.c_int, [vp, u64, P(vp)]),
+        "pp_solver_encode_factors": (i64, [vp, vp, u64]),
+        "pp_solver_decode_factors": (C.c_int, [vp, vp, u64]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -137,6 +143,14 @@ def check(rc):
 
 def _p(a, ct):
     return a.ctypes.data_as(C.POINTER(ct))
+
+
+def _encode(fn, *args):
+    """Runs a size-then-fill encoder of the byte-string format -> bytes."""
+    n = check(fn(*args, None, 0))
+    buf = C.create_string_buffer(n)
+    check(fn(*args, buf, n))
+    return buf.raw
 
 
 class ProgramInfo(C.Structure):
@@ -226,6 +240,32 @@ class HMatrix:
         else:
             raise ValueError(kind)
         self.h, self.kind, self.n, self.params = h, kind, n, kw
+
+    def to_bytes(self, arena=None):
+        """Byte string of the structure and, when given, the FP64 arena."""
+        if arena is not None:
+            arena = np.ascontiguousarray(arena, dtype=np.float64)
+            if arena.size != self.storage:
+                raise ValueError(f"arena has {arena.size} elements, the structure {self.storage}")
+            return _encode(lib().pp_hmatrix_encode, self.h, _p(arena, C.c_double))
+        return _encode(lib().pp_hmatrix_encode, self.h, None)
+
+    @classmethod
+    def from_bytes(cls, data):
+        """(HMatrix, arena or None) from a string made by to_bytes."""
+        L = lib()
+        data = bytes(data)
+        h, has = C.c_void_p(), C.c_int()
+        check(L.pp_hmatrix_decode(data, len(data), None, 0, C.byref(has), C.byref(h)))
+        self = cls.__new__(cls)
+        self.h, self.kind, self.n, self.params = h, "decoded", int(L.pp_hmatrix_size(h)), {}
+        arena = None
+        if has.value:
+            arena = np.empty(self.storage)
+            h2 = C.c_void_p()
+            check(L.pp_hmatrix_decode(data, len(data), _p(arena, C.c_double), arena.size, None, C.byref(h2)))
+            L.pp_hmatrix_destroy(h2)
+        return self, arena
 
     @property
     def storage(self):
@@ -383,6 +423,17 @@ class Program:
         fl = np.zeros(max(n, 1), np.int64)
         check(L.pp_program_schedule_entries(self.h, w, _p(ins, C.c_int32), _p(ex, C.c_int32), _p(fl, C.c_int64)))
         return [(int(i), bool(x), int(f)) for i, x, f in zip(ins[:n], ex[:n], fl[:n])]
+
+    def to_bytes(self):
+        """Byte string of the instructions, dependencies, schedule and streams."""
+        return _encode(lib().pp_program_encode, self.h)
+
+    @classmethod
+    def from_bytes(cls, data):
+        data = bytes(data)
+        h = C.c_void_p()
+        check(lib().pp_program_decode(data, len(data), C.byref(h)))
+        return cls(h, None)
 
     def __del__(self):
         if getattr(self, "h", None) and _lib is not None:
@@ -579,6 +630,18 @@ class Solver:
 
     def factor_stream_ms(self):
         return float(lib().pp_stream_last_ms(lib().pp_solver_factor_stream(self.h)))
+
+    def factors_to_bytes(self):
+        """Byte string of the factor arenas (after factor()), loadable with
+        factors_from_bytes into a solver over the same matrices, precision and
+        solve mode."""
+        return _encode(lib().pp_solver_encode_factors, self.h)
+
+    def factors_from_bytes(self, data):
+        """Loads factors made by factors_to_bytes; solve() then runs without
+        factor()."""
+        data = bytes(data)
+        check(lib().pp_solver_decode_factors(self.h, data, len(data)))
 
     @property
     def factor_flops(self):

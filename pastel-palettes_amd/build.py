@@ -26,6 +26,7 @@ HOST_SOURCES = [
     "capi_host.cpp",
     "runtime/runtime.cpp",
     "runtime/solver.cpp",
+    "io/wire.cpp",
 ]
 HIP_SOURCES = ["kernels/vm.hip"]
 

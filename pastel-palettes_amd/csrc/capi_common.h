@@ -33,6 +33,21 @@ struct pp_hmatrix {
 // aligned.
 inline int64_t inverse_base(const Hierarchical* h) { return (h->getStorage() + 63) / 64 * 64; }
 
+// What a decoded program carries in place of its DAG, schedule and encoder
+// (the accessors read it when `dag` is null).
+struct pp_program_image {
+  int32_t n_insts = 0;
+  int64_t n_edges = 0, flops = 0, comm_waits = 0;
+  double dag_density = 0;
+  std::vector<std::vector<int32_t>> streams;  // per worker
+  std::vector<std::vector<int32_t>> preds;    // per instruction: (inst, mask) pairs
+  struct entry {
+    int32_t inst, ex_w;
+    int64_t start_flops;
+  };
+  std::vector<std::vector<entry>> sched;  // per worker
+};
+
 struct pp_program {
   h_ops_tree tree;
   std::unique_ptr<h_ops_dag> dag;
@@ -44,5 +59,13 @@ struct pp_program {
   int64_t scratch = 0;              // per-worker scratch elements
   long long primary_flops = -1;     // FLOPs of the factorization ops alone (factor + inverse programs)
   double dag_ms = 0, schedule_ms = 0, encode_ms = 0;
+  std::unique_ptr<pp_program_image> image;  // decoded programs only
   ~pp_program();
 };
+
+namespace ppc {
+// FNV-1a 64 of the matrix structure as the byte-string format encodes it.
+uint64_t tree_fingerprint(const pp_hmatrix* h);
+int program_insts(const pp_program* p);
+const std::vector<int32_t>& program_stream(const pp_program* p, int worker);
+}  // namespace ppc

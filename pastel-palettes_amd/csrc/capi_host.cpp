@@ -249,6 +249,12 @@ int pp_hmatrix_leaf(const pp_hmatrix* h, int i, int64_t info[8]) {
 
 pp_program::~pp_program() = default;
 
+int ppc::program_insts(const pp_program* p) { return p->dag ? p->dag->getLength() : p->image->n_insts; }
+
+const std::vector<int32_t>& ppc::program_stream(const pp_program* p, int worker) {
+  return p->dag ? p->mgr->getStream(worker) : p->image->streams[worker];
+}
+
 namespace {
 
 double ms_since(std::chrono::steady_clock::time_point t0) {
@@ -399,18 +405,21 @@ void pp_program_destroy(pp_program* p) { delete p; }
 int pp_program_get_info(const pp_program* p, pp_program_info* info) {
   if (p == nullptr || info == nullptr) return set_error(PP_ERR_INVALID, "null");
   std::memset(info, 0, sizeof(*info));
-  info->n_insts = p->dag->getLength();
+  const pp_program_image* im = p->image.get();
+  info->n_insts = ppc::program_insts(p);
   info->workers = p->workers;
   info->n_slots = p->n_slots;
-  info->n_edges = p->dag->getEdgeCount();
-  info->flops = p->dag->getFlops();
+  info->n_edges = p->dag ? p->dag->getEdgeCount() : im->n_edges;
+  info->flops = p->dag ? p->dag->getFlops() : im->flops;
   int64_t t = 0;
   for (int64_t e : p->temp_elems) t += e;
   info->temp_elems = t;
   info->scratch_elems = p->scratch;
-  info->total_words = p->mgr->totalWords();
-  info->comm_waits = p->sched->getCommWaitCounts();
-  info->dag_density = p->dag->dag_density();
+  int64_t words = 0;
+  for (int w = 0; w < p->workers; ++w) words += (int64_t)ppc::program_stream(p, w).size();
+  info->total_words = words;
+  info->comm_waits = p->dag ? p->sched->getCommWaitCounts() : im->comm_waits;
+  info->dag_density = p->dag ? p->dag->dag_density() : im->dag_density;
   info->schedule_ms = p->schedule_ms;
   info->dag_ms = p->dag_ms;
   info->encode_ms = p->encode_ms;
@@ -424,12 +433,12 @@ int64_t pp_program_temp_elems(const pp_program* p, int member) {
 
 int pp_program_worker_length(const pp_program* p, int worker) {
   if (p == nullptr || worker < 0 || worker >= p->workers) return set_error(PP_ERR_INVALID, "bad worker");
-  return (int)p->mgr->getStream(worker).size();
+  return (int)ppc::program_stream(p, worker).size();
 }
 
 int pp_program_worker_words(const pp_program* p, int worker, int32_t* out, int cap) {
   if (p == nullptr || worker < 0 || worker >= p->workers) return set_error(PP_ERR_INVALID, "bad worker");
-  const auto& s = p->mgr->getStream(worker);
+  const auto& s = ppc::program_stream(p, worker);
   const int n = (int)s.size();
   if (out != nullptr) std::memcpy(out, s.data(), sizeof(int32_t) * (size_t)std::min(n, cap));
   return n;
@@ -452,7 +461,16 @@ int64_t pp_program_inst_flops(const pp_program* p, int inst) {
 }
 
 int pp_program_inst_preds(const pp_program* p, int inst, int32_t* preds, int32_t* masks, int cap) {
-  if (p == nullptr || inst < 0 || inst >= p->dag->getLength()) return set_error(PP_ERR_INVALID, "bad instruction");
+  if (p == nullptr || inst < 0 || inst >= ppc::program_insts(p)) return set_error(PP_ERR_INVALID, "bad instruction");
+  if (!p->dag) {
+    const auto& v = p->image->preds[inst];
+    const int n = (int)v.size() / 2;
+    for (int i = 0; i < n && i < cap; ++i) {
+      if (preds) preds[i] = v[2 * i];
+      if (masks) masks[i] = v[2 * i + 1];
+    }
+    return n;
+  }
   int n = 0;
   for (const dependency_linked_list* d = p->dag->getDepList_To(inst); d != nullptr; d = d->getNext(), ++n) {
     if (n < cap) {
@@ -465,12 +483,20 @@ int pp_program_inst_preds(const pp_program* p, int inst, int32_t* preds, int32_t
 
 int pp_program_schedule_length(const pp_program* p, int worker) {
   if (p == nullptr || worker < 0 || worker >= p->workers) return set_error(PP_ERR_INVALID, "bad worker");
-  return p->sched->getLength(worker);
+  return p->dag ? p->sched->getLength(worker) : (int)p->image->sched[worker].size();
 }
 
 int pp_program_schedule_entries(const pp_program* p, int worker, int32_t* insts, int32_t* ex_w, int64_t* start_flops) {
   if (p == nullptr) return set_error(PP_ERR_INVALID, "null program");
-  return fill_entries(p->sched.get(), worker, insts, ex_w, start_flops);
+  if (p->dag) return fill_entries(p->sched.get(), worker, insts, ex_w, start_flops);
+  if (worker < 0 || worker >= p->workers) return set_error(PP_ERR_INVALID, "bad worker");
+  const auto& v = p->image->sched[worker];
+  for (size_t i = 0; i < v.size(); ++i) {
+    if (insts) insts[i] = v[i].inst;
+    if (ex_w) ex_w[i] = v[i].ex_w;
+    if (start_flops) start_flops[i] = v[i].start_flops;
+  }
+  return (int)v.size();
 }
 
 }  // extern "C"

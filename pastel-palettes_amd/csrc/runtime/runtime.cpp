@@ -284,12 +284,12 @@ int pp_stream_upload_program(pp_ctx* ctx, const pp_program* prog, const void* co
     auto s = new pp_stream();
     s->ctx = ctx;
     s->workers = prog->workers;
-    s->n_insts = prog->dag->getLength();
+    s->n_insts = ppc::program_insts(prog);
     s->precision = precision;
     s->n_ptrs = n_ptrs;
     for (int i = 0; i < n_ptrs; ++i) s->ptrs.push_back(const_cast<void*>(ptr_table[i]));
     std::vector<const std::vector<int32_t>*> refs;
-    for (int w = 0; w < prog->workers; ++w) refs.push_back(&prog->mgr->getStream(w));
+    for (int w = 0; w < prog->workers; ++w) refs.push_back(&ppc::program_stream(prog, w));
     const int eb = elem_bytes(precision);
     int rc = stream_common(ctx, s, refs, s->n_insts, (uint64_t)prog->scratch * eb, sketch_need(prog));
     // temp slots the caller left empty are owned (and zeroed per launch) here

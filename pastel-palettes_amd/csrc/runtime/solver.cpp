@@ -25,6 +25,7 @@
 #include <string>
 #include <vector>
 
+#include "../io/wire.h"
 #include "../matrix/Hierarchical.h"
 #include "runtime_internal.h"
 
@@ -337,7 +338,7 @@ int solver_create(pp_ctx* ctx, const pp_hmatrix* const* hs, int count, const dou
                                        precision, &s->fstream)))
       break;
     // H-LU FLOPs only: the inversions are part of the solve's set-up
-    s->factor_flops = s->fprog->primary_flops >= 0 ? (double)s->fprog->primary_flops : (double)s->fprog->dag->getFlops();
+    s->factor_flops = s->fprog->primary_flops >= 0 ? (double)s->fprog->primary_flops : (double)s->fprog->tree.getFlops();
     s->fstream->role = 1;
     s->member_res.assign(count, 0.);
   } while (false);
@@ -611,6 +612,101 @@ int pp_solver_factor_arena(pp_solver* s, void** dptr, int* eb) {
   *dptr = s->d_F;
   *eb = elem_bytes(s->precision);
   return PP_OK;
+}
+
+// FACT payload: i32 precision | i32 solve_mode | i32 members | i32
+// elem_bytes | i64 storage_total | i64 n_total | per member: u64 structure
+// fingerprint | i64 arena_off | i64 row_off | u64 FNV-1a 64 of the FP64
+// arenas | i64 elements | factor arenas (elements x elem_bytes).  The FP64
+// arenas stay with the solver that loads the string: their checksum ties the
+// factors to the matrices they were computed from.
+namespace {
+
+int fp64_arena_sum(pp_solver* s, uint64_t* sum) {
+  std::vector<double> a((size_t)s->storage_total);
+  HIP_TRY(hipDeviceSynchronize());
+  HIP_TRY(hipMemcpy(a.data(), s->d_A, 8 * a.size(), hipMemcpyDeviceToHost));
+  *sum = ppw::fnv1a64(a.data(), 8 * a.size());
+  return PP_OK;
+}
+
+void put_factor_header(ppw::writer& w, pp_solver* s, uint64_t arena_sum) {
+  w.put<int32_t>(s->precision);
+  w.put<int32_t>(s->solve_mode);
+  w.put<int32_t>((int32_t)s->hms.size());
+  w.put<int32_t>(elem_bytes(s->precision));
+  w.put<int64_t>(s->storage_total);
+  w.put<int64_t>(s->n_total);
+  for (size_t m = 0; m < s->hms.size(); ++m) {
+    w.put<uint64_t>(ppc::tree_fingerprint(s->hms[m]));
+    w.put<int64_t>(s->arena_off[m]);
+    w.put<int64_t>(s->row_off[m]);
+  }
+  w.put<uint64_t>(arena_sum);
+}
+
+}  // namespace
+
+int64_t pp_solver_encode_factors(pp_solver* s, uint8_t* buf, uint64_t cap) {
+  try {
+    if (s == nullptr) return set_error(PP_ERR_INVALID, "null solver");
+    const uint64_t eb = (uint64_t)elem_bytes(s->precision);
+    ppw::writer head;
+    put_factor_header(head, s, 0);
+    const uint64_t payload = head.bytes.size() + 8 + eb * (uint64_t)s->storage_total;
+    const uint64_t total = 16 + 16 + (payload + 7) / 8 * 8 + 8;
+    if (buf == nullptr || cap < total) return (int64_t)total;
+    uint64_t sum = 0;
+    int rc = fp64_arena_sum(s, &sum);
+    if (rc) return rc;
+    ppw::writer w;
+    put_factor_header(w, s, sum);
+    w.put<int64_t>(s->storage_total);
+    const size_t at = w.bytes.size();
+    w.bytes.resize(at + eb * (size_t)s->storage_total);
+    HIP_TRY(hipMemcpy(w.bytes.data() + at, s->d_F, eb * (size_t)s->storage_total, hipMemcpyDeviceToHost));
+    ppw::container c;
+    c.add(ppw::kFactor, std::move(w.bytes));
+    if (c.size() != total) return set_error(PP_ERR_INTERNAL, "factor string size");
+    c.write(buf);
+    return (int64_t)total;
+  } catch (const ppw::wire_error& e) {
+    return set_error(PP_ERR_INVALID, e.what());
+  } catch (...) {
+    return from_exception();
+  }
+}
+
+int pp_solver_decode_factors(pp_solver* s, const uint8_t* buf, uint64_t len) {
+  try {
+    if (s == nullptr) return set_error(PP_ERR_INVALID, "null solver");
+    const auto secs = ppw::parse(buf, len);
+    const ppw::section* f = ppw::find(secs, ppw::kFactor);
+    if (f == nullptr) throw ppw::wire_error("no FACT section");
+    ppw::writer mine;
+    put_factor_header(mine, s, 0);
+    const size_t hb = mine.bytes.size();
+    ppw::reader r(f->data, f->length);
+    r.need(hb);
+    // everything up to the arena checksum must match this solver exactly
+    if (std::memcmp(r.here(), mine.bytes.data(), hb - 8) != 0)
+      throw ppw::wire_error("factors belong to another solver layout (precision, solve mode, members or structure)");
+    r.skip(hb - 8);
+    const uint64_t sum = r.get<uint64_t>();
+    const int64_t n = r.count(elem_bytes(s->precision));
+    if (n != s->storage_total || r.left() != (uint64_t)n * elem_bytes(s->precision))
+      throw ppw::wire_error("factor arena size");
+    uint64_t here = 0;
+    const int rc = fp64_arena_sum(s, &here);
+    if (rc) return rc;
+    if (sum != here) throw ppw::wire_error("factors were computed from other matrix entries");
+    HIP_TRY(hipMemcpy(s->d_F, r.here(), (size_t)n * elem_bytes(s->precision), hipMemcpyHostToDevice));
+    return PP_OK;
+  } catch (const ppw::wire_error& e) {
+    return set_error(PP_ERR_INVALID, e.what());
+  } catch (...) {
+    return from_exception();
+  }
 }
 
 int pp_solver_destroy(pp_solver* s) {

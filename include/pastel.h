@@ -277,6 +277,29 @@ double pp_solver_factor_flops(const pp_solver* s);
 double pp_solver_last_factor_ms(const pp_solver* s);
 double pp_solver_solve_flops(const pp_solver* s, int nrhs);
 
+/* ------------------------------------------------------------------ construction on the device
+ * Fills an H-matrix arena (device, FP64, pp_hmatrix_storage elements) from a
+ * kernel function of points (device, n x dim row-major FP64):
+ *   PP_KERNEL_EXP1D  A_ij = exp(-|x_i - x_j| / param)          (dim 1)
+ *   PP_KERNEL_INV2D  A_ij = 1 / (|p_i - p_j|_2 + param)        (dim 2)
+ * plus `shift` on the diagonal.  Dense leaves are evaluated entry by entry.
+ * A low-rank leaf of rank r gets a partially pivoted adaptive cross
+ * approximation of rank k = min(max(4r, 64), 256, rows, columns), truncated to
+ * rank r (shifted CholeskyQR3 of both factors, one-sided Jacobi SVD of the
+ * k x k core) when k > r; one workgroup per leaf.  `pivots` (device,
+ * optional, pp_hmatrix_build_pivots entries) receives the (row, column) pivot
+ * pairs of every cross approximation in leaf order, -1 for steps left empty.
+ * Synchronous on hip_stream (NULL = the context's stream).
+ * Limits: rank <= 256 and <= 65536 rows per low-rank leaf.  No counterpart in
+ * the reference, whose H-matrices come from an external compressor; its SVD /
+ * sampling routines (src/kernel.cu:1084-1372) are unused there. */
+#define PP_KERNEL_EXP1D 0
+#define PP_KERNEL_INV2D 1
+int pp_hmatrix_build_kernel(pp_ctx* ctx, const pp_hmatrix* h, int kind, const double* points, double param,
+                            double shift, double* arena, int32_t* pivots, void* hip_stream);
+/* Length of the pivot array of pp_hmatrix_build_kernel (2 x sum of ranks). */
+int64_t pp_hmatrix_build_pivots(const pp_hmatrix* h);
+
 /* ------------------------------------------------------------------ byte-string format
  * H-matrices, instruction programs and factorizations as byte strings, for
  * files, caches and the wire.  No counterpart in the reference, which has no

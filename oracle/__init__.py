@@ -58,6 +58,13 @@ def lib():
         L.or_dense_residual.argtypes = [dp, ip, dp, dp, ip, ip]
         L.or_dense_residual.restype = C.c_double
         L.or_sketch.argtypes = [dp, C.c_int64, C.c_uint64]
+        L.or_kernel_entry.argtypes = [ip, dp, ip, ip, C.c_double, C.c_double]
+        L.or_kernel_entry.restype = C.c_double
+        L.or_kernel_block.argtypes = [ip, dp, ip, ip, ip, ip, C.c_double, C.c_double, dp]
+        L.or_aca.argtypes = [ip, dp, ip, ip, ip, ip, C.c_double, C.c_double, ip, dp, dp, C.POINTER(C.c_int)]
+        L.or_aca_dense.argtypes = [dp, ip, ip, ip, dp, dp, C.POINTER(C.c_int)]
+        L.or_truncate.argtypes = [dp, dp, ip, ip, ip, ip, dp, dp]
+        L.or_truncate.restype = ip
         _lib = L
     return _lib
 
@@ -182,6 +189,77 @@ def run_streams(streams, slots, n_insts, scratch_elems, rnd=None, threads=1):
     rc = lib().or_run_streams(sp, W, pp, _d(scratch), stride, _d(rnd), n_insts, threads)
     if rc != 0:
         raise RuntimeError("oracle stream run failed (deadlock or bad entry)")
+
+
+def kernel_block(kind, pts, r0, ny, c0, nx, param, shift=0.0):
+    """ny x nx kernel entries at global (r0, c0) (kind 0: exp1d, 1: inv2d)."""
+    pts = np.ascontiguousarray(pts, np.float64)
+    out = np.zeros((ny, nx))
+    lib().or_kernel_block(kind, _d(pts), r0, ny, c0, nx, param, shift, _d(out))
+    return out
+
+
+def aca(kind, pts, r0, ny, c0, nx, param, shift, rank):
+    """(U, V, pivots) of the fixed-rank cross approximation of a kernel block."""
+    pts = np.ascontiguousarray(pts, np.float64)
+    u, v = np.zeros((ny, rank)), np.zeros((nx, rank))
+    piv = np.zeros(2 * rank, np.int32)
+    lib().or_aca(kind, _d(pts), r0, ny, c0, nx, param, shift, rank, _d(u), _d(v),
+                 piv.ctypes.data_as(C.POINTER(C.c_int)))
+    return u, v, piv.reshape(rank, 2)
+
+
+def aca_dense(a, rank):
+    """(U, V, pivots) of the fixed-rank cross approximation of a matrix."""
+    a = np.ascontiguousarray(a, np.float64)
+    ny, nx = a.shape
+    u, v = np.zeros((ny, rank)), np.zeros((nx, rank))
+    piv = np.zeros(2 * rank, np.int32)
+    lib().or_aca_dense(_d(a), ny, nx, rank, _d(u), _d(v), piv.ctypes.data_as(C.POINTER(C.c_int)))
+    return u, v, piv.reshape(rank, 2)
+
+
+def truncate(u, v, r):
+    """(Uo, Vo): rank-r truncation of U V^T (U ny x k, V nx x k) by the
+    device builder's algorithm (pitch padded to a multiple of 32)."""
+    ny, k = u.shape
+    nx = v.shape[0]
+    kp = (k + 31) // 32 * 32
+    up, vp = np.zeros((ny, kp)), np.zeros((nx, kp))
+    up[:, :k], vp[:, :k] = u, v
+    uo, vo = np.zeros((ny, r)), np.zeros((nx, r))
+    if lib().or_truncate(_d(up), _d(vp), ny, nx, kp, r, _d(uo), _d(vo)) != 0:
+        raise RuntimeError("truncation failed (Cholesky pivot)")
+    return uo, vo
+
+
+def aca_rank(ny, nx, r):
+    """Rank of the device builder's cross approximation of a low-rank leaf."""
+    k = min(max(4 * r, 64), 256, ny, nx)
+    return k if k > r else r
+
+
+def build_arena(leaves, storage, kind, pts, param, shift):
+    """(arena, pivots) an H-matrix structure gets from the device builder:
+    leaves as HMatrix.leaves() dicts; pivots ((row, column) per step of each
+    low-rank leaf's cross approximation) in leaf order."""
+    arena = np.zeros(storage)
+    pivs = []
+    for lf in leaves:
+        ny, nx = lf["ny"], lf["nx"]
+        if lf["type"] == "dense":
+            arena[lf["off"]:lf["off"] + ny * nx] = kernel_block(kind, pts, lf["row0"], ny, lf["col0"], nx, param,
+                                                                shift).ravel()
+        else:
+            r = lf["rank"]
+            k = aca_rank(ny, nx, r)
+            u, v, piv = aca(kind, pts, lf["row0"], ny, lf["col0"], nx, param, shift, k)
+            if k > r:
+                u, v = truncate(u, v, r)
+            arena[lf["off"]:lf["off"] + ny * r] = u.ravel()
+            arena[lf["off_v"]:lf["off_v"] + nx * r] = v.ravel()
+            pivs.append(piv)
+    return arena, (np.concatenate(pivs) if pivs else np.zeros((0, 2), np.int32))
 
 
 def dense_lu(a, threads=8):

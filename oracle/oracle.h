@@ -67,6 +67,24 @@ double or_dense_residual(const double* A, int n, const double* X, const double* 
 /* splitmix64 sketch values in [-1, 1) (same stream as the library's). */
 void or_sketch(double* out, int64_t count, uint64_t seed);
 
+/* Kernel entry (kind 0: exp(-|x_i - x_j| / param), 1-D points; kind 1:
+ * 1 / (|p_i - p_j| + param), 2-D points) + shift where i == j. */
+double or_kernel_entry(int kind, const double* pts, int i, int j, double param, double shift);
+/* ny x nx block of entries at global (r0, c0), row-major. */
+void or_kernel_block(int kind, const double* pts, int r0, int ny, int c0, int nx, double param, double shift,
+                     double* out);
+/* Fixed-rank partially pivoted cross approximation A ~ U V^T of the kernel
+ * block (U ny x r, V nx x r, row-major); piv (optional) gets 2r pivot
+ * indices (row, column), -1 for columns left zero. */
+void or_aca(int kind, const double* pts, int r0, int ny, int c0, int nx, double param, double shift, int r, double* U,
+            double* V, int* piv);
+/* The same on an explicit ny x nx row-major matrix. */
+void or_aca_dense(const double* A, int ny, int nx, int r, double* U, double* V, int* piv);
+/* Truncation of a rank-k cross approximation (U ny x kp, V nx x kp, pitch
+ * kp, columns >= k zero; both overwritten) to rank r into Uo (ny x r) and Vo
+ * (nx x r) by the device builder's algorithm; 1 when a Cholesky pivot fails. */
+int or_truncate(double* U, double* V, int ny, int nx, int kp, int r, double* Uo, double* Vo);
+
 #ifdef __cplusplus
 }
 #endif

@@ -16,6 +16,7 @@ LIB_PATH = os.path.join(HERE, "libpastel.so")
 
 FP64, FP32, MIXED = 0, 1, 2
 SOLVE_HINV, SOLVE_LEAFINV, SOLVE_SUBST = 0, 1, 2  # solver modes (pp_solver_create_batch_mode)
+KERNEL_EXP1D, KERNEL_INV2D = 0, 1  # kernels of the device builder (pp_hmatrix_build_kernel)
 PRECISIONS = {"fp64": FP64, "fp32": FP32, "mixed": MIXED}
 
 _lib = None
@@ -120,6 +121,8 @@ def lib():
         "pp_solver_member": (C.c_int, [vp, C.c_int, P(i64), P(C.c_int), P(dp)]),
         "pp_solver_use_graph": (C.c_int, [vp, C.c_int]),
         "pp_solver_last_factor_ms": (dp, [vp]),
+        "pp_hmatrix_build_kernel": (C.c_int, [vp, vp, C.c_int, vp, dp, dp, vp, vp, vp]),
+        "pp_hmatrix_build_pivots": (i64, [vp]),
         "pp_hmatrix_encode": (i64, [vp, P(dp), vp, u64]),
         "pp_hmatrix_decode": (C.c_int, [vp, u64, P(dp), u64, P(C.c_int), P(vp)]),
         "pp_program_encode": (i64, [vp, vp, u64]),
@@ -240,6 +243,20 @@ class HMatrix:
         else:
             raise ValueError(kind)
         self.h, self.kind, self.n, self.params = h, kind, n, kw
+
+    def build_on_device(self, ctx, kind, points_ptr, param, shift, arena_ptr, pivots_ptr=None, hip_stream=None):
+        """Fills the device arena at arena_ptr (storage FP64 elements) from the
+        kernel `kind` (KERNEL_EXP1D / KERNEL_INV2D) of the device points at
+        points_ptr: dense leaves evaluated, low-rank leaves by fixed-rank
+        cross approximation.  pivots_ptr: optional device int32 array of
+        build_pivots entries."""
+        check(lib().pp_hmatrix_build_kernel(ctx.h, self.h, kind, C.c_void_p(int(points_ptr)), param, shift,
+                                            C.c_void_p(int(arena_ptr)),
+                                            C.c_void_p(int(pivots_ptr)) if pivots_ptr else None, hip_stream))
+
+    @property
+    def build_pivots(self):
+        return int(lib().pp_hmatrix_build_pivots(self.h))
 
     def to_bytes(self, arena=None):
         """Byte string of the structure and, when given, the FP64 arena."""

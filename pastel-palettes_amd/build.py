@@ -27,8 +27,9 @@ HOST_SOURCES = [
     "runtime/runtime.cpp",
     "runtime/solver.cpp",
     "io/wire.cpp",
+    "runtime/build.cpp",
 ]
-HIP_SOURCES = ["kernels/vm.hip"]
+HIP_SOURCES = ["kernels/vm.hip", "kernels/build.hip"]
 
 CXXFLAGS = ["-O2", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-function", f"-I{ROCM}/include",
             "-D__HIP_PLATFORM_AMD__"]

@@ -29,6 +29,34 @@ struct MatvecPlanDev {
   int n_tiles, n_lowrank;
 };
 
+// One leaf of an H-matrix to construct.  Dense: entries at `off` (ld nx);
+// low rank: U (ny x rank) at `off`, V (nx x rank) at `off_v`, ld rank, and
+// its 2 * rank pivot indices at `aux` of the pivot array.
+struct BuildLeaf {
+  int32_t row0, col0, ny, nx, rank, ld;  // ld: row pitch of U and V (>= rank)
+  int64_t off, off_v, aux;
+};
+// A cross approximation of rank k (pitch kp, a multiple of 32) in scratch,
+// truncated to rank r into the arena: U (ny x r) at out_u, V (nx x r) at
+// out_v.  work: 6 kp^2 + 2 kp scratch elements; status: 0, or 1 when a
+// Cholesky pivot failed.
+struct RecompressLeaf {
+  int32_t ny, nx, k, kp, r, status;
+  int64_t u, v, work, out_u, out_v;
+};
+constexpr int kBuildMaxRank = 256;
+constexpr int kBuildMaxRows = 65536;
+
+// Fills an arena from kernel entries (kind 0: exp(-|x_i - x_j| / param) on
+// 1-D points, 1: 1 / (|p_i - p_j| + param) on 2-D points, + shift on the
+// global diagonal): dense leaves entry by entry, low-rank leaves by
+// fixed-rank partially pivoted cross approximation.
+hipError_t launch_build(int kind, const BuildLeaf* dense, int n_dense, int max_dense_elems, const BuildLeaf* lowrank,
+                        int n_lowrank, const double* pts, double param, double shift, double* arena,
+                        int32_t* pivots, hipStream_t s);
+
+hipError_t launch_recompress(RecompressLeaf* leaves, int n, double* scratch, double* arena, hipStream_t s);
+
 // factor: launch under the factorization symbol (vm_factor_kernel)
 hipError_t launch_vm(const VmArgs& a, int precision, int workers, int threads, bool factor, uint32_t* epoch_dev,
                      int32_t* status, hipStream_t s);

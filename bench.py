@@ -165,9 +165,10 @@ def main():
     ap.add_argument("--only", choices=["fp64", "mixed", "both"], default="both")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run only a plain 256 MiB device copy (PMC byte-counter calibration)")
-    ap.add_argument("--workload", choices=["hlu", "batch", "serve"], default="hlu",
+    ap.add_argument("--workload", choices=["hlu", "batch", "serve", "build"], default="hlu",
                     help="hlu: the N=16384 factor+solve step (default); batch: many matrices in one launch "
-                         "with autotuned leaves; serve: Poisson request stream")
+                         "with autotuned leaves; serve: Poisson request stream; build: H-matrix construction "
+                         "on the device")
     ap.add_argument("--batch-count", type=int, default=512)
     ap.add_argument("--serve-seconds", type=float, default=20.0)
     args = ap.parse_args()
@@ -193,6 +194,9 @@ def main():
         if args.workload == "batch":
             out = wl.run_batch(pp, torch, ctx, count=args.batch_count, steps=args.steps, warmup=args.warmup,
                                log=log)
+        elif args.workload == "build":
+            out = wl.run_build(pp, torch, ctx, n=args.n, levels=args.levels, rank=args.rank, steps=args.steps,
+                               warmup=args.warmup, log=log)
         else:
             out = wl.run_serve(pp, torch, ctx, duration=args.serve_seconds, log=log)
         print(json.dumps(out), flush=True)

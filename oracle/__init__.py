@@ -235,7 +235,7 @@ def truncate(u, v, r):
 
 def aca_rank(ny, nx, r):
     """Rank of the device builder's cross approximation of a low-rank leaf."""
-    k = min(max(4 * r, 64), 256, ny, nx)
+    k = min(max(4 * r, 64, min(ny, nx) // 16), 256, ny, nx)
     return k if k > r else r
 
 

@@ -521,7 +521,12 @@ static void aca_run(const aca_src* s, int ny, int nx, int r, double* U, double* 
       }
     }
     used[i] = 1;
-    if (best == 0.0) { /* row represented exactly: smallest unused row */
+    if (best == 0.0) { /* row represented exactly: the step stays empty (zero
+                          column, pivots -1) and the next starts at the
+                          smallest unused row */
+      for (int t = 0; t < ny; ++t) U[(int64_t)t * r + k] = 0.0;
+      if (piv) piv[2 * k] = piv[2 * k + 1] = -1;
+      ++k;
       i = -1;
       for (int t = 0; t < ny; ++t)
         if (!used[t]) {
@@ -576,13 +581,38 @@ void or_aca_dense(const double* A, int ny, int nx, int r, double* U, double* V, 
  * pitch kp, columns >= k zero; both overwritten) to rank r (Uo: ny x r,
  * Vo: nx x r), as the device builder does it: unit columns, shifted
  * CholeskyQR3 of each factor, one-sided Jacobi SVD of the kp x kp core with
- * cyclic round-robin pairs, the r columns of largest norm.  Returns 1 when a
- * Cholesky pivot is not positive, else 0. */
-static int trunc_orth(double* X, int m, int kp, double* d, double* G, double* Rt, double* T) {
-  for (int c = 0; c < kp; ++c) {
+ * cyclic round-robin pairs, the r columns of largest norm.  Column norms and
+ * Gram matrices are sums over 256-row slices (each slice summed first, the
+ * slices then added in order); X <- X R^{-1} through R^{-1} formed column by
+ * column; the Jacobi dot products are 64 lane sums combined by an xor
+ * butterfly.  Returns 1 when a Cholesky pivot is not positive, else 0. */
+#define OR_SLICE 256 /* rows per partial sum of the device builder */
+
+/* sum of 64 lane partials (lane l: elements l, l + 64, ... in order) by an
+ * xor butterfly, as a 64-wide wavefront reduces them */
+static double lane_dot(const double* x, const double* y, int n) {
+  double p[64], q[64];
+  for (int l = 0; l < 64; ++l) {
     double acc = 0.0;
-    for (int t = 0; t < m; ++t) acc = fma(X[(int64_t)t * kp + c], X[(int64_t)t * kp + c], acc);
-    d[c] = sqrt(acc);
+    for (int i = l; i < n; i += 64) acc = fma(x[i], y[i], acc);
+    p[l] = acc;
+  }
+  for (int o = 32; o >= 1; o >>= 1) {
+    for (int l = 0; l < 64; ++l) q[l] = p[l] + p[l ^ o];
+    memcpy(p, q, sizeof(p));
+  }
+  return p[0];
+}
+
+static int trunc_orth(double* X, int m, int kp, double* d, double* G, double* Rt, double* T, double* Ri) {
+  for (int c = 0; c < kp; ++c) {
+    double tot = 0.0;
+    for (int s0 = 0; s0 < m; s0 += OR_SLICE) {
+      double acc = 0.0;
+      for (int t = s0; t < m && t < s0 + OR_SLICE; ++t) acc = fma(X[(int64_t)t * kp + c], X[(int64_t)t * kp + c], acc);
+      tot = tot + acc;
+    }
+    d[c] = sqrt(tot);
   }
   for (int t = 0; t < m; ++t)
     for (int c = 0; c < kp; ++c)
@@ -591,9 +621,14 @@ static int trunc_orth(double* X, int m, int kp, double* d, double* G, double* Rt
   for (int pass = 0; pass < 3; ++pass) {
     for (int a = 0; a < kp; ++a)
       for (int b = 0; b < kp; ++b) {
-        double acc = 0.0;
-        for (int t = 0; t < m; ++t) acc = fma(X[(int64_t)t * kp + a], X[(int64_t)t * kp + b], acc);
-        G[a * kp + b] = acc;
+        double tot = 0.0;
+        for (int s0 = 0; s0 < m; s0 += OR_SLICE) {
+          double acc = 0.0;
+          for (int t = s0; t < m && t < s0 + OR_SLICE; ++t)
+            acc = fma(X[(int64_t)t * kp + a], X[(int64_t)t * kp + b], acc);
+          tot = tot + acc;
+        }
+        G[a * kp + b] = tot;
       }
     double tr = 0.0;
     for (int c = 0; c < kp; ++c) tr = tr + G[c * kp + c];
@@ -607,14 +642,27 @@ static int trunc_orth(double* X, int m, int kp, double* d, double* G, double* Rt
       for (int a = j + 1; a < kp; ++a)
         for (int c = a; c < kp; ++c) G[a * kp + c] = fma(-G[j * kp + a], G[j * kp + c], G[a * kp + c]);
     }
-    for (int t = 0; t < m; ++t) {
-      double* x = X + (int64_t)t * kp;
-      for (int c = 0; c < kp; ++c) {
-        double v = x[c];
-        for (int a = 0; a < c; ++a) v = fma(-x[a], G[a * kp + c], v);
-        x[c] = v / G[c * kp + c];
+    /* R^{-1} column by column, then X <- X R^{-1} (sums over a <= c) */
+    for (int c = 0; c < kp; ++c) {
+      for (int a = c + 1; a < kp; ++a) Ri[a * kp + c] = 0.0;
+      Ri[c * kp + c] = 1.0 / G[c * kp + c];
+      for (int a = c - 1; a >= 0; --a) {
+        double v = 0.0;
+        for (int b = a + 1; b <= c; ++b) v = fma(G[a * kp + b], Ri[b * kp + c], v);
+        Ri[a * kp + c] = -v / G[a * kp + a];
       }
     }
+    double* row = (double*)malloc(sizeof(double) * kp);
+    for (int t = 0; t < m; ++t) {
+      double* x = X + (int64_t)t * kp;
+      memcpy(row, x, sizeof(double) * kp);
+      for (int c = 0; c < kp; ++c) {
+        double v = 0.0;
+        for (int a = 0; a <= c; ++a) v = fma(row[a], Ri[a * kp + c], v);
+        x[c] = v;
+      }
+    }
+    free(row);
     for (int a = 0; a < kp; ++a)
       for (int c = 0; c < kp; ++c) {
         double v = 0.0;
@@ -628,10 +676,10 @@ static int trunc_orth(double* X, int m, int kp, double* d, double* G, double* Rt
 
 int or_truncate(double* U, double* V, int ny, int nx, int kp, int r, double* Uo, double* Vo) {
   const size_t q = (size_t)kp * kp;
-  double* w = (double*)calloc(6 * q + 2 * (size_t)kp, sizeof(double));
-  double *Ru = w, *Rv = w + q, *G = w + 2 * q, *T = w + 3 * q, *N = w + 4 * q, *J = w + 5 * q, *du = w + 6 * q,
-         *dv = du + kp;
-  if (trunc_orth(U, ny, kp, du, G, Ru, T) || trunc_orth(V, nx, kp, dv, G, Rv, T)) {
+  double* w = (double*)calloc(7 * q + 2 * (size_t)kp, sizeof(double));
+  double *Ru = w, *Rv = w + q, *G = w + 2 * q, *T = w + 3 * q, *N = w + 4 * q, *J = w + 5 * q, *Ri = w + 6 * q,
+         *du = w + 7 * q, *dv = du + kp;
+  if (trunc_orth(U, ny, kp, du, G, Ru, T, Ri) || trunc_orth(V, nx, kp, dv, G, Rv, T, Ri)) {
     free(w);
     return 1;
   }
@@ -652,12 +700,7 @@ int or_truncate(double* U, double* V, int ny, int nx, int kp, int r, double* Uo,
         const int p = p0 < p1 ? p0 : p1, qq = p0 < p1 ? p1 : p0;
         double* mp = N + (size_t)p * kp;
         double* mq = N + (size_t)qq * kp;
-        double al = 0.0, be = 0.0, ga = 0.0;
-        for (int i = 0; i < kp; ++i) {
-          al = fma(mp[i], mp[i], al);
-          be = fma(mq[i], mq[i], be);
-          ga = fma(mp[i], mq[i], ga);
-        }
+        const double al = lane_dot(mp, mp, kp), be = lane_dot(mq, mq, kp), ga = lane_dot(mp, mq, kp);
         if (ga != 0.0 && fabs(ga) > 1e-15 * sqrt(al * be)) {
           const double zeta = (be - al) / (2.0 * ga);
           const double t = (zeta >= 0.0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));

@@ -29,33 +29,72 @@ struct MatvecPlanDev {
   int n_tiles, n_lowrank;
 };
 
-// One leaf of an H-matrix to construct.  Dense: entries at `off` (ld nx);
-// low rank: U (ny x rank) at `off`, V (nx x rank) at `off_v`, ld rank, and
-// its 2 * rank pivot indices at `aux` of the pivot array.
-struct BuildLeaf {
-  int32_t row0, col0, ny, nx, rank, ld;  // ld: row pitch of U and V (>= rank)
-  int64_t off, off_v, aux;
-};
-// A cross approximation of rank k (pitch kp, a multiple of 32) in scratch,
-// truncated to rank r into the arena: U (ny x r) at out_u, V (nx x r) at
-// out_v.  work: 6 kp^2 + 2 kp scratch elements; status: 0, or 1 when a
-// Cholesky pivot failed.
-struct RecompressLeaf {
-  int32_t ny, nx, k, kp, r, status;
-  int64_t u, v, work, out_u, out_v;
-};
+// ---- H-matrix construction (build.hip)
 constexpr int kBuildMaxRank = 256;
 constexpr int kBuildMaxRows = 65536;
+constexpr int kBuildSlice = 256;  // rows / columns per workgroup
 
-// Fills an arena from kernel entries (kind 0: exp(-|x_i - x_j| / param) on
-// 1-D points, 1: 1 / (|p_i - p_j| + param) on 2-D points, + shift on the
-// global diagonal): dense leaves entry by entry, low-rank leaves by
-// fixed-rank partially pivoted cross approximation.
-hipError_t launch_build(int kind, const BuildLeaf* dense, int n_dense, int max_dense_elems, const BuildLeaf* lowrank,
-                        int n_lowrank, const double* pts, double param, double shift, double* arena,
-                        int32_t* pivots, hipStream_t s);
+// Kernel function: kind 0: exp(-|x_i - x_j| / param) on 1-D points, 1:
+// 1 / (|p_i - p_j| + param) on 2-D points, + shift on the global diagonal.
+struct BuildArgs {
+  const double* pts;
+  double param, shift;
+};
+// A dense leaf: entries at `off` of the arena (ld nx).
+struct BuildLeaf {
+  int32_t row0, col0, ny, nx, rank, ld;
+  int64_t off, off_v, aux;
+};
+// A cross approximation of rank k into U (ny x ld) and V (nx x ld).
+struct AcaLeaf {
+  int32_t row0, col0, ny, nx, k, ld;
+  double* U;
+  double* V;
+  int32_t* piv;    // 2k pivot indices or null
+  uint32_t* used;  // ceil(ny / 32) words, zero at the start
+  int32_t col_part, ncol, row_part, nrow;  // its slice partials
+};
+struct AcaState {
+  int32_t i, jp, zero, next;  // pivot row (-1: done), pivot column, empty step, row after an empty step
+  double d;                   // pivot value
+};
+struct BuildSlice {
+  int32_t leaf, first, count, part;
+};
+struct BuildPartial {
+  double m, v;
+  int32_t idx, pad;
+};
+// Truncation of a rank-k approximation (pitch kp) to rank r.
+struct TruncFactor {  // U_k or V_k of one leaf
+  double *G, *Rt, *T, *d, *Ri;
+  double* parts;  // column-norm partials, slice q at parts + q * part_stride
+  double* gparts;  // Gram partials, slice q at gparts + q * gpart_stride
+  int64_t part_stride, gpart_stride;
+  int32_t kp, nslices, m, status;
+};
+struct TruncSlice {
+  double* X;  // the factor (m x kp)
+  double* part;
+  double* gpart;
+  const double* d;
+  int32_t first, count, kp, factor;  // factor index 2 leaf + (0: U, 1: V)
+};
+struct TruncLeaf {
+  double *N, *J, *out_u, *out_v;
+  int32_t* sel;
+  int32_t kp, r, status, rotated;
+};
 
-hipError_t launch_recompress(RecompressLeaf* leaves, int n, double* scratch, double* arena, hipStream_t s);
+hipError_t launch_dense_fill(int kind, const BuildLeaf* dense, int n_dense, int max_elems, const BuildArgs& a,
+                             double* arena, hipStream_t s);
+// max_k steps of five launches each over all leaves
+hipError_t launch_aca(int kind, const AcaLeaf* leaves, AcaState* st, int n_leaves, int max_k,
+                      const BuildSlice* col_slices, int n_col, const BuildSlice* row_slices, int n_row,
+                      BuildPartial* part, const BuildArgs& a, hipStream_t s);
+// synchronises s once per Jacobi sweep (flag: one device int)
+hipError_t launch_truncate(TruncLeaf* leaves, int n_leaves, int max_kp, TruncFactor* factors,
+                           const TruncSlice* slices, int n_slices, int* flag, hipStream_t s);
 
 // factor: launch under the factorization symbol (vm_factor_kernel)
 hipError_t launch_vm(const VmArgs& a, int precision, int workers, int threads, bool factor, uint32_t* epoch_dev,

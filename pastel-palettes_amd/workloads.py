@@ -202,3 +202,69 @@ def run_serve(pp, torch, ctx, keys=12, duration=20.0, load=0.8, burst=48, slots=
         "max_residual": s["max_residual"], "converged": s["converged"],
         "per_size": per_n, "evictions": server.evictions, "build_s": build_s,
     }
+
+
+def _hmatrix_rows(hm, arena, rows, n):
+    """Rows `rows` of the matrix an arena represents (dense n-vectors)."""
+    out = np.zeros((len(rows), n))
+    pos = {r: i for i, r in enumerate(rows)}
+    for lf in hm.leaves():
+        r0, c0, ny, nx = lf["row0"], lf["col0"], lf["ny"], lf["nx"]
+        hit = [r for r in rows if r0 <= r < r0 + ny]
+        for r in hit:
+            t = r - r0
+            if lf["type"] == "dense":
+                out[pos[r], c0:c0 + nx] = arena[lf["off"] + t * nx:lf["off"] + (t + 1) * nx]
+            else:
+                k = lf["rank"]
+                u = arena[lf["off"] + t * k:lf["off"] + (t + 1) * k]
+                v = arena[lf["off_v"]:lf["off_v"] + nx * k].reshape(nx, k)
+                out[pos[r], c0:c0 + nx] = v @ u
+    return out
+
+
+def run_build(pp, torch, ctx, n=16384, levels=6, rank=32, steps=5, warmup=1, sample_rows=64, log=print):
+    """H-matrix construction on the device (2-D 1/(r + 1e-3) kernel of the
+    benchmark configuration) against the host builder (randomised SVD with
+    multithreaded BLAS); compression error on sampled rows of the kernel."""
+    hm = pp.HMatrix("hodlr", n, levels=levels, rank=rank)
+    kern = pp.config_kernel(1, n)
+    pts = torch.from_numpy(np.ascontiguousarray(kern.pts, np.float64)).cuda()
+    arena = torch.zeros(hm.storage, dtype=torch.float64, device="cuda")
+
+    def build():
+        hm.build_on_device(ctx, pp.KERNEL_INV2D, pts.data_ptr(), 1e-3, kern.shift, arena.data_ptr())
+
+    for _ in range(warmup):
+        build()
+    torch.cuda.synchronize()
+    times = []
+    for _ in range(steps):
+        t0 = time.perf_counter()
+        build()
+        times.append(1e3 * (time.perf_counter() - t0))
+    times.sort()
+    log(f"[build] device: {times}")
+    t0 = time.perf_counter()
+    host = pp.fill_arena(hm, kern)
+    host_s = time.perf_counter() - t0
+    log(f"[build] host randomised SVD: {host_s:.2f} s")
+    dev = arena.cpu().numpy()
+    rng = np.random.default_rng(11)
+    rows = sorted(rng.choice(n, sample_rows, replace=False).tolist())
+    exact = np.stack([kern.block(r, 1, 0, n)[0] for r in rows])
+    err = lambda a: float(np.linalg.norm(exact - _hmatrix_rows(hm, a, rows, n)) / np.linalg.norm(exact))
+    stored = sum(l["ny"] * l["nx"] if l["type"] == "dense" else (l["ny"] + l["nx"]) * l["rank"] for l in hm.leaves())
+    med = times[len(times) // 2]
+    return {
+        "workload": f"device H-matrix construction: hodlr n={n} levels={levels} rank={rank}, 2-D 1/(r+1e-3) "
+                    f"kernel (Morton points, shift n/4); dense leaves evaluated, low-rank leaves by cross "
+                    f"approximation of rank max(4r,64) truncated to r",
+        "ms_per_build": med, "builds_per_s": 1e3 / med, "ms_min": times[0], "ms_max": times[-1],
+        "stored_entries": int(stored), "stored_gb_per_s": stored * 8 / (med * 1e-3) / 1e9,
+        "host_build_s": host_s, "host_builder": "numpy randomised SVD (oversampling 10, one power iteration), "
+                                                "multithreaded BLAS",
+        "speedup_vs_host": host_s * 1e3 / med,
+        "sampled_rows": sample_rows,
+        "compression_error_device": err(dev), "compression_error_host": err(host),
+    }

@@ -284,12 +284,14 @@ double pp_solver_solve_flops(const pp_solver* s, int nrhs);
  *   PP_KERNEL_INV2D  A_ij = 1 / (|p_i - p_j|_2 + param)        (dim 2)
  * plus `shift` on the diagonal.  Dense leaves are evaluated entry by entry.
  * A low-rank leaf of rank r gets a partially pivoted adaptive cross
- * approximation of rank k = min(max(4r, 64), 256, rows, columns), truncated to
- * rank r (shifted CholeskyQR3 of both factors, one-sided Jacobi SVD of the
- * k x k core) when k > r; one workgroup per leaf.  `pivots` (device,
+ * approximation of rank k = min(max(4r, 64, min(rows, columns) / 16), 256,
+ * rows, columns), truncated to rank r (shifted CholeskyQR3 of both factors,
+ * one-sided Jacobi SVD of the k x k core) when k > r; every leaf is spread over
+ * 256-row slices, one workgroup each.  `pivots` (device,
  * optional, pp_hmatrix_build_pivots entries) receives the (row, column) pivot
  * pairs of every cross approximation in leaf order, -1 for steps left empty.
- * Synchronous on hip_stream (NULL = the context's stream).
+ * Synchronous on hip_stream (NULL = the context's stream); allocates its
+ * scratch through the context.
  * Limits: rank <= 256 and <= 65536 rows per low-rank leaf.  No counterpart in
  * the reference, whose H-matrices come from an external compressor; its SVD /
  * sampling routines (src/kernel.cu:1084-1372) are unused there. */

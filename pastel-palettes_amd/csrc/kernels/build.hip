@@ -312,7 +312,17 @@ __global__ void __launch_bounds__(256) tr_gram_kernel(const TruncSlice* __restri
   }
 }
 
-// per factor: G = sum of the slice Gram matrices, shift (first pass),
+// G = sum of the slice Gram matrices in slice order (grid: factor x 256 entries)
+__global__ void __launch_bounds__(256) tr_gsum_kernel(TruncFactor* fs) {
+  TruncFactor* F = fs + blockIdx.x;
+  const int kp = F->kp, e = blockIdx.y * 256 + threadIdx.x;
+  if (F->status || e >= kp * kp) return;
+  double tot = 0.0;
+  for (int q = 0; q < F->nslices; ++q) tot = tot + F->gparts[(int64_t)q * F->gpart_stride + e];
+  F->G[e] = tot;
+}
+
+// per factor (G summed by tr_gsum_kernel): shift (first pass),
 // Cholesky G = R^T R (R in the upper triangle of G), Rt <- R Rt
 __global__ void __launch_bounds__(256) tr_chol_kernel(TruncFactor* fs, int pass) {
   TruncFactor* F = fs + blockIdx.x;
@@ -320,12 +330,8 @@ __global__ void __launch_bounds__(256) tr_chol_kernel(TruncFactor* fs, int pass)
   double *G = F->G, *Rt = F->Rt, *T = F->T;
   const double* d = F->d;
   if (F->status) return;
-  for (int e = tid; e < kp * kp; e += 256) {
-    double tot = 0.0;
-    for (int q = 0; q < F->nslices; ++q) tot = tot + F->gparts[(int64_t)q * F->gpart_stride + e];
-    G[e] = tot;
-    if (pass == 0) Rt[e] = (e / kp == e % kp) ? 1.0 : 0.0;
-  }
+  if (pass == 0)
+    for (int e = tid; e < kp * kp; e += 256) Rt[e] = (e / kp == e % kp) ? 1.0 : 0.0;
   __syncthreads();
   if (tid == 0) {
     double tr = 0.0;
@@ -567,6 +573,7 @@ hipError_t launch_truncate(TruncLeaf* leaves, int n_leaves, int max_kp, TruncFac
   hipLaunchKernelGGL(tr_scale_kernel, dim3(n_slices), dim3(256), 0, s, slices);
   for (int pass = 0; pass < 3; ++pass) {
     hipLaunchKernelGGL(tr_gram_kernel, dim3(n_slices), dim3(256), 0, s, slices);
+    hipLaunchKernelGGL(tr_gsum_kernel, dim3(2 * n_leaves, (max_kp * max_kp + 255) / 256), dim3(256), 0, s, factors);
     hipLaunchKernelGGL(tr_chol_kernel, dim3(2 * n_leaves), dim3(256), 0, s, factors, pass);
     hipLaunchKernelGGL(tr_trsm_kernel, dim3(n_slices), dim3(256), 0, s, slices, factors);
   }

@@ -94,6 +94,18 @@ int pp_program_factor(const pp_hmatrix* h, int workers, pp_program** out);
 int pp_program_solve(const pp_hmatrix* h, int nrhs, int ld, int rhs_split, int workers, pp_program** out);
 /* One program factoring (solve != 0: solving) `count` matrices at once. */
 int pp_program_factor_batch(const pp_hmatrix* const* hs, int count, int workers, pp_program** out);
+/* Windowed factorization (replaces the kernel_size loop of
+ * dev_hierarchical_GETRF, src/launcher.cu:146-167, and the start_index /
+ * length_max arguments of generateLaunchArgsFromTree, src/launcher.cu:46-53):
+ * the factorization's h-ops cut into ceil(ops / window_ops) consecutive
+ * windows (window_ops <= 0: one), each with its own DAG, schedule and
+ * streams; launching the windows in order computes what the whole program
+ * computes.  Every window addresses the whole temp area (slot 1,
+ * pp_program_temp_elems(window, 0) elements): pass the same buffer to every
+ * window's stream; the first window's launch clears it, later windows keep
+ * what earlier ones wrote (their upload fails without that buffer). */
+int pp_program_factor_num_windows(const pp_hmatrix* h, int window_ops);
+int pp_program_factor_window(const pp_hmatrix* h, int workers, int window_ops, int index, pp_program** out);
 int pp_program_solve_batch(const pp_hmatrix* const* hs, int count, const int32_t* nrhs, const int32_t* ld,
                            int rhs_split, int workers, pp_program** out);
 

@@ -67,6 +67,8 @@ def lib():
         "pp_program_factor_batch": (C.c_int, [P(vp), C.c_int, C.c_int, P(vp)]),
         "pp_program_solve_batch": (C.c_int, [P(vp), C.c_int, P(i32), P(i32), C.c_int, C.c_int, P(vp)]),
         "pp_program_factor_inv_batch": (C.c_int, [P(vp), C.c_int, C.c_int, P(vp)]),
+        "pp_program_factor_num_windows": (C.c_int, [vp, C.c_int]),
+        "pp_program_factor_window": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, P(vp)]),
         "pp_program_solve_inv_batch": (C.c_int, [P(vp), C.c_int, P(i32), P(i32), C.c_int, C.c_int, P(vp)]),
         "pp_program_destroy": (None, [vp]),
         "pp_program_get_info": (C.c_int, [vp, vp]),
@@ -440,6 +442,19 @@ class Program:
         fl = np.zeros(max(n, 1), np.int64)
         check(L.pp_program_schedule_entries(self.h, w, _p(ins, C.c_int32), _p(ex, C.c_int32), _p(fl, C.c_int64)))
         return [(int(i), bool(x), int(f)) for i, x, f in zip(ins[:n], ex[:n], fl[:n])]
+
+    @classmethod
+    def factor_windows(cls, hm, workers, window_ops):
+        """The factorization cut into windows of window_ops h-ops, each its own
+        program; launched in order they factor the matrix.  Every window's
+        stream must get the same temp buffer (slot 1)."""
+        L = lib()
+        out = []
+        for i in range(check(L.pp_program_factor_num_windows(hm.h, window_ops))):
+            h = C.c_void_p()
+            check(L.pp_program_factor_window(hm.h, workers, window_ops, i, C.byref(h)))
+            out.append(cls(h, [hm]))
+        return out
 
     def to_bytes(self):
         """Byte string of the instructions, dependencies, schedule and streams."""

@@ -58,6 +58,7 @@ struct pp_program {
   std::vector<int64_t> temp_elems;  // per batch member (slot 3*i+1)
   int64_t scratch = 0;              // per-worker scratch elements
   long long primary_flops = -1;     // FLOPs of the factorization ops alone (factor + inverse programs)
+  bool zero_temps = true;           // false: a window after the first (temps carry over)
   double dag_ms = 0, schedule_ms = 0, encode_ms = 0;
   std::unique_ptr<pp_program_image> image;  // decoded programs only
   ~pp_program();

@@ -305,6 +305,44 @@ int pp_program_factor(const pp_hmatrix* h, int workers, pp_program** out) {
   return pp_program_factor_batch(&h, 1, workers, out);
 }
 
+int pp_program_factor_num_windows(const pp_hmatrix* h, int window_ops) {
+  try {
+    if (h == nullptr) return set_error(PP_ERR_INVALID, "null matrix");
+    dev_temp tmp(1);
+    std::unique_ptr<const h_index> root(h->h->getRootIndex());
+    std::unique_ptr<h_ops_tree> t(h->h->generateOps_GETRF(root.get(), &tmp, 0));
+    return window_ops <= 0 ? 1 : (t->length() + window_ops - 1) / window_ops;
+  } catch (...) {
+    return from_exception();
+  }
+}
+
+int pp_program_factor_window(const pp_hmatrix* h, int workers, int window_ops, int index, pp_program** out) {
+  try {
+    if (h == nullptr || workers < 1 || out == nullptr) return set_error(PP_ERR_INVALID, "bad args");
+    dev_temp tmp(1);
+    std::unique_ptr<const h_index> root(h->h->getRootIndex());
+    std::unique_ptr<h_ops_tree> t(h->h->generateOps_GETRF(root.get(), &tmp, 0));
+    const int len = t->length();
+    const int size = window_ops <= 0 ? len : window_ops;
+    const int first = index * size;
+    if (index < 0 || (first >= len && !(len == 0 && index == 0)))
+      return set_error(PP_ERR_INVALID, "window index out of range");
+    auto p = std::make_unique<pp_program>();
+    p->n_slots = 3;
+    for (int i = first; i < std::min(len, first + size); ++i) p->tree.push(h_op(t->getOp(i)));
+    // every window addresses the whole factorization's temp area; only the
+    // first clears it, later windows read what earlier ones left there
+    p->temp_elems.push_back(tmp.size());
+    p->zero_temps = index == 0;
+    finish_program(p.get(), workers);
+    *out = p.release();
+    return PP_OK;
+  } catch (...) {
+    return from_exception();
+  }
+}
+
 int pp_program_factor_inv_batch(const pp_hmatrix* const* hs, int count, int workers, pp_program** out) {
   return ppc::factor_inverse_program(hs, count, workers, true, out);
 }

@@ -9,6 +9,7 @@
 //   same storage plan and checks the arena size against `storage`.
 // ARNA payload: i64 count | f64 elements[count] (count = storage).
 // PROG payload: i32 workers | i32 n_slots | i32 n_insts | i32 n_members |
+//   i32 flags (bit 0: a window after the first, temps carried over) | i32 0 |
 //   i64 n_edges | i64 flops | i64 primary_flops | i64 scratch |
 //   i64 comm_waits | f64 dag_density | i64 temp_elems[n_members] |
 //   per instruction: i32 operation | i32 n_words | i64 flops | i64 scratch |
@@ -146,6 +147,8 @@ std::vector<uint8_t> program_payload(const pp_program* p) {
   w.put<int32_t>(p->n_slots);
   w.put<int32_t>(n);
   w.put<int32_t>((int32_t)p->temp_elems.size());
+  w.put<int32_t>(p->zero_temps ? 0 : 1);
+  w.put<int32_t>(0);
   w.put<int64_t>(info.n_edges);
   w.put<int64_t>(info.flops);
   w.put<int64_t>(p->primary_flops);
@@ -208,9 +211,12 @@ std::unique_ptr<pp_program> decode_program(const ppw::section& s) {
   p->n_slots = r.get<int32_t>();
   const int n = r.get<int32_t>();
   const int members = r.get<int32_t>();
+  const int flags = r.get<int32_t>();
+  (void)r.get<int32_t>();
   if (p->workers < 1 || p->workers > (1 << 16) || p->n_slots < 0 || p->n_slots > (1 << 24) || n < 0 ||
-      members < 0 || members > p->n_slots)
+      members < 0 || members > p->n_slots || (flags & ~1) != 0)
     throw wire_error("bad program header");
+  p->zero_temps = (flags & 1) == 0;
   im->n_insts = n;
   im->n_edges = r.get<int64_t>();
   im->flops = r.get<int64_t>();

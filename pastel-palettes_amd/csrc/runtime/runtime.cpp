@@ -298,10 +298,14 @@ int pp_stream_upload_program(pp_ctx* ctx, const pp_program* prog, const void* co
       const int64_t n = prog->temp_elems[i];
       if (n <= 0) continue;
       if (s->ptrs[slot] == nullptr) {
+        if (!prog->zero_temps) {
+          rc = set_error(PP_ERR_INVALID, "a window after the first needs the temp buffer of the first window");
+          break;
+        }
         rc = ctx_alloc(ctx, (uint64_t)n * eb, &s->ptrs[slot]);
         if (rc == PP_OK) s->owned.push_back(s->ptrs[slot]);
       }
-      s->zero_slots.emplace_back(slot, n);
+      if (prog->zero_temps) s->zero_slots.emplace_back(slot, n);
     }
     if (rc == PP_OK) rc = push_ptrs(s);
     if (rc != PP_OK) {

@@ -171,6 +171,8 @@ def main():
                          "on the device")
     ap.add_argument("--batch-count", type=int, default=512)
     ap.add_argument("--serve-seconds", type=float, default=20.0)
+    ap.add_argument("--serve-slots", type=int, default=4)
+    ap.add_argument("--host-build", action="store_true", help="serve: build the matrices on the host")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path (run with --gpus 1)",
@@ -198,7 +200,8 @@ def main():
             out = wl.run_build(pp, torch, ctx, n=args.n, levels=args.levels, rank=args.rank, steps=args.steps,
                                warmup=args.warmup, log=log)
         else:
-            out = wl.run_serve(pp, torch, ctx, duration=args.serve_seconds, log=log)
+            out = wl.run_serve(pp, torch, ctx, duration=args.serve_seconds, slots=args.serve_slots,
+                               device_build=not args.host_build, log=log)
         print(json.dumps(out), flush=True)
         return
 

@@ -13,7 +13,9 @@ serve: a Poisson stream of factor + solve requests over a pool of resident
     refinement; HBM-budgeted admission.  The open-loop rate is a fraction of
     the peak rate measured closed-loop on the same pool.
 
-Matrices are built on the host (not timed) by a pool of spawned processes.
+Batch matrices are built on the host (not timed) by a pool of spawned
+processes; the serving pool is built on the device (pp_hmatrix_build_kernel)
+unless the host builder is asked for.
 """
 import math
 import multiprocessing as mp
@@ -145,20 +147,29 @@ def run_batch(pp, torch, ctx, count=512, nrhs=4, steps=3, warmup=1, tol=1e-10, p
 
 
 def run_serve(pp, torch, ctx, keys=12, duration=20.0, load=0.8, burst=48, slots=4, tol=1e-10, budget_gb=200,
-              log=print):
+              device_build=True, log=print):
     from . import serve
     rng = np.random.default_rng(5000)
     sizes = log_uniform_sizes(rng, keys, 1024, 32768, 256)
     jobs = [(n, max(1, int(round(math.log2(n / 256)))), 32, 200 + i) for i, n in enumerate(sizes)]
-    log(f"[serve] building {keys} matrices (N = {sizes})")
+    log(f"[serve] building {keys} matrices (N = {sizes}) on the {'device' if device_build else 'host'}")
     t0 = time.perf_counter()
-    arenas = build_arenas(jobs, host_procs())
+    matrices = {}
+    if device_build:
+        # construction on the device: points uploaded, arena filled in place
+        for i, (n, lv, r, seed) in enumerate(jobs):
+            hm = pp.HMatrix("hodlr", n, levels=lv, rank=r)
+            pts = torch.from_numpy(np.ascontiguousarray(pp.points_2d_morton(seed, n))).cuda()
+            arena = torch.empty(hm.storage, dtype=torch.float64, device="cuda")
+            hm.build_on_device(ctx, pp.KERNEL_INV2D, pts.data_ptr(), 1e-3, n / 4.0, arena.data_ptr())
+            matrices[f"m{i}"] = (hm, arena)
+    else:
+        arenas = build_arenas(jobs, host_procs())
+        for i, ((n, lv, r, _), a) in enumerate(zip(jobs, arenas)):
+            matrices[f"m{i}"] = (pp.HMatrix("hodlr", n, levels=lv, rank=r), torch.from_numpy(a).cuda())
+        del arenas
     build_s = time.perf_counter() - t0
     log(f"[serve] built in {build_s:.1f} s")
-    matrices = {}
-    for i, ((n, lv, r, _), a) in enumerate(zip(jobs, arenas)):
-        matrices[f"m{i}"] = (pp.HMatrix("hodlr", n, levels=lv, rank=r), torch.from_numpy(a).cuda())
-    del arenas
     names = sorted(matrices)
     workers = min(ctx.cu_count, 256) // slots
     server = serve.Server(pp, torch, ctx, matrices, pp.MIXED, int(budget_gb * 1e9), slots=slots,
@@ -191,7 +202,8 @@ def run_serve(pp, torch, ctx, keys=12, duration=20.0, load=0.8, burst=48, slots=
     return {
         "workload": f"serving: Poisson requests over {keys} resident HODLR matrices, N log-uniform [1024, 32768], "
                     f"rank 32, leaf ~256, 1-64 rhs, 30% reuse, mixed precision + FP64 refinement to {tol:g}, "
-                    f"{slots} slots x {workers} workers, HBM budget {budget_gb} GB",
+                    f"{slots} slots x {workers} workers, HBM budget {budget_gb} GB, matrices built on the "
+                    f"{'device' if device_build else 'host'}",
         "sizes": sizes,
         "peak_rps": peak["throughput_rps"],
         "rate_rps": rate,

@@ -26,6 +26,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 PEAK_TFLOPS = {"fp64": 78.6, "fp32": 157.3, "bf16": 2500.0}
+HBM_BYTES_PER_S = 8.0e12
 
 
 def log(*a):
@@ -43,6 +44,21 @@ def factor_flop_split(prog):
         else:
             upd += f
     return leaf, upd
+
+
+def lower_bound_ms(prog, mixed):
+    """T_lb = sum_i max(F_i / P_i, B_i / BW) over the factor program's
+    instructions: P_i the FP64 MFMA peak (plain), or FP32 for leaf getrf /
+    trsm and BF16 for products (mixed); B_i the instruction's algorithmic
+    bytes at the storage width (8 bytes FP64, 4 FP32); BW = 8 TB/s."""
+    t = 0.0
+    for i in range(prog.info()["n_insts"]):
+        op = int(prog.inst_words(i)[2])
+        f = prog.inst_flops(i)
+        b = prog.inst_bytes(i) * (0.5 if mixed else 1.0)
+        p = (PEAK_TFLOPS["fp32"] if op in (1, 2, 3, 9) else PEAK_TFLOPS["bf16"]) if mixed else PEAK_TFLOPS["fp64"]
+        t += max(f / (p * 1e12), b / (HBM_BYTES_PER_S))
+    return 1e3 * t
 
 
 def run_precision(pp, torch, ctx, hm, arena_dev, b, prec, steps, warmup, workers, wavefronts, nrhs, tol):
@@ -232,6 +248,7 @@ def main():
         peak = PEAK_TFLOPS["fp64"]
         dtype = "f64"
     achieved = r["factor_flops"] / (r["factor_kernel_ms"] * 1e-3) / 1e12
+    t_lb = lower_bound_ms(prog, head == "mixed")
 
     cpu = None
     if not args.no_cpu:
@@ -269,7 +286,10 @@ def main():
                      "frac": achieved / peak, "traffic": measured_traffic(
                          f"vm_factor_kernel_{head}_hodlr_n{args.n}_l{args.levels}_r{args.rank}"),
                      "kernel": "vm_factor_kernel (factorization launch)", "factor_kernel_ms": r["factor_kernel_ms"],
-                     "factor_flops": r["factor_flops"]},
+                     "factor_flops": r["factor_flops"],
+                     "t_lb_ms": t_lb, "t_lb_frac": t_lb / r["factor_kernel_ms"],
+                     "t_lb_note": "sum over the H-LU instructions of max(FLOPs / MFMA peak, algorithmic bytes / "
+                                  "8 TB/s); the inverse construction in the same launch is not counted"},
         "cpu_baseline": cpu,
     }
     if "fp64" in res:

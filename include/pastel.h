@@ -152,6 +152,10 @@ int pp_program_worker_words(const pp_program* p, int worker, int32_t* out, int c
 /* execute words of instruction `inst` (id slot filled); returns length */
 int pp_program_inst_words(const pp_program* p, int inst, int32_t* out, int cap);
 int64_t pp_program_inst_flops(const pp_program* p, int inst);
+/* Algorithmic bytes of instruction `inst` at 8 bytes per element: operands
+ * read once, operands written counted twice (read and written); 0 for a
+ * decoded program (operand maps are not part of the byte-string format). */
+int64_t pp_program_inst_bytes(const pp_program* p, int inst);
 /* predecessors (to-list) of inst; returns count (fills up to cap) */
 int pp_program_inst_preds(const pp_program* p, int inst, int32_t* preds, int32_t* masks, int cap);
 /* scheduler entries of one worker (like pp_schedule_worker_entries) */

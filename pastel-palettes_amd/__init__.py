@@ -77,6 +77,7 @@ def lib():
         "pp_program_worker_words": (C.c_int, [vp, C.c_int, P(i32), C.c_int]),
         "pp_program_inst_words": (C.c_int, [vp, C.c_int, P(i32), C.c_int]),
         "pp_program_inst_flops": (i64, [vp, C.c_int]),
+        "pp_program_inst_bytes": (i64, [vp, C.c_int]),
         "pp_program_inst_preds": (C.c_int, [vp, C.c_int, P(i32), P(i32), C.c_int]),
         "pp_program_schedule_length": (C.c_int, [vp, C.c_int]),
         "pp_program_schedule_entries": (C.c_int, [vp, C.c_int, P(i32), P(i32), P(i64)]),
@@ -425,6 +426,10 @@ class Program:
 
     def inst_flops(self, i):
         return int(check(lib().pp_program_inst_flops(self.h, i)))
+
+    def inst_bytes(self, i):
+        """Algorithmic bytes at 8 per element (written operands twice)."""
+        return int(check(lib().pp_program_inst_bytes(self.h, i)))
 
     def preds(self, i):
         L = lib()

@@ -493,6 +493,14 @@ int pp_program_inst_words(const pp_program* p, int inst, int32_t* out, int cap) 
   return n;
 }
 
+int64_t pp_program_inst_bytes(const pp_program* p, int inst) {
+  if (p == nullptr || inst < 0 || inst >= p->tree.length()) return set_error(PP_ERR_INVALID, "bad instruction");
+  int64_t b = 0;
+  for (const auto& d : p->tree.getOp(inst).data.getData())
+    b += (d.first ? 1 : 2) * (int64_t)(d.second.width * d.second.height);
+  return b;
+}
+
 int64_t pp_program_inst_flops(const pp_program* p, int inst) {
   if (p == nullptr || inst < 0 || inst >= p->tree.length()) return set_error(PP_ERR_INVALID, "bad instruction");
   return p->tree.getOp(inst).flops;

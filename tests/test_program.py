@@ -307,3 +307,22 @@ def test_oracle_hierarchical_inverse(pp, orc, n, levels, rank):
     li, ui = dense_inverses(hm, ifac[hm.inverse_base:])
     assert np.max(np.abs(li - np.linalg.inv(l))) < 1e-11 * np.max(np.abs(np.linalg.inv(l)))
     assert np.max(np.abs(ui - np.linalg.inv(u))) < 1e-11 * np.max(np.abs(np.linalg.inv(u)))
+
+
+def test_instruction_bytes(pp):
+    """Algorithmic bytes per instruction: operands read once, written twice."""
+    hm = pp.HMatrix("dense", 512, leaf=128)
+    prog = pp.Program.factor(hm, 2)
+    seen = set()
+    for i in range(prog.info()["n_insts"]):
+        w = prog.inst_words(i)
+        op = int(w[2])
+        b = prog.inst_bytes(i)
+        if op == 1:  # getrf of a 128 x 128 leaf: read and written
+            assert b == 2 * 128 * 128 * 8
+        elif op in (2, 3):  # trsm: 128 x 128 B written, the triangular factor read
+            assert b == 2 * 128 * 128 * 8 + 128 * 128 * 8
+        elif op == 4:  # gemm M -= A B on 128 x 128 blocks
+            assert b == 2 * 128 * 128 * 8 + 2 * 128 * 128 * 8
+        seen.add(op)
+    assert {1, 2, 3, 4} <= seen

@@ -323,11 +323,12 @@ __global__ void __launch_bounds__(256) tr_gsum_kernel(TruncFactor* fs) {
 }
 
 // per factor (G summed by tr_gsum_kernel): shift (first pass),
-// Cholesky G = R^T R (R in the upper triangle of G), Rt <- R Rt
+// Cholesky G = R^T R (R in the upper triangle of G), R^{-1} (Rt <- R Rt
+// follows in tr_rprod_kernel)
 __global__ void __launch_bounds__(256) tr_chol_kernel(TruncFactor* fs, int pass) {
   TruncFactor* F = fs + blockIdx.x;
   const int kp = F->kp, tid = threadIdx.x;
-  double *G = F->G, *Rt = F->Rt, *T = F->T;
+  double *G = F->G, *Rt = F->Rt;
   const double* d = F->d;
   if (F->status) return;
   if (pass == 0)
@@ -369,14 +370,24 @@ __global__ void __launch_bounds__(256) tr_chol_kernel(TruncFactor* fs, int pass)
       Ri[a * kp + c] = -v / G[a * kp + a];
     }
   }
-  for (int e = tid; e < kp * kp; e += 256) {
-    const int a = e / kp, c = e % kp;
-    double v = 0.0;
-    for (int b = a; b <= c; ++b) v = fma(G[a * kp + b], Rt[b * kp + c], v);
-    T[e] = c >= a ? v : 0.0;
-  }
-  __syncthreads();
-  for (int e = tid; e < kp * kp; e += 256) Rt[e] = T[e];
+}
+
+// T = R Rt (both upper; grid: factor x 256 entries), then Rt = T
+__global__ void __launch_bounds__(256) tr_rprod_kernel(TruncFactor* fs) {
+  const TruncFactor& F = fs[blockIdx.x];
+  const int kp = F.kp, e = blockIdx.y * 256 + threadIdx.x;
+  if (F.status || e >= kp * kp) return;
+  const int a = e / kp, c = e % kp;
+  double v = 0.0;
+  for (int b = a; b <= c; ++b) v = fma(F.G[a * kp + b], F.Rt[b * kp + c], v);
+  F.T[e] = c >= a ? v : 0.0;
+}
+
+__global__ void __launch_bounds__(256) tr_rcopy_kernel(TruncFactor* fs) {
+  const TruncFactor& F = fs[blockIdx.x];
+  const int kp = F.kp, e = blockIdx.y * 256 + threadIdx.x;
+  if (F.status || e >= kp * kp) return;
+  F.Rt[e] = F.T[e];
 }
 
 // rows of a slice <- row R^{-1}
@@ -575,6 +586,10 @@ hipError_t launch_truncate(TruncLeaf* leaves, int n_leaves, int max_kp, TruncFac
     hipLaunchKernelGGL(tr_gram_kernel, dim3(n_slices), dim3(256), 0, s, slices);
     hipLaunchKernelGGL(tr_gsum_kernel, dim3(2 * n_leaves, (max_kp * max_kp + 255) / 256), dim3(256), 0, s, factors);
     hipLaunchKernelGGL(tr_chol_kernel, dim3(2 * n_leaves), dim3(256), 0, s, factors, pass);
+    hipLaunchKernelGGL(tr_rprod_kernel, dim3(2 * n_leaves, (max_kp * max_kp + 255) / 256), dim3(256), 0, s,
+                       factors);
+    hipLaunchKernelGGL(tr_rcopy_kernel, dim3(2 * n_leaves, (max_kp * max_kp + 255) / 256), dim3(256), 0, s,
+                       factors);
     hipLaunchKernelGGL(tr_trsm_kernel, dim3(n_slices), dim3(256), 0, s, slices, factors);
   }
   hipLaunchKernelGGL(tr_core_kernel, dim3(n_leaves), dim3(256), 0, s, leaves, factors);

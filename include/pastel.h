@@ -210,6 +210,10 @@ int pp_stream_destroy(pp_ctx* ctx, pp_stream* s);
 int pp_stream_set_watchdog(pp_ctx* ctx, pp_stream* s, int ms);
 /* Kernel time of the last launch in ms (HIP events around the VM kernel). */
 double pp_stream_last_ms(const pp_stream* s);
+/* accum operations of the last waited launch whose CholeskyQR3 failed and
+ * fell back to Householder QR (a numerical-health count; no reference
+ * counterpart, the reference always uses Givens QR). */
+int pp_stream_qr_fallbacks(const pp_stream* s);
 /* Maximum workers that fit co-resident for a precision / wave count. */
 int pp_max_workers(pp_ctx* ctx, int precision, int wavefronts_per_inst);
 

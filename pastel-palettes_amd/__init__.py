@@ -99,6 +99,7 @@ def lib():
         "pp_stream_destroy": (C.c_int, [vp, vp]),
         "pp_stream_set_watchdog": (C.c_int, [vp, vp, C.c_int]),
         "pp_stream_last_ms": (dp, [vp]),
+        "pp_stream_qr_fallbacks": (C.c_int, [vp]),
         "pp_max_workers": (C.c_int, [vp, C.c_int, C.c_int]),
         "pp_exec_single": (C.c_int, [vp, C.c_int, P(i32), C.c_int, P(vp), C.c_int, u64, u64]),
         "pp_exec_single_timed": (C.c_int, [vp, C.c_int, P(i32), C.c_int, P(vp), C.c_int, u64, u64, C.c_int, C.c_int,
@@ -560,6 +561,11 @@ class Stream:
         a = np.zeros(n, np.uint64)
         lib().pp_read_trace(self.ctx.h, self.h, _p(a, C.c_uint64), n)
         return a
+
+    def qr_fallbacks(self):
+        """accum operations of the last waited launch that fell back to
+        Householder QR."""
+        return int(check(lib().pp_stream_qr_fallbacks(self.h)))
 
     def last_ms(self):
         return float(lib().pp_stream_last_ms(self.h))

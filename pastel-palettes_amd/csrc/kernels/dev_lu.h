@@ -593,16 +593,21 @@ __device__ __forceinline__ volatile PP_LDS int* chol_fail_flag() {
 // when its plain factorisation meets a non-positive pivot, G + s I is
 // factorised instead, s = 11 (rows k + k (k + 1)) u tr(G) (the shift of
 // shifted CholeskyQR3, Fukaya et al., SIAM J. Sci. Comput. 42 (2020)); the
-// failure flag is reset there.  Later factorisations are never shifted: a
-// failure sets the flag (the caller then orthonormalises by Householder).
+// failure flag is reset there.  Later factorisations are not shifted (a
+// failure sets the flag and the caller orthonormalises by Householder) unless
+// `retry_all`: then every factorisation that meets a non-positive pivot is
+// retried with the shift.  The mixed-precision mode uses that: its sketches of
+// rank-48 updates are often too ill-conditioned for FP32 CholeskyQR3, the
+// Householder fallback costs several times the accum, and the factorization is
+// a preconditioner for FP64 refinement built from BF16 products anyway.
 // Entries below the diagonal are set to zero.
 template <typename T>
-__device__ __forceinline__ void chol_upper(T* G, int k, int rows, bool first) {
+__device__ __forceinline__ void chol_upper(T* G, int k, int rows, bool first, bool retry_all = false) {
   PP_LDS T* s = (PP_LDS T*)lds_base();  // k x (k + 1)
   const int sp = k + 1, tid = threadIdx.x, nt = blockDim.x;
   const T u = sizeof(T) == 8 ? T(1.1102230246251565e-16) : T(5.9604644775390625e-08);
   bool ok = false;
-  for (int attempt = 0; attempt < (first ? 2 : 1) && !ok; ++attempt) {
+  for (int attempt = 0; attempt < ((first || retry_all) ? 2 : 1) && !ok; ++attempt) {
     for (int e = tid; e < k * k; e += nt) {
       const int i = e / k, j = e - i * k;
       s[i * sp + j] = G[e];

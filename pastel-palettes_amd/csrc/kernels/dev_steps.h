@@ -501,7 +501,7 @@ __device__ __forceinline__ void exec_step(const Step<T>& st) {
       copy_block<T>(st.C, st.ldc, st.A, st.lda, st.m, st.n);
       break;
     case ST_CHOL:
-      chol_upper<T>(st.C, st.m, st.k, st.aT);
+      chol_upper<T>(st.C, st.m, st.k, st.aT, GMODE == MODE_BF16);
       break;
     case ST_LUUPD:
       lu_update<T, FMODE>(st.C, st.ldc, st.m, st.n, st.lda, st.k, st.A);
@@ -515,8 +515,10 @@ __device__ __forceinline__ void exec_step(const Step<T>& st) {
 }
 
 // Executes one execute entry; returns its length in words or -1.
+// fallbacks (optional): counts the Householder fallbacks of accum.
 template <typename T, int FMODE, int GMODE>
-__device__ __forceinline__ int exec_entry(const int32_t* pc, void* const* ptrs, const void* rnd, T* tmp) {
+__device__ __forceinline__ int exec_entry(const int32_t* pc, void* const* ptrs, const void* rnd, T* tmp,
+                                          int32_t* fallbacks = nullptr) {
   const int len = entry_length(word(pc, 2));
   if (len < 0) return -1;
   for (int s = 0;; ++s) {
@@ -524,6 +526,7 @@ __device__ __forceinline__ int exec_entry(const int32_t* pc, void* const* ptrs, 
     const int r = make_step<T>(pc, ptrs, s, tmp, reinterpret_cast<const T*>(rnd), st);
     if (r < 0) return -1;
     if (r == 0) break;
+    if (st.kind == ST_QR && fallbacks != nullptr && threadIdx.x == 0) atomicAdd(fallbacks, 1);
     exec_step<T, FMODE, GMODE>(st);
   }
   return len;

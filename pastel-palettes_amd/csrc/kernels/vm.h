@@ -33,7 +33,8 @@ struct VmArgs {
   const void* rnd;            // sketch matrix Omega (storage precision)
   uint64_t* trace;            // optional: per-entry timestamps
   const int64_t* trace_off;   // [workers] start of each worker's trace
-  int32_t* status;            // [0] code, [1] worker, [2] instruction, [3] abort flag, [4] entries executed
+  int32_t* status;            // [0] code, [1] worker, [2] instruction, [3] abort flag, [4] entries executed,
+                              // [5] Householder fallbacks of accum
   int64_t spin_limit;         // wall-clock ticks (100 MHz) a wait may last
   const uint32_t* epoch;      // device counter bumped before every launch
   int32_t n_insts;

@@ -71,7 +71,7 @@ __device__ __forceinline__ void vm_run(const VmArgs& a) {
       break;
     }
     const int32_t id = word(pc, 1);
-    const int len = exec_entry<T, FMODE, GMODE>(pc, a.ptrs, a.rnd, tmp);
+    const int len = exec_entry<T, FMODE, GMODE>(pc, a.ptrs, a.rnd, tmp, a.status + 5);
     if (len < 0) {
       if (threadIdx.x == 0) record_error(a.status, VM_BAD_OPERATION, w, id);
       break;

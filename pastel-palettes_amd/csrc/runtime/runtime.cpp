@@ -319,6 +319,8 @@ int pp_stream_upload_program(pp_ctx* ctx, const pp_program* prog, const void* co
   }
 }
 
+int pp_stream_qr_fallbacks(const pp_stream* s) { return s ? s->last_fallbacks : set_error(PP_ERR_INVALID, "null"); }
+
 int pp_stream_set_slot(pp_ctx* ctx, pp_stream* s, int slot, void* dptr) {
   if (ctx == nullptr || s == nullptr || slot < 0 || slot >= s->n_ptrs) return set_error(PP_ERR_INVALID, "bad slot");
   s->ptrs[slot] = dptr;
@@ -367,6 +369,7 @@ int pp_wait(pp_ctx* ctx, pp_stream* s, int timeout_ms, int32_t* first_unsignalle
   }
   int32_t st[8];
   HIP_TRY(hipMemcpy(st, s->d_status, sizeof(st), hipMemcpyDeviceToHost));
+  s->last_fallbacks = st[5];
   if (st[0] == pp::VM_OK) return PP_OK;
   if (first_unsignalled) {
     uint32_t epoch = 0;

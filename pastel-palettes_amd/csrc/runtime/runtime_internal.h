@@ -80,6 +80,7 @@ struct pp_stream {
   hipStream_t last = nullptr;
   bool launched = false;
   int role = 0;  // 1: a solver's factorization program (own kernel symbol)
+  int last_fallbacks = 0;  // Householder fallbacks of accum in the last waited launch
 };
 
 namespace ppr {

@@ -210,3 +210,28 @@ def test_upload_rejects_unknown_opcode(pp, ctx):
     rc = L.pp_stream_upload(ctx.h, arr, lens.ctypes.data_as(C.POINTER(C.c_int32)), 1, (C.c_void_p * 1)(None), 1, 64,
                             1, 0, C.byref(h))
     assert rc == -1
+
+
+def test_mixed_rank48_accum_without_fallback(pp, ctx, orc):
+    """Mixed precision at rank 48: every accum orthonormalises by CholeskyQR
+    (no Householder fallback), and the refined solve still reaches 1e-10."""
+    n, nrhs = 4096, 4
+    print(f"[case] mixed rank 48 n={n}", flush=True)
+    hm, kern, arena = pp.build_config(1, n, levels=4, rank=48)
+    prog = pp.Program.factor_inv_batch([hm], 64)
+    full = hm.with_inverse_area(arena)
+    d = torch.from_numpy(full).float().cuda()
+    s = pp.Stream(ctx, prog, [d.data_ptr(), None, None], pp.MIXED)
+    s.set_watchdog(20000)
+    s.launch(4)
+    s.wait()
+    assert s.qr_fallbacks() == 0
+    s.close()
+    solver = pp.Solver(ctx, hm, arena, precision=pp.MIXED, workers=64, nrhs_max=nrhs)
+    solver.factor()
+    b = pp.rhs(1, n, nrhs)
+    db = torch.from_numpy(b).cuda()
+    dx = torch.zeros_like(db)
+    rep = solver.solve(db.data_ptr(), dx.data_ptr(), nrhs, tol=1e-10, max_iter=40)
+    solver.close()
+    assert rep["converged"] and rep["residual"] <= 1e-10

@@ -74,7 +74,7 @@ int pp_hmatrix_build_kernel(pp_ctx* ctx, const pp_hmatrix* h, int kind, const do
     }
     // scratch plan (element offsets; pointers are fixed up after allocation)
     carver cv;
-    std::vector<int64_t> o_u(lrs.size(), -1), o_v(lrs.size(), -1), o_used(lrs.size()), o_tr(lrs.size(), -1);
+    std::vector<int64_t> o_u(lrs.size(), -1), o_v(lrs.size(), -1), o_used(lrs.size());
     const int64_t o_state = cv.take(((int64_t)lrs.size() * sizeof(pp::AcaState) + 7) / 8);
     int n_col = 0, n_row = 0;
     for (size_t i = 0; i < lrs.size(); ++i) {
@@ -99,7 +99,6 @@ int pp_hmatrix_build_kernel(pp_ctx* ctx, const pp_hmatrix* h, int kind, const do
     };
     std::vector<fplan> fp(2 * trunc.size());
     std::vector<int64_t> o_N(trunc.size()), o_J(trunc.size()), o_sel(trunc.size());
-    int n_tslices = 0;
     for (size_t q = 0; q < trunc.size(); ++q) {
       const lr& L = lrs[trunc[q]];
       const int64_t kp = L.kp;
@@ -108,7 +107,6 @@ int pp_hmatrix_build_kernel(pp_ctx* ctx, const pp_hmatrix* h, int kind, const do
         const int m = f == 0 ? L.b->ny : L.b->nx;
         fplan& F = fp[2 * q + f];
         F.nsl = (m + S - 1) / S;
-        n_tslices += F.nsl;
         F.G = cv.take(kp * kp);
         F.Rt = cv.take(kp * kp);
         F.T = cv.take(kp * kp);
@@ -121,9 +119,18 @@ int pp_hmatrix_build_kernel(pp_ctx* ctx, const pp_hmatrix* h, int kind, const do
       o_J[q] = cv.take(kp * kp);
       o_sel[q] = cv.take((L.b->rank + 1) / 2);
     }
+    // device allocations of this call, freed on every way out
+    struct freer {
+      pp_ctx* c;
+      std::vector<void*> p;
+      ~freer() {
+        for (void* x : p) ctx_free(c, x);
+      }
+    } owned{ctx, {}};
     double* d_s = nullptr;
     int rc = ctx_alloc_t(ctx, std::max<int64_t>(cv.used, 1), &d_s);
     if (rc) return rc;
+    owned.p.push_back(d_s);
     // tables
     std::vector<pp::AcaLeaf> al(lrs.size());
     std::vector<pp::AcaState> as(lrs.size());
@@ -208,12 +215,10 @@ int pp_hmatrix_build_kernel(pp_ctx* ctx, const pp_hmatrix* h, int kind, const do
     pp::TruncFactor* d_tf = nullptr;
     pp::TruncLeaf* d_tl = nullptr;
     pp::TruncSlice* d_ts = nullptr;
-    std::vector<void*> tabs;
+    // one device array per table
     auto table = [&](auto** dp, const auto& v) {
-      using E = typename std::remove_reference<decltype(v)>::type::value_type;
       if (rc == PP_OK) rc = ctx_alloc_t(ctx, (int64_t)v.size(), dp);
-      if (rc == PP_OK) tabs.push_back(*dp);
-      (void)sizeof(E);
+      if (rc == PP_OK) owned.p.push_back(*dp);
     };
     table(&d_dense, dense);
     table(&d_al, al);
@@ -255,8 +260,6 @@ int pp_hmatrix_build_kernel(pp_ctx* ctx, const pp_hmatrix* h, int kind, const do
     for (size_t q = 0; rc == PP_OK && q < tl.size(); ++q)
       if (tl[q].status != 0)
         rc = set_error(PP_ERR_INTERNAL, "truncation of a cross approximation failed (Cholesky pivot)");
-    for (void* p : tabs) ctx_free(ctx, p);
-    ctx_free(ctx, d_s);
     return rc;
   } catch (...) {
     return from_exception();

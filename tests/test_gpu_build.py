@@ -27,7 +27,7 @@ def device_build(pp, ctx, hm, kern, kind, param):
     return arena.cpu().numpy(), piv.cpu().numpy()[:hm.build_pivots].reshape(-1, 2)
 
 
-@pytest.mark.parametrize("n,levels,rank", [(1024, 2, 8), (2048, 3, 16), (4096, 4, 32)])
+@pytest.mark.parametrize("n,levels,rank", [(1024, 2, 8), (2048, 3, 16), (4096, 4, 32), (1000, 2, 20), (3000, 3, 24)])
 def test_build_inv2d_bit_exact(pp, orc, ctx, n, levels, rank):
     print(f"[case] build inv2d n={n} levels={levels} rank={rank}", flush=True)
     hm = pp.HMatrix("hodlr", n, levels=levels, rank=rank)

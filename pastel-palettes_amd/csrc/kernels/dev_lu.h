@@ -172,6 +172,24 @@ __device__ __forceinline__ void lu_update(T* M, int ld, int p, int pw, int nx, i
       for (int h = 0; h < 2; ++h)
 #pragma unroll
         for (int e = 0; e < EPL; ++e) ub[c][h][e] = ut[(c * KW + EPL * g + e) * UT + 16 * h + li];
+    // C values of a tile (clamped addresses; the stores are masked)
+    auto load_c = [&](int rt, T (&cv)[2][2][4]) {
+      const int r0 = rt * 32;
+#pragma unroll
+      for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            const int row = r0 + a * 16 + P::orow(l, r), col = c0 + j * 16 + li;
+            cv[a][j][r] = gload<T>(A22 + (int64_t)(row < rm ? row : rm - 1) * ld + (col < rn ? col : rn - 1));
+          }
+    };
+    // FP32: the next tile's C values travel with its L21 fragments (two
+    // buffers); FP64: two C buffers would not fit the register file next to
+    // the fragments (the kernel spilled), so a tile's C values are requested
+    // when its products start and consumed after them
+    constexpr bool kPrefetchC = sizeof(T) == 4;
     auto load_tile = [&](int rt, T (&fa)[CH][2][EPL], T (&cv)[2][2][4]) {
       const int r0 = rt * 32;
       OperandLane<T, EPL> la[2];
@@ -184,18 +202,11 @@ __device__ __forceinline__ void lu_update(T* M, int ld, int p, int pw, int nx, i
           if (lvec) la[h].template load<true>(fa[c][h], c * KW + EPL * g, pw);
           else la[h].template load<false>(fa[c][h], c * KW + EPL * g, pw);
         }
-#pragma unroll
-      for (int a = 0; a < 2; ++a)
-#pragma unroll
-        for (int j = 0; j < 2; ++j)
-#pragma unroll
-          for (int r = 0; r < 4; ++r) {
-            const int row = r0 + a * 16 + P::orow(l, r), col = c0 + j * 16 + li;
-            cv[a][j][r] = gload<T>(A22 + (int64_t)(row < rm ? row : rm - 1) * ld + (col < rn ? col : rn - 1));
-          }
+      if constexpr (kPrefetchC) load_c(rt, cv);
     };
-    auto apply_tile = [&](int rt, const T (&fa)[CH][2][EPL], const T (&cv)[2][2][4]) {
+    auto apply_tile = [&](int rt, const T (&fa)[CH][2][EPL], T (&cv)[2][2][4]) {
       const int r0 = rt * 32;
+      if constexpr (!kPrefetchC) load_c(rt, cv);
       acc_t acc[2][2];
 #pragma unroll
       for (int a = 0; a < 2; ++a) acc[a][0] = acc[a][1] = P::zero();
@@ -215,7 +226,14 @@ __device__ __forceinline__ void lu_update(T* M, int ld, int p, int pw, int nx, i
             if (row < rm && col < rn) gstore<T>(A22 + (int64_t)row * ld + col, cv[a][j][r] - acc[a][j][r]);
           }
     };
+    // (in FP64 cv0 and cv1 are each live only inside one apply_tile)
     T fa0[CH][2][EPL], fa1[CH][2][EPL], cv0[2][2][4], cv1[2][2][4];
+    if constexpr (!kPrefetchC) {
+      for (int rt = 0; rt < rtn; ++rt) {
+        load_tile(rt, fa0, cv0);
+        apply_tile(rt, fa0, cv0);
+      }
+    } else {
     load_tile(0, fa0, cv0);
     for (int rt = 0; rt < rtn; rt += 2) {
       // past the last tile the last one is requested again, so every
@@ -226,6 +244,7 @@ __device__ __forceinline__ void lu_update(T* M, int ld, int p, int pw, int nx, i
         load_tile(rt + 2 < rtn ? rt + 2 : rt + 1, fa0, cv0);
         apply_tile(rt + 1, fa1, cv1);
       }
+    }
     }
   }
   __syncthreads();

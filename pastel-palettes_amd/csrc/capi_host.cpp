@@ -276,6 +276,86 @@ void finish_program(pp_program* p, int workers) {
   p->scratch = p->tree.maxScratch();
 }
 
+// Two-phase program: ops [0, F) (a factorization) scheduled exactly as the
+// factorization alone, ops [F, n) (the inverse construction) by their own
+// schedule, appended to every worker's stream after its factorization
+// entries.  Each op of the second phase is preceded by waits for the
+// first-phase ops it depends on; no first-phase op depends on the second, so
+// every wait ends.  The second phase only fills the time a worker has left
+// after its factorization work and never delays it.
+void finish_two_phase(pp_program* p, int workers, int F) {
+  auto t0 = std::chrono::steady_clock::now();
+  const int n = p->tree.length(), I = n - F;
+  h_ops_dag full(&p->tree, 0, 0);
+  h_ops_dag dag_f(&p->tree, 0, F), dag_i(&p->tree, F, I);
+  p->dag_ms = ms_since(t0);
+  t0 = std::chrono::steady_clock::now();
+  instructions_scheduler s_f(&dag_f, workers), s_i(&dag_i, workers);
+  p->schedule_ms = ms_since(t0);
+  t0 = std::chrono::steady_clock::now();
+  instructions_manager m_f(workers, &dag_f, &s_f, &p->tree, 0), m_i(workers, &dag_i, &s_i, &p->tree, F);
+  auto im = std::make_unique<pp_program_image>();
+  im->n_insts = n;
+  im->n_edges = full.getEdgeCount();
+  im->flops = full.getFlops();
+  im->dag_density = full.dag_density();
+  im->preds.resize(n);
+  for (int i = 0; i < n; ++i)
+    for (const dependency_linked_list* d = full.getDepList_To(i); d != nullptr; d = d->getNext()) {
+      im->preds[i].push_back(d->getInst());
+      im->preds[i].push_back((int)d->getDep());
+    }
+  im->streams.resize(workers);
+  im->sched.resize(workers);
+  int64_t waits = 0;
+  for (int w = 0; w < workers; ++w) {
+    std::vector<int32_t>& out = im->streams[w];
+    const std::vector<int32_t>& a = m_f.getStream(w);
+    out.assign(a.begin(), a.end() - 1);  // without its finish word
+    int64_t end_f = 0;
+    for (const instructions_queue* q = s_f.getSchedule(w); q != nullptr; q = q->getNext()) {
+      im->sched[w].push_back({q->getInst(), q->getExW() ? 1 : 0, q->getStartingFlops()});
+      end_f = std::max<int64_t>(end_f, q->getStartingFlops());
+      if (!q->getExW()) ++waits;
+    }
+    const std::vector<int32_t>& b = m_i.getStream(w);
+    size_t at = 0;
+    const instructions_queue* q = s_i.getSchedule(w);
+    while (at < b.size() && b[at] != finish) {
+      if (b[at] == signal_wait) {
+        out.push_back(signal_wait);
+        out.push_back(b[at + 1] + F);
+        out.push_back(b[at + 2]);
+        at += wait_l;
+      } else {
+        const int id = b[at + 1] + F, len = operationLength(b[at + 2]);
+        for (const dependency_linked_list* d = full.getDepList_To(id); d != nullptr; d = d->getNext())
+          if (d->getInst() < F) {
+            out.push_back(signal_wait);
+            out.push_back(d->getInst());
+            out.push_back(0);
+            im->sched[w].push_back({d->getInst(), 0, end_f});
+            ++waits;
+          }
+        out.insert(out.end(), b.begin() + (ptrdiff_t)at, b.begin() + (ptrdiff_t)at + len);
+        out[out.size() - len + 1] = id;
+        at += len;
+      }
+      if (q != nullptr) {
+        im->sched[w].push_back({q->getInst() + F, q->getExW() ? 1 : 0, end_f + q->getStartingFlops()});
+        if (!q->getExW()) ++waits;
+        q = q->getNext();
+      }
+    }
+    out.push_back(finish);
+  }
+  im->comm_waits = waits;
+  p->encode_ms = ms_since(t0);
+  p->workers = workers;
+  p->scratch = p->tree.maxScratch();
+  p->image = std::move(im);
+}
+
 }  // namespace
 
 extern "C" {
@@ -364,6 +444,7 @@ int ppc::factor_inverse_program(const pp_hmatrix* const* hs, int count, int work
       p->tree.append(*t);
     }
     p->primary_flops = p->tree.getFlops();
+    const int n_factor = p->tree.length();
     // the inversions follow every factorization in program order, so each
     // depends only on the factor blocks it reads; HODLR-shaped matrices get
     // the full hierarchical inverses, others the leaf inverses only
@@ -376,7 +457,7 @@ int ppc::factor_inverse_program(const pp_hmatrix* const* hs, int count, int work
       p->tree.append(*v);
       p->temp_elems.push_back(tmps[i].size());
     }
-    finish_program(p.get(), workers);
+    finish_two_phase(p.get(), workers, n_factor);
     *out = p.release();
     return PP_OK;
   } catch (...) {

@@ -250,6 +250,23 @@ __device__ __forceinline__ void lu_update(T* M, int ld, int p, int pw, int nx, i
   __syncthreads();
 }
 
+// LDS ordering between waves (panel LU, narrow solve).  The LDS unit executes the operations of one
+// wave in issue order, so (a) lanes of a wave see the wave's earlier LDS
+// writes, and (b) a wave that reads a counter written after some data by
+// another wave also sees that data.  Only compiler reordering has to be
+// prevented, which a signal fence does without waiting on memory counters.
+__device__ __forceinline__ void lds_order() {
+  __atomic_signal_fence(__ATOMIC_SEQ_CST);
+  __builtin_amdgcn_wave_barrier();
+}
+
+// Waits until the LDS counter reaches `target` (written only by wave 0 of the
+// panel, which never waits on the other waves, so the wait always ends).
+__device__ __forceinline__ void lds_wait_ge(volatile PP_LDS int* c, int target) {
+  while (*c < target) __builtin_amdgcn_s_sleep(1);
+  lds_order();
+}
+
 // ---------------------------------------------------------------- narrow solve
 // In-place triangular solve with few right-hand-side columns, the right-hand
 // side resident in LDS for the whole solve:
@@ -302,8 +319,8 @@ __device__ __forceinline__ void tile_from_lds(const T (&fa)[NB / Mma<T, MODE>::K
 // LDS bytes of narrow_solve for m rows and w columns.
 template <typename T>
 __device__ __forceinline__ int narrow_solve_bytes(int m, int w) {
-  const int ntc = (w + 31) >> 5, wp = ntc * 32 + 16 / (int)sizeof(T);
-  return ((m + NB - 1) / NB) * NB * wp * (int)sizeof(T);
+  const int ntc = (w + 31) >> 5, wp = ntc * 32 + 16 / (int)sizeof(T), nblk = (m + NB - 1) / NB;
+  return nblk * NB * wp * (int)sizeof(T) + nblk * (int)sizeof(int);  // W, then one ready flag per block
 }
 
 template <typename T, int MODE>
@@ -330,9 +347,78 @@ __device__ __forceinline__ void narrow_solve(T* Wg, int ldw, bool w_trans, int m
     if (r < m && c < w) v = w_trans ? Wg[(int64_t)c * ldw + r] : Wg[(int64_t)r * ldw + c];
     Ws[r * wp + c] = v;
   }
+  volatile PP_LDS int* ready = (volatile PP_LDS int*)(Ws + mr * wp);
+  for (int e = tid; e < nblk; e += nt) ready[e] = 0;
   __syncthreads();
   constexpr int CH = NB / P::KW, PRE = 2;  // T fragments of up to PRE tiles are requested early
-  for (int s = 0; s < nblk; ++s) {
+  if (forward) {
+    // Row-block ownership: wave wv owns blocks wv, wv + nw, ... and takes
+    // them in increasing order.  For its block b it applies W_b -= T_{b,j}
+    // X_j for j = 0 .. b-1 in order, each as soon as X_j is published (the
+    // T fragments are requested before the wait), then forms X_b = D_b W_b
+    // in place and publishes it (ready[b]).  Every element sees the same
+    // operations in the same order as in the step-synchronous form below;
+    // block 0 waits for nothing and a block only waits for blocks before
+    // it, so every wait ends.
+    for (int b = wv; b < nblk; b += nw) {
+      const int rb = b * NB, bsb = (m - rb) < NB ? (m - rb) : NB;
+      // D_b's fragments are requested first (they wait for nothing)
+      OperandLane<T, P::EPL> ld_[2];
+#pragma unroll
+      for (int h = 0; h < 2; ++h) ld_[h].init(dinv + (int64_t)b * NB * NB, NB, !f_trans, 16 * h + li, bsb);
+      T fd[CH][2][P::EPL];
+      load_a_frags<T, MODE>(ld_, bsb, fd);
+      for (int j = 0; j < b; ++j) {
+        const int r0 = j * NB;  // blocks before the last one are full
+        OperandLane<T, P::EPL> la[2];
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+          if (f_trans) la[h].init(F + (int64_t)r0 * ldf, ldf, false, rb + 16 * h + li, m);
+          else la[h].init(F + r0, ldf, true, rb + 16 * h + li, m);
+        }
+        T fa[CH][2][P::EPL];
+        load_a_frags<T, MODE>(la, NB, fa);
+        lds_wait_ge(ready + j, 1);
+        for (int ct = 0; ct < ntc; ++ct) {
+          const int c0 = ct * 32;
+          acc_t acc[2][2];
+#pragma unroll
+          for (int a = 0; a < 2; ++a) acc[a][0] = acc[a][1] = P::zero();
+          tile_from_lds<T, MODE>(fa, Ws + r0 * wp, wp, c0, acc);
+#pragma unroll
+          for (int a = 0; a < 2; ++a)
+#pragma unroll
+            for (int jj = 0; jj < 2; ++jj)
+#pragma unroll
+              for (int r = 0; r < 4; ++r) {
+                const int row = rb + a * 16 + P::orow(l, r);
+                PP_LDS T* wpx = Ws + row * wp + c0 + jj * 16 + li;
+                if (row < m) *wpx = *wpx - acc[a][jj][r];
+              }
+        }
+      }
+      for (int ct = 0; ct < ntc; ++ct) {
+        const int c0 = ct * 32;
+        acc_t acc[2][2];
+#pragma unroll
+        for (int a = 0; a < 2; ++a) acc[a][0] = acc[a][1] = P::zero();
+        tile_from_lds<T, MODE>(fd, Ws + rb * wp, wp, c0, acc);
+#pragma unroll
+        for (int a = 0; a < 2; ++a)
+#pragma unroll
+          for (int jj = 0; jj < 2; ++jj)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+              const int row = a * 16 + P::orow(l, r);
+              Ws[(rb + row) * wp + c0 + jj * 16 + li] = row < bsb ? acc[a][jj][r] : T(0);
+            }
+      }
+      lds_order();
+      if (l == 0) ready[b] = 1;
+    }
+    __syncthreads();
+  }
+  for (int s = 0; s < (forward ? 0 : nblk); ++s) {
     const int i = forward ? s : nblk - 1 - s;
     const int r0 = i * NB, bs = (m - r0) < NB ? (m - r0) : NB;
     const int q_lo = forward ? r0 + NB : 0, q_hi = forward ? mr : r0;
@@ -426,23 +512,6 @@ __device__ __forceinline__ void narrow_solve(T* Wg, int ldw, bool w_trans, int m
 }
 
 // ---------------------------------------------------------------- getrf
-// LDS ordering inside the panel.  The LDS unit executes the operations of one
-// wave in issue order, so (a) lanes of a wave see the wave's earlier LDS
-// writes, and (b) a wave that reads a counter written after some data by
-// another wave also sees that data.  Only compiler reordering has to be
-// prevented, which a signal fence does without waiting on memory counters.
-__device__ __forceinline__ void lds_order() {
-  __atomic_signal_fence(__ATOMIC_SEQ_CST);
-  __builtin_amdgcn_wave_barrier();
-}
-
-// Waits until the LDS counter reaches `target` (written only by wave 0 of the
-// panel, which never waits on the other waves, so the wait always ends).
-__device__ __forceinline__ void lds_wait_ge(volatile PP_LDS int* c, int target) {
-  while (*c < target) __builtin_amdgcn_s_sleep(1);
-  lds_order();
-}
-
 // Value v of lane `lane` on every lane (v_readlane into scalar registers,
 // which an FMA can take as an operand).
 template <typename T>

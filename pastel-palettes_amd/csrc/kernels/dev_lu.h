@@ -350,17 +350,20 @@ __device__ __forceinline__ void narrow_solve(T* Wg, int ldw, bool w_trans, int m
   volatile PP_LDS int* ready = (volatile PP_LDS int*)(Ws + mr * wp);
   for (int e = tid; e < nblk; e += nt) ready[e] = 0;
   __syncthreads();
-  constexpr int CH = NB / P::KW, PRE = 2;  // T fragments of up to PRE tiles are requested early
-  if (forward) {
-    // Row-block ownership: wave wv owns blocks wv, wv + nw, ... and takes
-    // them in increasing order.  For its block b it applies W_b -= T_{b,j}
-    // X_j for j = 0 .. b-1 in order, each as soon as X_j is published (the
-    // T fragments are requested before the wait), then forms X_b = D_b W_b
-    // in place and publishes it (ready[b]).  Every element sees the same
-    // operations in the same order as in the step-synchronous form below;
-    // block 0 waits for nothing and a block only waits for blocks before
-    // it, so every wait ends.
-    for (int b = wv; b < nblk; b += nw) {
+  constexpr int CH = NB / P::KW;
+  {
+    // Row-block ownership: wave wv owns the blocks at distance wv, wv + nw,
+    // ... from the first block of the solve order (increasing for forward,
+    // decreasing for backward) and takes them in that order.  For its block
+    // b it applies W_b -= T_{b,j} X_j for the blocks j solved before b, in
+    // solve order, each as soon as X_j is published (the T fragments are
+    // requested before the wait), then forms X_b = D_b W_b in place and
+    // publishes it (ready[b]).  Every element sees the same operations in
+    // the same order as in the step-synchronous form; the first block waits
+    // for nothing and a block only waits for blocks solved before it, so
+    // every wait ends.
+    for (int ob = wv; ob < nblk; ob += nw) {
+      const int b = forward ? ob : nblk - 1 - ob;
       const int rb = b * NB, bsb = (m - rb) < NB ? (m - rb) : NB;
       // D_b's fragments are requested first (they wait for nothing)
       OperandLane<T, P::EPL> ld_[2];
@@ -368,8 +371,9 @@ __device__ __forceinline__ void narrow_solve(T* Wg, int ldw, bool w_trans, int m
       for (int h = 0; h < 2; ++h) ld_[h].init(dinv + (int64_t)b * NB * NB, NB, !f_trans, 16 * h + li, bsb);
       T fd[CH][2][P::EPL];
       load_a_frags<T, MODE>(ld_, bsb, fd);
-      for (int j = 0; j < b; ++j) {
-        const int r0 = j * NB;  // blocks before the last one are full
+      for (int oj = 0; oj < ob; ++oj) {
+        const int j = forward ? oj : nblk - 1 - oj;
+        const int r0 = j * NB, bsj = (m - r0) < NB ? (m - r0) : NB;
         OperandLane<T, P::EPL> la[2];
 #pragma unroll
         for (int h = 0; h < 2; ++h) {
@@ -377,7 +381,7 @@ __device__ __forceinline__ void narrow_solve(T* Wg, int ldw, bool w_trans, int m
           else la[h].init(F + r0, ldf, true, rb + 16 * h + li, m);
         }
         T fa[CH][2][P::EPL];
-        load_a_frags<T, MODE>(la, NB, fa);
+        load_a_frags<T, MODE>(la, bsj, fa);
         lds_wait_ge(ready + j, 1);
         for (int ct = 0; ct < ntc; ++ct) {
           const int c0 = ct * 32;
@@ -415,86 +419,6 @@ __device__ __forceinline__ void narrow_solve(T* Wg, int ldw, bool w_trans, int m
       }
       lds_order();
       if (l == 0) ready[b] = 1;
-    }
-    __syncthreads();
-  }
-  for (int s = 0; s < (forward ? 0 : nblk); ++s) {
-    const int i = forward ? s : nblk - 1 - s;
-    const int r0 = i * NB, bs = (m - r0) < NB ? (m - r0) : NB;
-    const int q_lo = forward ? r0 + NB : 0, q_hi = forward ? mr : r0;
-    const int nrt = (q_hi - q_lo) >> 5, ntiles = nrt * ntc;
-    // the rest update's T fragments do not depend on X_i: request them
-    // before X_i is formed
-    auto t_lanes = [&](int t, OperandLane<T, P::EPL> (&la)[2]) {
-      const int q0 = q_lo + (t / ntc) * 32;
-#pragma unroll
-      for (int h = 0; h < 2; ++h) {
-        if (f_trans) la[h].init(F + (int64_t)r0 * ldf, ldf, false, q0 + 16 * h + li, m);
-        else la[h].init(F + r0, ldf, true, q0 + 16 * h + li, m);
-      }
-    };
-    T pre[PRE][CH][2][P::EPL];
-#pragma unroll
-    for (int q = 0; q < PRE; ++q) {
-      const int t = wv + q * nw;
-      if (t < ntiles) {
-        OperandLane<T, P::EPL> la[2];
-        t_lanes(t, la);
-        load_a_frags<T, MODE>(la, bs, pre[q]);
-      }
-    }
-    // X_i = D_i W_i, one column tile per wave, in place
-    if (wv < ntc) {
-      const int c0 = wv * 32;
-      OperandLane<T, P::EPL> la[2];
-#pragma unroll
-      for (int h = 0; h < 2; ++h) la[h].init(dinv + (int64_t)i * NB * NB, NB, !f_trans, 16 * h + li, bs);
-      T fa[CH][2][P::EPL];
-      load_a_frags<T, MODE>(la, bs, fa);
-      acc_t acc[2][2];
-#pragma unroll
-      for (int a = 0; a < 2; ++a) acc[a][0] = acc[a][1] = P::zero();
-      tile_from_lds<T, MODE>(fa, Ws + r0 * wp, wp, c0, acc);
-#pragma unroll
-      for (int a = 0; a < 2; ++a)
-#pragma unroll
-        for (int j = 0; j < 2; ++j)
-#pragma unroll
-          for (int r = 0; r < 4; ++r) {
-            const int row = a * 16 + P::orow(l, r);
-            Ws[(r0 + row) * wp + c0 + j * 16 + li] = row < bs ? acc[a][j][r] : T(0);
-          }
-    }
-    __syncthreads();
-    // W_rest -= T_{rest,i} X_i
-    auto update = [&](int t, const T (&fa)[CH][2][P::EPL]) {
-      const int q0 = q_lo + (t / ntc) * 32, c0 = (t % ntc) * 32;
-      acc_t acc[2][2];
-#pragma unroll
-      for (int a = 0; a < 2; ++a) acc[a][0] = acc[a][1] = P::zero();
-      tile_from_lds<T, MODE>(fa, Ws + r0 * wp, wp, c0, acc);
-#pragma unroll
-      for (int a = 0; a < 2; ++a)
-#pragma unroll
-        for (int j = 0; j < 2; ++j)
-#pragma unroll
-          for (int r = 0; r < 4; ++r) {
-            const int row = q0 + a * 16 + P::orow(l, r);
-            PP_LDS T* wpx = Ws + row * wp + c0 + j * 16 + li;
-            if (row < m) *wpx = *wpx - acc[a][j][r];
-          }
-    };
-#pragma unroll
-    for (int q = 0; q < PRE; ++q) {
-      const int t = wv + q * nw;
-      if (t < ntiles) update(t, pre[q]);
-    }
-    for (int t = wv + PRE * nw; t < ntiles; t += nw) {
-      OperandLane<T, P::EPL> la[2];
-      t_lanes(t, la);
-      T fa[CH][2][P::EPL];
-      load_a_frags<T, MODE>(la, bs, fa);
-      update(t, fa);
     }
     __syncthreads();
   }

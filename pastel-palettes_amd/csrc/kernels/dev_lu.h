@@ -450,6 +450,21 @@ __device__ __forceinline__ T bcast_lane(T v, int lane) {
   }
 }
 
+// Stores rows [0, nr) x columns [0, pw) of a wave's LDS tile S (pitch NB + 1)
+// to G (pitch ld).  Lanes 0-31 and 32-63 each write one row of consecutive
+// elements, so a store instruction covers two contiguous row segments instead
+// of 64 elements in 64 different rows (one row per lane, as the elimination
+// holds them).  Called by one wave; S was written by the same wave.
+template <typename T>
+__device__ __forceinline__ void wave_store_rows(T* G, int64_t ld, const PP_LDS T* S, int nr, int pw) {
+  lds_order();
+  const int l = lane_id(), c = l & 31;
+#pragma unroll 4
+  for (int i = l >> 5; i < nr; i += 2)
+    if (c < pw) G[i * ld + c] = S[i * (NB + 1) + c];
+  lds_order();
+}
+
 // Unblocked LU of the panel M[p:ny, p:p+pw] (pw <= NB) in place, run by three
 // roles concurrently (no workgroup barrier inside):
 //   wave 0     factors the pw x pw diagonal block.  Lane i owns row i in
@@ -465,14 +480,20 @@ __device__ __forceinline__ T bcast_lane(T v, int lane) {
 //              pivot rows as soon as it is published.
 // Multipliers are formed with the reciprocal of the pivot, which the owner of
 // the next pivot row computes right after its pivot element is updated.
+// Results leave through a per-wave LDS tile (wave_store_rows).
 template <typename T>
 __device__ __forceinline__ void panel_lu(T* M, int ld, int p, int pw, int ny, T* linv) {
   constexpr int UP = NB + 2;              // pitch of U: 16-byte aligned rows
   constexpr int GRP = 8;                  // rows per publication
+  constexpr int SP = NB + 1;              // pitch of the output tiles
   PP_LDS T* U = (PP_LDS T*)lds_base();    // [NB][UP]: final rows, column NB = 1 / u_jj
   PP_LDS T* Lc = U + NB * UP;              // [NB][NB]: multiplier column j at Lc[j * NB + i]
   volatile PP_LDS int* cols = (volatile PP_LDS int*)(Lc + NB * NB);
   volatile PP_LDS int* rows = cols + 1;
+  // output tiles: waves 0 and 1 32 rows each, waves 2 and 3 128 rows each
+  PP_LDS T* tiles = (PP_LDS T*)(lds_base() + 17 * 1024);
+  static_assert((NB * UP + NB * NB) * sizeof(T) + 8 <= 17 * 1024, "panel LDS layout");
+  static_assert(17 * 1024 + (2 * NB + 256) * SP * sizeof(T) <= kLdsBytes - 16, "panel LDS layout");
   const int tid = threadIdx.x, l = lane_id(), w = wave_id();
   const int R = ny - p;
   if (tid == 0) {
@@ -521,11 +542,13 @@ __device__ __forceinline__ void panel_lu(T* M, int ld, int p, int pw, int ny, T*
         }
       }
     }
+    PP_LDS T* S = tiles;
     if (own) {
 #pragma unroll
       for (int c = 0; c < NB; ++c)
-        if (c < pw) row[c] = x[c];
+        if (c < pw) S[l * SP + c] = x[c];
     }
+    wave_store_rows(M + (int64_t)p * ld + p, ld, S, pw, pw);
   } else if (w == 1) {
     const bool own = l < pw;
     T e[NB];
@@ -541,11 +564,13 @@ __device__ __forceinline__ void panel_lu(T* M, int ld, int p, int pw, int ny, T*
         for (int c = 0; c <= j; ++c) e[c] -= lij * bcast_lane(e[c], j);  // row j of L11^{-1} is final
       }
     }
+    PP_LDS T* S = tiles + NB * SP;
     if (own) {
 #pragma unroll
       for (int c = 0; c < NB; ++c)
-        if (c < pw) linv[l * NB + c] = e[c];
+        if (c < pw) S[l * SP + c] = e[c];
     }
+    wave_store_rows(linv, NB, S, pw, pw);
   } else {
     const int nb_t = blockDim.x - 128, bt = tid - 128;
     for (int r0 = pw; r0 < R; r0 += 2 * nb_t) {
@@ -583,11 +608,18 @@ __device__ __forceinline__ void panel_lu(T* M, int ld, int p, int pw, int ny, T*
           }
         }
       }
+      // this wave's rows: r0 + off + [0, 64) and r0 + nb_t + off + [0, 64)
+      PP_LDS T* S = tiles + (2 * NB + (w - 2) * 128) * SP;
 #pragma unroll
       for (int c = 0; c < NB; ++c) {
-        if (oa && c < pw) pa[c] = xa[c];
-        if (ob && c < pw) pb[c] = xb[c];
+        if (c < pw) {
+          S[l * SP + c] = xa[c];
+          S[(64 + l) * SP + c] = xb[c];
+        }
       }
+      const int a0 = r0 + (bt - l), b0 = a0 + nb_t;
+      wave_store_rows(M + (int64_t)(p + a0) * ld + p, ld, S, min(64, R - a0), pw);
+      wave_store_rows(M + (int64_t)(p + b0) * ld + p, ld, S + 64 * SP, min(64, R - b0), pw);
     }
   }
   __syncthreads();

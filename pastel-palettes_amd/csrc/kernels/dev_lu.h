@@ -101,6 +101,184 @@ __device__ __forceinline__ void diag_inverses(const T* F, int ldf, int n, T* out
 }
 
 // ---------------------------------------------------------------- panel update
+// Pitch of the LDS operand tiles of lu_update_lds: rows of NB elements padded
+// so that the 16-byte fragment reads of eight consecutive lanes fall in
+// distinct banks.
+template <typename T>
+constexpr int lu_lds_pitch() { return NB + (sizeof(T) == 8 ? 2 : 4); }
+
+// Whether lu_update_lds holds L21 and U12 of an rm x rn trailing block in LDS.
+template <typename T>
+__device__ __forceinline__ bool lu_update_fits(int rm, int rn) {
+  const int rows = ((rm + 31) & ~31) + ((rn + 31) & ~31);
+  return (int64_t)rows * lu_lds_pitch<T>() * sizeof(T) <= kLdsBytes - 1024;
+}
+
+// U12 = L11^{-1} A12 and A22 -= L21 U12 with both operands of the trailing
+// product in LDS (rm, rn > 0; lu_update_fits):
+//   1. the workgroup copies L21 (rows padded to whole tiles, columns past pw
+//      and rows past rm zero) into LDS with coalesced loads, while each wave
+//      forms U12 tiles (32 columns each; stored to A12 and, transposed, to
+//      LDS with rows past pw zero);
+//   2. after one barrier the rm x rn tiles of A22 are dealt round-robin to
+//      the waves (better balanced than whole columns); a wave keeps the C
+//      values of two tiles in flight and reads both fragments from LDS.
+// Same products and summation order per element as lu_update's.  At 256 x
+// 256 the trailing products run at the FP64 MFMA's issue rate (about 147
+// cycles per 16x16x4 instruction measured on one CU, so about 2 us per 32 x
+// 32 x 32 tile).
+template <typename T, int MODE>
+__device__ __forceinline__ void lu_update_lds(T* M, int ld, int p, int pw, int nx, int ny, const T* linv) {
+  using P = Mma<T, MODE>;
+  using acc_t = typename P::acc_t;
+  constexpr int KW = P::KW, EPL = P::EPL, CH = NB / KW, PA = lu_lds_pitch<T>();
+  static_assert(EPL * sizeof(T) % 16 == 0 && NB == 32, "fragment reads are whole 16-byte vectors");
+  const int rn = nx - p - pw, rm = ny - p - pw;
+  const int nw = blockDim.x >> 6, w = wave_id(), l = lane_id(), g = l >> 4, li = l & 15;
+  const int ctn = (rn + 31) >> 5, rtn = (rm + 31) >> 5;
+  T* A12 = M + (int64_t)p * ld + p + pw;
+  const T* L21 = M + (int64_t)(p + pw) * ld + p;
+  T* A22 = M + (int64_t)(p + pw) * ld + p + pw;
+  PP_LDS T* La = (PP_LDS T*)lds_base();  // [rtn * 32][PA]: L21
+  PP_LDS T* Ub = La + rtn * 32 * PA;      // [ctn * 32][PA]: U12 transposed
+  const int nt = rtn * ctn;               // trailing tiles t = ct * rtn + rt
+  auto load_c = [&](int t, T (&cv)[2][2][4]) {
+    const int r0 = (t % rtn) * 32, c0 = (t / rtn) * 32;
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+      for (int j = 0; j < 2; ++j)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int row = r0 + a * 16 + P::orow(l, r), col = c0 + j * 16 + li;
+          cv[a][j][r] = gload<T>(A22 + (int64_t)(row < rm ? row : rm - 1) * ld + (col < rn ? col : rn - 1));
+        }
+  };
+  // 1a. L21 -> LDS: 16-byte loads, NB / V threads per row, all in flight
+  {
+    constexpr int V = 16 / sizeof(T), TPR = NB / V;
+    typedef T vec_t __attribute__((ext_vector_type(V)));
+    const bool vec = pw == NB && (ld % V) == 0 && ((reinterpret_cast<uintptr_t>(M) & 15) == 0) && (p % V) == 0;
+    const int k0 = (threadIdx.x % TPR) * V, rstep = blockDim.x / TPR;
+#pragma unroll 16
+    for (int r = threadIdx.x / TPR; r < rtn * 32; r += rstep) {
+      vec_t x;
+      if (vec) {
+        x = gload<vec_t>(L21 + (int64_t)(r < rm ? r : rm - 1) * ld + k0);
+      } else {
+#pragma unroll
+        for (int e = 0; e < V; ++e)
+          x[e] = gload<T>(L21 + (int64_t)(r < rm ? r : rm - 1) * ld + (k0 + e < pw ? k0 + e : pw - 1));
+      }
+#pragma unroll
+      for (int e = 0; e < V; ++e)
+        if (r >= rm || k0 + e >= pw) x[e] = T(0);
+      *(PP_LDS vec_t*)(La + r * PA + k0) = x;
+    }
+  }
+  // the C values of the wave's first tile are requested before the U12
+  // products (phase 1 does not touch A22); past the last tile the last one
+  // is requested again
+  T cv0[2][2][4], cv1[2][2][4];
+  const int tl = nt - 1;
+  load_c(w < nt ? w : tl, cv0);
+  // 1b. U12 tiles
+  for (int ct = w; ct < ctn; ct += nw) {
+    const int c0 = ct * 32;
+    OperandLane<T, EPL> la[2], lb[2];
+    T fa[CH][2][EPL], fb[CH][2][EPL];
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      la[h].init(linv, NB, true, 16 * h + li, pw);
+      lb[h].init(A12, ld, false, c0 + 16 * h + li, rn);
+    }
+#pragma unroll
+    for (int c = 0; c < CH; ++c)
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        la[h].template load<false>(fa[c][h], c * KW + EPL * g, pw);
+        lb[h].template load<false>(fb[c][h], c * KW + EPL * g, pw);
+#pragma unroll
+        for (int e = 0; e < EPL; ++e)  // k >= pw: the clamped loads cancel through A
+          if (c * KW + EPL * g + e >= pw) fa[c][h][e] = T(0);
+      }
+    acc_t acc[2][2];
+#pragma unroll
+    for (int a = 0; a < 2; ++a) acc[a][0] = acc[a][1] = P::zero();
+#pragma unroll
+    for (int c = 0; c < CH; ++c)
+#pragma unroll
+      for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) acc[a][j] = P::mma_chunk(fa[c][a], fb[c][j], acc[a][j]);
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+      for (int j = 0; j < 2; ++j)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int row = a * 16 + P::orow(l, r), col = j * 16 + li;
+          const T v = acc[a][j][r];
+          Ub[(c0 + col) * PA + row] = row < pw ? v : T(0);
+          if (row < pw && c0 + col < rn) gstore<T>(A12 + (int64_t)row * ld + c0 + col, v);
+        }
+  }
+  __syncthreads();
+  load_c(w + nw < nt ? w + nw : tl, cv1);
+  // 2. trailing tiles
+  auto apply = [&](int t, const T (&cv)[2][2][4]) {
+    const int r0 = (t % rtn) * 32, c0 = (t / rtn) * 32;
+    typedef T vec_t __attribute__((ext_vector_type(16 / sizeof(T))));
+    constexpr int V = 16 / sizeof(T);
+    T fa[CH][2][EPL], fb[CH][2][EPL];
+#pragma unroll
+    for (int c = 0; c < CH; ++c)
+#pragma unroll
+      for (int h = 0; h < 2; ++h)
+#pragma unroll
+        for (int v = 0; v < EPL / V; ++v) {
+          const int k = c * KW + EPL * g + v * V;
+          const vec_t x = *(const PP_LDS vec_t*)(La + (r0 + 16 * h + li) * PA + k);
+          const vec_t y = *(const PP_LDS vec_t*)(Ub + (c0 + 16 * h + li) * PA + k);
+#pragma unroll
+          for (int e = 0; e < V; ++e) {
+            fa[c][h][v * V + e] = x[e];
+            fb[c][h][v * V + e] = y[e];
+          }
+        }
+    acc_t acc[2][2];
+#pragma unroll
+    for (int a = 0; a < 2; ++a) acc[a][0] = acc[a][1] = P::zero();
+#pragma unroll
+    for (int c = 0; c < CH; ++c)
+#pragma unroll
+      for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) acc[a][j] = P::mma_chunk(fa[c][a], fb[c][j], acc[a][j]);
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+      for (int j = 0; j < 2; ++j)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int row = r0 + a * 16 + P::orow(l, r), col = c0 + j * 16 + li;
+          if (row < rm && col < rn) gstore<T>(A22 + (int64_t)row * ld + col, cv[a][j][r] - acc[a][j][r]);
+        }
+  };
+  // two tiles in flight: a buffer is refilled right after its tile's stores
+  // are issued, so the wait for the next tile's values never includes the
+  // stores before it (the memory counter retires in issue order)
+  for (int t = w; t < nt; t += 2 * nw) {
+    apply(t, cv0);
+    load_c(t + 2 * nw < nt ? t + 2 * nw : tl, cv0);
+    if (t + nw < nt) {
+      apply(t + nw, cv1);
+      load_c(t + 3 * nw < nt ? t + 3 * nw : tl, cv1);
+    }
+  }
+  __syncthreads();
+}
+
 // After the panel at p (width pw, multipliers L21 below, L11^{-1} in linv):
 // U12 = L11^{-1} A12 and A22 -= L21 U12 in one step.  Each wave owns column
 // tiles of U12: it forms the 32-column tile (stored to A12), keeps it as the
@@ -119,6 +297,14 @@ __device__ __forceinline__ void lu_update(T* M, int ld, int p, int pw, int nx, i
   T* A12 = M + (int64_t)p * ld + p + pw;
   const T* L21 = M + (int64_t)(p + pw) * ld + p;
   T* A22 = M + (int64_t)(p + pw) * ld + p + pw;
+  // FP64 only: in FP32 the register-resident version below is as fast and
+  // the LDS version's registers make the factor kernel spill
+  if constexpr (sizeof(T) == 8) {
+    if (rm > 0 && rn > 0 && lu_update_fits<T>(rm, rn)) {
+      lu_update_lds<T, MODE>(M, ld, p, pw, nx, ny, linv);
+      return;
+    }
+  }
   PP_LDS T* ut = (PP_LDS T*)lds_base() + w * NB * UT;
   const bool lvec = pw == NB && (ld % V) == 0 && ((reinterpret_cast<uintptr_t>(M) & 15) == 0) && (p % V) == 0;
   for (int ct = w; ct < ctn; ct += nw) {

@@ -322,6 +322,32 @@ int pp_hmatrix_build_kernel(pp_ctx* ctx, const pp_hmatrix* h, int kind, const do
 /* Length of the pivot array of pp_hmatrix_build_kernel (2 x sum of ranks). */
 int64_t pp_hmatrix_build_pivots(const pp_hmatrix* h);
 
+/* ------------------------------------------------------------------ pivoting
+ * Partial-pivoted LU for dense blocks that are not diagonally dominant (the
+ * H-LU path factors its diagonal leaves without pivoting).  Replaces the
+ * reference's pivot helpers, src/kernel.cu:958-1051 (warpReduceMax_Index,
+ * blockReduceMax_Index, blockSwapRows, blockApplyPivot, resetPivot), which
+ * no opcode of the reference dispatches.
+ *
+ * pp_dense_getrf_pivoted: `count` row-major ny x nx blocks (device memory,
+ * pitch ld, `stride` elements apart) are factored in place, P A = L U (L unit
+ * lower, U upper).  pivots (device, count x ny) receives each block's
+ * permutation in the reference's convention: row i of P A is row p[i] of A.
+ * Column j's pivot is its largest |a_ij|, i >= j, the smallest row on ties.
+ * info (device, count values): 0, or 1 + the first column whose pivot is
+ * zero (that column is left as it is).  precision PP_FP64 or PP_FP32 (the
+ * block's element type).  Asynchronous on hip_stream (NULL = the context's
+ * stream); one 256-thread workgroup per block. */
+int pp_dense_getrf_pivoted(pp_ctx* ctx, int precision, void* a, int ld, int ny, int nx, int count, int64_t stride,
+                           int32_t* pivots, int32_t* info, void* hip_stream);
+/* Rows of the row-major ny x nx block m (device, pitch ld) permuted in place
+ * by pivots (device, ny entries): row r <- row p[r] (recover = 0), or row p[r]
+ * <- row r (recover = 1, the inverse) -- blockApplyPivot, src/kernel.cu:
+ * 1023-1043.  pivots is copied to the host and checked to be a permutation
+ * first (the call waits for hip_stream); ny * element size <= 48 KiB. */
+int pp_dense_apply_pivot(pp_ctx* ctx, int precision, void* m, int ld, int nx, int ny, const int32_t* pivots,
+                         int recover, void* hip_stream);
+
 /* ------------------------------------------------------------------ byte-string format
  * H-matrices, instruction programs and factorizations as byte strings, for
  * files, caches and the wire.  No counterpart in the reference, which has no

@@ -53,6 +53,10 @@ def lib():
         L.or_exec.restype = ip
         L.or_run_streams.argtypes = [C.POINTER(C.POINTER(i32)), ip, C.POINTER(dp), dp, C.c_int64, dp, ip, ip]
         L.or_run_streams.restype = ip
+        L.or_getrf_pivoted.argtypes = [dp, ip, ip, ip, C.POINTER(i32)]
+        L.or_getrf_pivoted.restype = ip
+        L.or_getrf_pivoted_f32.argtypes = [C.POINTER(C.c_float), ip, ip, ip, C.POINTER(i32)]
+        L.or_getrf_pivoted_f32.restype = ip
         L.or_dense_lu.argtypes = [dp, ip, ip]
         L.or_dense_lu_solve.argtypes = [dp, ip, dp, ip, ip]
         L.or_dense_residual.argtypes = [dp, ip, dp, dp, ip, ip]
@@ -260,6 +264,21 @@ def build_arena(leaves, storage, kind, pts, param, shift):
             arena[lf["off_v"]:lf["off_v"] + nx * r] = v.ravel()
             pivs.append(piv)
     return arena, (np.concatenate(pivs) if pivs else np.zeros((0, 2), np.int32))
+
+
+def getrf_pivoted(a):
+    """(lu, p, info): partial-pivoted LU of the row-major block a (float64 or
+    float32, computed in that precision); row i of P a is row p[i] of a."""
+    a = np.array(a, copy=True, order="C")
+    ny, nx = a.shape
+    p = np.zeros(max(ny, 1), np.int32)
+    pp = p.ctypes.data_as(C.POINTER(C.c_int32))
+    if a.dtype == np.float32:
+        info = lib().or_getrf_pivoted_f32(a.ctypes.data_as(C.POINTER(C.c_float)), ny, nx, nx, pp)
+    else:
+        a = a.astype(np.float64)
+        info = lib().or_getrf_pivoted(_d(a), ny, nx, nx, pp)
+    return a, p[:ny], info
 
 
 def dense_lu(a, threads=8):

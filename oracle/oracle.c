@@ -753,3 +753,61 @@ int or_truncate(double* U, double* V, int ny, int nx, int kp, int r, double* Uo,
   free(w);
   return 0;
 }
+
+/* ---------------------------------------------------------------- pivoting
+ * Partial-pivoted LU of a row-major ny x nx block (pitch ld) in place:
+ * P A = L U, L unit lower (multipliers below the diagonal), U upper.
+ * The pivot vector is a permutation in the reference's convention
+ * (blockApplyPivot, src/kernel.cu:1023-1043: row i of P A is row p[i] of A;
+ * resetPivot, src/kernel.cu:1046-1050, starts it as the identity).  Column j's
+ * pivot is the row of largest |a_ij|, i >= j (blockReduceMax_Index,
+ * src/kernel.cu:974-1005); ties go to the smallest row index (the
+ * reference's reduction breaks ties by lane, which depends on its block
+ * size).  Rows are exchanged whole (blockSwapRows, src/kernel.cu:1008-1012).
+ * Multipliers are l = a_ij / u_jj (correctly rounded division) and each
+ * trailing element takes one fused multiply-add per step, a_ik <- fma(-l,
+ * u_jk, a_ik), so any implementation that keeps those operations is
+ * bit-identical.  A zero pivot column is left as it is (every entry at or
+ * below the diagonal is zero, so the update would change nothing).
+ * Returns 0, or 1 + the first column whose pivot is zero. */
+#define OR_GETRF_PIVOTED(NAME, T, FMA, ABS)                                            \
+  int NAME(T* A, int ny, int nx, int ld, int32_t* p) {                                 \
+    int info = 0;                                                                      \
+    const int K = ny < nx ? ny : nx;                                                   \
+    for (int i = 0; i < ny; ++i) p[i] = i;                                             \
+    for (int j = 0; j < K; ++j) {                                                      \
+      int best = j;                                                                    \
+      T bv = ABS(A[(int64_t)j * ld + j]);                                              \
+      for (int i = j + 1; i < ny; ++i) {                                               \
+        const T v = ABS(A[(int64_t)i * ld + j]);                                       \
+        if (v > bv) {                                                                  \
+          bv = v;                                                                      \
+          best = i;                                                                    \
+        }                                                                              \
+      }                                                                                \
+      if (best != j) {                                                                 \
+        for (int k = 0; k < nx; ++k) {                                                 \
+          const T t = A[(int64_t)j * ld + k];                                          \
+          A[(int64_t)j * ld + k] = A[(int64_t)best * ld + k];                          \
+          A[(int64_t)best * ld + k] = t;                                               \
+        }                                                                              \
+        const int32_t t = p[j];                                                        \
+        p[j] = p[best];                                                                \
+        p[best] = t;                                                                   \
+      }                                                                                \
+      const T piv = A[(int64_t)j * ld + j];                                            \
+      if (piv == (T)0) {                                                               \
+        if (info == 0) info = j + 1;                                                   \
+        continue;                                                                      \
+      }                                                                                \
+      for (int i = j + 1; i < ny; ++i) {                                               \
+        T* row = &A[(int64_t)i * ld];                                                  \
+        const T l = row[j] / piv;                                                      \
+        row[j] = l;                                                                    \
+        for (int k = j + 1; k < nx; ++k) row[k] = FMA(-l, A[(int64_t)j * ld + k], row[k]); \
+      }                                                                                \
+    }                                                                                  \
+    return info;                                                                       \
+  }
+OR_GETRF_PIVOTED(or_getrf_pivoted, double, fma, fabs)
+OR_GETRF_PIVOTED(or_getrf_pivoted_f32, float, fmaf, fabsf)

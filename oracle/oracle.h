@@ -85,6 +85,11 @@ void or_aca_dense(const double* A, int ny, int nx, int r, double* U, double* V, 
  * (nx x r) by the device builder's algorithm; 1 when a Cholesky pivot fails. */
 int or_truncate(double* U, double* V, int ny, int nx, int kp, int r, double* Uo, double* Vo);
 
+/* Partial-pivoted LU (row-major ny x nx, pitch ld) in place; p: permutation,
+   row i of P A is row p[i] of A.  Returns 0 or 1 + first zero-pivot column. */
+int or_getrf_pivoted(double* A, int ny, int nx, int ld, int32_t* p);
+int or_getrf_pivoted_f32(float* A, int ny, int nx, int ld, int32_t* p);
+
 #ifdef __cplusplus
 }
 #endif

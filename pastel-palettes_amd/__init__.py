@@ -127,6 +127,8 @@ def lib():
         "pp_solver_last_factor_ms": (dp, [vp]),
         "pp_hmatrix_build_kernel": (C.c_int, [vp, vp, C.c_int, vp, dp, dp, vp, vp, vp]),
         "pp_hmatrix_build_pivots": (i64, [vp]),
+        "pp_dense_getrf_pivoted": (C.c_int, [vp, C.c_int, vp, C.c_int, C.c_int, C.c_int, C.c_int, i64, vp, vp, vp]),
+        "pp_dense_apply_pivot": (C.c_int, [vp, C.c_int, vp, C.c_int, C.c_int, C.c_int, vp, C.c_int, vp]),
         "pp_hmatrix_encode": (i64, [vp, P(dp), vp, u64]),
         "pp_hmatrix_decode": (C.c_int, [vp, u64, P(dp), u64, P(C.c_int), P(vp)]),
         "pp_program_encode": (i64, [vp, vp, u64]),
@@ -519,6 +521,17 @@ class Context:
         check(lib().pp_mfma_probe(self.h, mode, _p(a, C.c_double), a.size, _p(b, C.c_double), b.size,
                                   _p(out, C.c_double)))
         return out
+
+    def getrf_pivoted(self, precision, a_ptr, ld, ny, nx, pivots_ptr, info_ptr, count=1, stride=0, stream=None):
+        """Partial-pivoted LU of `count` row-major ny x nx blocks in device
+        memory (pp_dense_getrf_pivoted); asynchronous on `stream`."""
+        check(lib().pp_dense_getrf_pivoted(self.h, precision, C.c_void_p(int(a_ptr)), ld, ny, nx, count, stride,
+                                           C.c_void_p(int(pivots_ptr)), C.c_void_p(int(info_ptr)), stream))
+
+    def apply_pivot(self, precision, m_ptr, ld, nx, ny, pivots_ptr, recover=False, stream=None):
+        """Rows of a device block permuted by a pivot vector (pp_dense_apply_pivot)."""
+        check(lib().pp_dense_apply_pivot(self.h, precision, C.c_void_p(int(m_ptr)), ld, nx, ny,
+                                         C.c_void_p(int(pivots_ptr)), int(bool(recover)), stream))
 
     def close(self):
         if getattr(self, "h", None):

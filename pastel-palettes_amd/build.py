@@ -28,8 +28,9 @@ HOST_SOURCES = [
     "runtime/solver.cpp",
     "io/wire.cpp",
     "runtime/build.cpp",
+    "runtime/pivot.cpp",
 ]
-HIP_SOURCES = ["kernels/vm.hip", "kernels/build.hip"]
+HIP_SOURCES = ["kernels/vm.hip", "kernels/build.hip", "kernels/pivot.hip"]
 
 CXXFLAGS = ["-O2", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-function", f"-I{ROCM}/include",
             "-D__HIP_PLATFORM_AMD__"]

@@ -96,6 +96,17 @@ hipError_t launch_aca(int kind, const AcaLeaf* leaves, AcaState* st, int n_leave
 hipError_t launch_truncate(TruncLeaf* leaves, int n_leaves, int max_kp, TruncFactor* factors,
                            const TruncSlice* slices, int n_slices, int* flag, hipStream_t s);
 
+// partial-pivoted LU of `count` row-major ny x nx blocks (stride elements
+// apart); pivots: count x ny permutations, info: count values
+hipError_t launch_getrf_pivoted(int fp64, void* a, int ld, int ny, int nx, int count, int64_t stride,
+                                int32_t* pivots, int32_t* info, hipStream_t s);
+// rows of M (ny x nx) permuted by pivots (recover: by the inverse); M's
+// column chunks pass through LDS, ny * element size <= kPivotLdsBytes
+constexpr int kPivotLdsBytes = 48 * 1024;
+int apply_pivot_chunk(int fp64, int ny);
+hipError_t launch_apply_pivot(int fp64, void* m, int ld, int nx, int ny, const int32_t* pivots, int recover,
+                              hipStream_t s);
+
 // factor: launch under the factorization symbol (vm_factor_kernel)
 hipError_t launch_vm(const VmArgs& a, int precision, int workers, int threads, bool factor, uint32_t* epoch_dev,
                      int32_t* status, hipStream_t s);

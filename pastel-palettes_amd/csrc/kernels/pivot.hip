@@ -55,7 +55,8 @@ __global__ __launch_bounds__(kPivThreads) void getrf_pivoted_kernel(T* A0, int l
   int info = 0;
   const int K = ny < nx ? ny : nx;
   for (int j = 0; j < K; ++j) {
-    // argmax of |a_ij| over i >= j; threads without a row offer (-1, ny)
+    // argmax of |a_ij| over i >= j; threads without a row (or with only NaN)
+    // offer the placeholder (-1, ny)
     T bv = T(-1);
     int bi = ny;
     for (int i = j + tid; i < ny; i += kPivThreads) {
@@ -87,7 +88,12 @@ __global__ __launch_bounds__(kPivThreads) void getrf_pivoted_kernel(T* A0, int l
           v = s_v[q];
           b = s_i[q];
         }
-      s_best = b;
+      // NaN never wins a comparison: a NaN diagonal keeps its row (as the
+      // sequential scan that starts from it does), and a column that is NaN
+      // at and below the diagonal leaves every thread's placeholder row ny,
+      // which is not a row -- the diagonal row is taken then as well
+      const T d = A[(int64_t)j * ld + j];
+      s_best = (b >= ny || d != d) ? j : b;
     }
     __syncthreads();
     const int best = s_best;

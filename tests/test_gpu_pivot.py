@@ -26,6 +26,8 @@ def run_getrf(pp, ctx, prec, blocks, ld):
     d = torch.from_numpy(host).cuda()
     piv = torch.zeros(count * ny, dtype=torch.int32, device="cuda")
     info = torch.full((count,), -7, dtype=torch.int32, device="cuda")
+    # the library's stream is not torch's: the fills above finish first
+    torch.cuda.synchronize()
     ctx.getrf_pivoted(prec, d.data_ptr(), ld, ny, nx, piv.data_ptr(), info.data_ptr(), count=count, stride=stride)
     torch.cuda.synchronize()
     return d.cpu().numpy()[:, :, :nx], piv.cpu().numpy().reshape(count, ny), info.cpu().numpy()
@@ -72,6 +74,7 @@ def test_apply_pivot(pp, ctx, prec, ny, nx, ld):
     p = rng.permutation(ny).astype(np.int32)
     d = torch.from_numpy(m.copy()).cuda()
     dp = torch.from_numpy(p).cuda()
+    torch.cuda.synchronize()
     ctx.apply_pivot(prec, d.data_ptr(), ld, nx, ny, dp.data_ptr())
     torch.cuda.synchronize()
     fwd = d.cpu().numpy()
@@ -86,6 +89,7 @@ def test_apply_pivot_rejects_non_permutation(pp, ctx):
     print("[case] apply_pivot bad permutation", flush=True)
     d = torch.zeros((8, 8), dtype=torch.float64, device="cuda")
     bad = torch.tensor([0, 1, 2, 3, 4, 5, 6, 6], dtype=torch.int32, device="cuda")
+    torch.cuda.synchronize()
     with pytest.raises(pp.PastelError):
         ctx.apply_pivot(0, d.data_ptr(), 8, 8, 8, bad.data_ptr())
     out = torch.tensor([0, 1, 2, 3, 4, 5, 6, 8], dtype=torch.int32, device="cuda")
@@ -110,3 +114,22 @@ def test_pivoted_factor_solves(pp, orc, ctx):
     for i in range(n - 1, -1, -1):
         x[i] = (y[i] - lu[i, i + 1:] @ x[i + 1:]) / lu[i, i]
     assert np.linalg.norm(a @ x - b) / np.linalg.norm(b) < 1e-10
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+def test_getrf_pivoted_nan_columns(pp, orc, ctx, prec):
+    """NaN entries never win the pivot search; a column that is NaN at and
+    below the diagonal keeps the diagonal row (the pivot stays a row of the
+    block), as in the sequential scan of the oracle."""
+    print(f"[case] getrf_pivoted NaN columns prec={prec}", flush=True)
+    npt, _ = DT[prec]
+    rng = np.random.default_rng(21)
+    a = rng.standard_normal((48, 48)).astype(npt)
+    a[30:, 30:] = np.nan  # NaN from column 30 on, through the updates
+    a[5, 2] = np.nan      # a lone NaN below a finite diagonal
+    g, gp, gi = run_getrf(pp, ctx, prec, [a], 48)
+    o, op, oi = orc.getrf_pivoted(a)
+    assert sorted(gp[0].tolist()) == list(range(48))
+    assert np.array_equal(gp[0], op)
+    assert gi[0] == oi
+    assert np.array_equal(g[0], o, equal_nan=True)

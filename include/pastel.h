@@ -166,7 +166,11 @@ int pp_program_schedule_entries(const pp_program* p, int worker, int32_t* insts,
 
 typedef struct pp_ctx pp_ctx;
 /* Context on `device`; allocations through it are capped at budget_bytes
- * (0 = no cap). */
+ * (0 = no cap).  The context owns a non-blocking HIP stream ("the context's
+ * stream", used wherever a call takes hip_stream = NULL): it does not wait
+ * for work on the legacy null stream, so data produced there (for example by
+ * a framework's default stream) must be complete, or the caller's stream
+ * passed instead. */
 int pp_ctx_create(int device, uint64_t budget_bytes, pp_ctx** out);
 int pp_ctx_destroy(pp_ctx* ctx);
 int pp_ctx_cu_count(const pp_ctx* ctx);

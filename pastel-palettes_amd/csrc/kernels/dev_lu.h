@@ -81,9 +81,22 @@ __device__ __forceinline__ void diag_inverses(const T* F, int ldf, int n, T* out
     const int r0 = b * NB;
     const int nb = b < nblk ? ((n - r0) < NB ? (n - r0) : NB) : 0;
     if (b < nblk) {
-      for (int e = lc; e < NB * NB; e += 32) {
-        const int i = e / NB, j = e - i * NB;
-        s[i * (NB + 1) + j] = (i < nb && j < nb) ? F[(int64_t)(r0 + i) * ldf + r0 + j] : T(0);
+      // eight loads in flight per lane, through the global address space
+      // with clamped addresses (see narrow_solve's staging)
+      for (int e0 = lc; e0 < NB * NB; e0 += 8 * 32) {
+        T v[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+          const int e = e0 + u * 32, i = e / NB, j = e - i * NB;
+          const bool in = i < nb && j < nb;
+          v[u] = gload<T>(F + (in ? (int64_t)(r0 + i) * ldf + r0 + j : (int64_t)r0 * ldf + r0));
+          if (!in) v[u] = T(0);
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+          const int e = e0 + u * 32, i = e / NB, j = e - i * NB;
+          s[i * (NB + 1) + j] = v[u];
+        }
       }
       if (UPPER && lc < nb) s[lc * (NB + 1) + NB] = T(1) / F[(int64_t)(r0 + lc) * ldf + r0 + lc];
     }
@@ -520,18 +533,37 @@ __device__ __forceinline__ void narrow_solve(T* Wg, int ldw, bool w_trans, int m
   const int tid = threadIdx.x, nt = blockDim.x, wv = wave_id(), nw = nt >> 6, l = lane_id(), li = l & 15;
   // stage W (in the order that keeps global accesses coalesced); rows >= m
   // and columns >= w are zero and stay zero
-  for (int e = tid; e < mr * wc; e += nt) {
-    int r, c;
-    if (w_trans) {
-      c = e / mr;
-      r = e - c * mr;
-    } else {
-      r = e / wc;
-      c = e - r * wc;
+  // Eight loads in flight per thread: the loads go through the global
+  // address space with clamped addresses (no branch), so the compiler may
+  // issue them ahead of the LDS stores of the same batch (a flat load could
+  // alias LDS and would wait for each store before it).
+  {
+    constexpr int U = 8;
+    const int total = mr * wc;
+    for (int e0 = tid; e0 < total; e0 += U * nt) {
+      T v[U];
+      int at[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int e = e0 + u * nt;
+        int r, c;
+        if (w_trans) {
+          c = e / mr;
+          r = e - c * mr;
+        } else {
+          r = e / wc;
+          c = e - r * wc;
+        }
+        const bool in = e < total && r < m && c < w;
+        const int64_t g = in ? (w_trans ? (int64_t)c * ldw + r : (int64_t)r * ldw + c) : 0;
+        v[u] = gload<T>(Wg + g);
+        if (!in) v[u] = T(0);
+        at[u] = e < total ? r * wp + c : -1;
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u)
+        if (at[u] >= 0) Ws[at[u]] = v[u];
     }
-    T v = T(0);
-    if (r < m && c < w) v = w_trans ? Wg[(int64_t)c * ldw + r] : Wg[(int64_t)r * ldw + c];
-    Ws[r * wp + c] = v;
   }
   volatile PP_LDS int* ready = (volatile PP_LDS int*)(Ws + mr * wp);
   for (int e = tid; e < nblk; e += nt) ready[e] = 0;

@@ -870,9 +870,20 @@ __device__ __forceinline__ void chol_upper(T* G, int k, int rows, bool first, bo
   const T u = sizeof(T) == 8 ? T(1.1102230246251565e-16) : T(5.9604644775390625e-08);
   bool ok = false;
   for (int attempt = 0; attempt < ((first || retry_all) ? 2 : 1) && !ok; ++attempt) {
-    for (int e = tid; e < k * k; e += nt) {
-      const int i = e / k, j = e - i * k;
-      s[i * sp + j] = G[e];
+    // eight loads in flight per thread (global address space, clamped
+    // addresses: see narrow_solve's staging)
+    for (int e0 = tid; e0 < k * k; e0 += 8 * nt) {
+      T v[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int e = e0 + u * nt;
+        v[u] = gload<T>(G + (e < k * k ? e : 0));
+      }
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int e = e0 + u * nt, i = e / k, j = e - i * k;
+        if (e < k * k) s[i * sp + j] = v[u];
+      }
     }
     __syncthreads();
     if (attempt == 1) {

@@ -462,11 +462,25 @@ __device__ __forceinline__ int make_step(const int32_t* pc, void* const* P, int 
   }
 }
 
+// dst and src never overlap (a copy step moves a block between an operand
+// and a worker's scratch), so each thread requests eight elements before it
+// stores them: element by element, every load would wait for the store
+// before it (they may alias as far as the compiler knows).
 template <typename T>
 __device__ __forceinline__ void copy_block(T* dst, int ldd, const T* src, int lds_, int m, int n) {
-  for (int e = threadIdx.x; e < m * n; e += blockDim.x) {
-    const int i = e / n, c = e - i * n;
-    dst[(int64_t)i * ldd + c] = src[(int64_t)i * lds_ + c];
+  const int total = m * n, nt = blockDim.x;
+  for (int e0 = threadIdx.x; e0 < total; e0 += 8 * nt) {
+    T v[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int e = e0 + u * nt < total ? e0 + u * nt : e0, i = e / n, c = e - i * n;
+      v[u] = gload<T>(src + (int64_t)i * lds_ + c);
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int e = e0 + u * nt, i = e / n, c = e - i * n;
+      if (e < total) gstore<T>(dst + (int64_t)i * ldd + c, v[u]);
+    }
   }
   __syncthreads();
 }

@@ -27,7 +27,15 @@ struct MatvecPlanDev {
   const int* contrib;  // pairs (block index, unused)
   const int* lr_list;  // indices of low-rank blocks
   int n_tiles, n_lowrank;
+  int64_t t_elems = 0;  // V^T x area (max t_off + rank * nrhs); 0: unsplit products
 };
+
+// V^T x of a low-rank block is formed by kHmvSplit workgroups over row
+// chunks of V (one workgroup per block would leave most of the GPU idle and
+// stream an 8192-row factor through one CU), whose partial products are
+// summed in chunk order by a second launch.  The t buffer of the matvec
+// launches then holds (1 + kHmvSplit) * t_elems elements.
+constexpr int kHmvSplit = 8;
 
 // ---- H-matrix construction (build.hip)
 constexpr int kBuildMaxRank = 256;

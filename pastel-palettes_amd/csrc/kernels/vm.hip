@@ -3,6 +3,8 @@
 // norms).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "dev_steps.h"
 #include "kernels.h"
 #include "vm.h"
@@ -203,6 +205,29 @@ __global__ __launch_bounds__(256) void hmv_lowrank_kernel(const MatvecBlock* blo
                       x + (int64_t)b.col0 * ldx, ldx, false);
 }
 
+// phase 1 split over row chunks: workgroup (block, s) forms the partial
+// product of rows [nx s / S, nx (s + 1) / S) of V_b into partial area s.
+template <typename T, int MODE>
+__global__ __launch_bounds__(256) void hmv_lowrank_part_kernel(const MatvecBlock* blocks, const int* lr_list,
+                                                                const T* A, const T* x, int ldx, int nrhs, T* tp,
+                                                                int64_t t_elems) {
+  const MatvecBlock b = blocks[lr_list[blockIdx.x]];
+  const int s = blockIdx.y, S = gridDim.y;
+  const int k0 = (int)((int64_t)b.nx * s / S), k1 = (int)((int64_t)b.nx * (s + 1) / S);
+  gemm_block<T, MODE>(b.rank, nrhs, k1 - k0, T(1), T(0), tp + s * t_elems + b.t_off, nrhs,
+                      A + b.off_v + (int64_t)k0 * b.rank, b.rank, true, x + (int64_t)(b.col0 + k0) * ldx, ldx, false);
+}
+
+// t[e] = sum over s in order of the partial areas
+template <typename T>
+__global__ void hmv_sum_kernel(T* t, const T* tp, int64_t t_elems, int S) {
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < t_elems; e += (int64_t)gridDim.x * blockDim.x) {
+    T v = tp[e];
+    for (int s = 1; s < S; ++s) v += tp[s * t_elems + e];
+    t[e] = v;
+  }
+}
+
 // phase 2: one workgroup per row tile: out = y + contributions of the blocks
 // covering the tile, in a fixed order.
 template <typename T, int MODE>
@@ -329,9 +354,16 @@ hipError_t launch_add_correction(double* x, int ldx, const void* d, int d_bytes,
 template <typename T, int MODE>
 static hipError_t launch_hmv_t(const MatvecPlanDev& p, const T* A, const T* x, int ldx, const T* y, int ldy, T* out,
                                int ldo, int nrhs, T* t, T sd, hipStream_t s) {
-  if (p.n_lowrank > 0)
+  if (p.n_lowrank > 0 && p.t_elems > 0) {
+    T* tp = t + p.t_elems;
+    hipLaunchKernelGGL((hmv_lowrank_part_kernel<T, MODE>), dim3(p.n_lowrank, kHmvSplit), dim3(256), 0, s, p.blocks,
+                       p.lr_list, A, x, ldx, nrhs, tp, p.t_elems);
+    const int g = (int)std::min<int64_t>((p.t_elems + 255) / 256, 1024);
+    hipLaunchKernelGGL((hmv_sum_kernel<T>), dim3(g), dim3(256), 0, s, t, (const T*)tp, p.t_elems, kHmvSplit);
+  } else if (p.n_lowrank > 0) {
     hipLaunchKernelGGL((hmv_lowrank_kernel<T, MODE>), dim3(p.n_lowrank), dim3(256), 0, s, p.blocks, p.lr_list, A, x,
                        ldx, nrhs, t);
+  }
   hipLaunchKernelGGL((hmv_rows_kernel<T, MODE>), dim3(p.n_tiles), dim3(256), 0, s, p.blocks, p.tiles, p.contrib, A, x,
                      ldx, (const T*)t, y, ldy, out, ldo, nrhs, sd);
   return hipGetLastError();

@@ -161,7 +161,8 @@ int build_plan(pp_solver* s) {
   }
   int rc;
   if ((rc = upload_plan(s, blocks, &s->plan))) return rc;
-  if ((rc = ctx_alloc_t(s->ctx, std::max<int64_t>(t_off, 1), &s->d_t))) return rc;
+  s->plan.t_elems = t_off;
+  if ((rc = ctx_alloc_t(s->ctx, std::max<int64_t>((1 + pp::kHmvSplit) * t_off, 1), &s->d_t))) return rc;
   s->owned.push_back(s->d_t);
   std::vector<int> seg;
   for (size_t m = 0; m < s->hms.size(); ++m) {
@@ -229,8 +230,12 @@ int build_inverse_plans(pp_solver* s) {
   int rc;
   if ((rc = upload_plan(s, lb, &s->lplan))) return rc;
   if ((rc = upload_plan(s, ub, &s->uplan))) return rc;
+  s->lplan.t_elems = tl;
+  s->uplan.t_elems = tu;
   const int eb = elem_bytes(s->precision);
-  if ((rc = ctx_alloc(s->ctx, (uint64_t)std::max<int64_t>(std::max(tl, tu), 1) * eb, &s->d_tinv))) return rc;
+  if ((rc = ctx_alloc(s->ctx, (uint64_t)std::max<int64_t>((1 + pp::kHmvSplit) * std::max(tl, tu), 1) * eb,
+                      &s->d_tinv)))
+    return rc;
   s->owned.push_back(s->d_tinv);
   if ((rc = ctx_alloc(s->ctx, (uint64_t)s->n_total * s->nrhs_max * eb, &s->d_rhs2))) return rc;
   s->owned.push_back(s->d_rhs2);

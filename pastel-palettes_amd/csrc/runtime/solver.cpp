@@ -87,22 +87,28 @@ int upload_plan(pp_solver* s, const std::vector<pp::MatvecBlock>& blocks, pp::Ma
   }
   if (s->n_total > 0x7fffffffLL) return set_error(PP_ERR_INVALID, "stacked rows exceed int32");
   std::sort(ranges.begin(), ranges.end());
+  // Row tiles of at most 64 rows (a diagonal leaf's rows split four ways at
+  // leaf 256): one workgroup each, so an N = 16384 product has 256 of them
+  // instead of 64.
+  constexpr int kTileRows = 64;
   std::vector<pp::MatvecTile> tiles;
   std::vector<int> contrib;
   for (const auto& r : ranges) {
-    pp::MatvecTile t{};
-    t.row0 = (int32_t)r.first;
-    t.rows = r.second;
-    t.first = (int)(contrib.size() / 2);
-    for (size_t i = 0; i < blocks.size(); ++i) {
-      const auto& b = blocks[i];
-      if (b.row0 <= r.first && r.first + r.second <= (int64_t)b.row0 + b.ny) {
-        contrib.push_back((int)i);
-        contrib.push_back(0);
+    for (int64_t r0 = r.first; r0 < r.first + r.second; r0 += kTileRows) {
+      pp::MatvecTile t{};
+      t.row0 = (int32_t)r0;
+      t.rows = (int)std::min<int64_t>(kTileRows, r.first + r.second - r0);
+      t.first = (int)(contrib.size() / 2);
+      for (size_t i = 0; i < blocks.size(); ++i) {
+        const auto& b = blocks[i];
+        if (b.row0 <= r0 && r0 + t.rows <= (int64_t)b.row0 + b.ny) {
+          contrib.push_back((int)i);
+          contrib.push_back(0);
+        }
       }
+      t.count = (int)(contrib.size() / 2) - t.first;
+      tiles.push_back(t);
     }
-    t.count = (int)(contrib.size() / 2) - t.first;
-    tiles.push_back(t);
   }
   int64_t covered = 0;
   for (const auto& t : tiles) covered += t.rows;

@@ -689,7 +689,8 @@ __device__ __forceinline__ void wave_store_rows(T* G, int64_t ld, const PP_LDS T
 //              registers; the pivot row of step j is read straight from lane
 //              j's registers (v_readlane), so the sequential chain touches no
 //              memory.  Multiplier column j goes to LDS after step j
-//              (counter `cols`), and final rows go to LDS in groups of eight
+//              (counter `cols`), and final rows go to LDS in groups of GRP (8 in
+//              FP64, 4 in FP32)
 //              together with their pivot reciprocals (counter `rows`).
 //   wave 1     forms L11^{-1} by the same row operations applied to the
 //              identity, following the multiplier columns, and writes it
@@ -702,7 +703,7 @@ __device__ __forceinline__ void wave_store_rows(T* G, int64_t ld, const PP_LDS T
 template <typename T>
 __device__ __forceinline__ void panel_lu(T* M, int ld, int p, int pw, int ny, T* linv) {
   constexpr int UP = NB + 2;              // pitch of U: 16-byte aligned rows
-  constexpr int GRP = 8;                  // rows per publication
+  constexpr int GRP = sizeof(T) == 4 ? 4 : 8;  // rows per publication (measured best per type)
   constexpr int SP = NB + 1;              // pitch of the output tiles
   PP_LDS T* U = (PP_LDS T*)lds_base();    // [NB][UP]: final rows, column NB = 1 / u_jj
   PP_LDS T* Lc = U + NB * UP;              // [NB][NB]: multiplier column j at Lc[j * NB + i]

@@ -683,6 +683,23 @@ __device__ __forceinline__ void wave_store_rows(T* G, int64_t ld, const PP_LDS T
   lds_order();
 }
 
+// 1 / x for the panel's pivot chain: the hardware reciprocal refined by
+// Newton steps r <- r + r (1 - x r) (two in FP64, one in FP32; each doubles
+// the correct bits of the estimate, and the result is within an ulp of
+// 1 / x), a shorter dependent sequence than the correctly rounded division.
+// Zero and non-finite pivots give non-finite multipliers either way.
+template <typename T>
+__device__ __forceinline__ T pivot_recip(T x) {
+  if constexpr (sizeof(T) == 8) {
+    T r = __builtin_amdgcn_rcp(x);
+    r = __builtin_fma(r, __builtin_fma(-x, r, T(1)), r);
+    return __builtin_fma(r, __builtin_fma(-x, r, T(1)), r);
+  } else {
+    const T r = __builtin_amdgcn_rcpf(x);
+    return __builtin_fmaf(r, __builtin_fmaf(-x, r, T(1)), r);
+  }
+}
+
 // Unblocked LU of the panel M[p:ny, p:p+pw] (pw <= NB) in place, run by three
 // roles concurrently (no workgroup barrier inside):
 //   wave 0     factors the pw x pw diagonal block.  Lane i owns row i in
@@ -726,7 +743,7 @@ __device__ __forceinline__ void panel_lu(T* M, int ld, int p, int pw, int ny, T*
     T x[NB];
 #pragma unroll
     for (int c = 0; c < NB; ++c) x[c] = (own && c < pw) ? row[c] : T(0);
-    T rinv = T(1) / x[0];  // reciprocal of the next pivot (meaningful on its owner)
+    T rinv = pivot_recip(x[0]);  // reciprocal of the next pivot (meaningful on its owner)
     T my_rinv = rinv;      // reciprocal of this lane's own pivot
 #pragma unroll
     for (int j = 0; j < NB; ++j) {
@@ -743,7 +760,7 @@ __device__ __forceinline__ void panel_lu(T* M, int ld, int p, int pw, int ny, T*
         if (j + 1 < NB) {
           const T u = bcast_lane(x[j + 1], j);
           x[j + 1] -= lv * u;
-          rinv = T(1) / x[j + 1];
+          rinv = pivot_recip(x[j + 1]);
           if (l == j + 1) my_rinv = rinv;
         }
 #pragma unroll

@@ -42,9 +42,21 @@ def _obj(src):
     return os.path.join(OBJ, src.replace("/", "__") + ".o")
 
 
-def _compile(src, hip):
+def _newest_header():
+    t = 0.0
+    for root in (CSRC, os.path.join(os.path.dirname(HERE), "include")):
+        for d, _, files in os.walk(root):
+            for f in files:
+                if f.endswith(".h"):
+                    t = max(t, os.path.getmtime(os.path.join(d, f)))
+    return t
+
+
+def _compile(src, hip, incremental=False, headers=0.0):
     out = _obj(src)
     path = os.path.join(CSRC, src)
+    if incremental and os.path.exists(out) and os.path.getmtime(out) >= max(os.path.getmtime(path), headers):
+        return out
     if hip:
         cmd = [os.path.join(ROCM, "bin", "hipcc")] + HIPFLAGS + ["-c", path, "-o", out]
     else:
@@ -56,11 +68,15 @@ def _compile(src, hip):
     return out
 
 
-def build(jobs=8, verbose=False):
+def build(jobs=8, verbose=False, incremental=False):
+    """Compiles every source (incremental: only sources newer than their
+    object or older than the newest header) and links libpastel.so."""
     os.makedirs(OBJ, exist_ok=True)
-    work = [(s, False) for s in HOST_SOURCES] + [(s, True) for s in HIP_SOURCES]
+    headers = _newest_header() if incremental else 0.0
+    work = [(s, True) for s in HIP_SOURCES] + [(s, False) for s in HOST_SOURCES]
     with cf.ThreadPoolExecutor(max_workers=max(1, jobs)) as ex:
-        objs = list(ex.map(lambda a: _compile(*a), work))
+        objs = list(ex.map(lambda a: _compile(a[0], a[1], incremental, headers), work))
+    objs = objs[len(HIP_SOURCES):] + objs[:len(HIP_SOURCES)]
     tmp = LIB + ".tmp"
     cmd = [os.path.join(ROCM, "bin", "hipcc"), "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs + [
         f"-L{ROCM}/lib", "-lamdhip64", "-Wl,-rpath," + f"{ROCM}/lib"]
@@ -76,9 +92,10 @@ def build(jobs=8, verbose=False):
 if __name__ == "__main__":
     ap = argparse.ArgumentParser()
     ap.add_argument("--jobs", type=int, default=8)
+    ap.add_argument("--incremental", action="store_true", help="recompile only what changed")
     a = ap.parse_args()
     try:
-        build(a.jobs, verbose=True)
+        build(a.jobs, verbose=True, incremental=a.incremental)
     except RuntimeError as e:
         print(e, file=sys.stderr)
         sys.exit(1)

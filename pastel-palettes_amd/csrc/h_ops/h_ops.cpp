@@ -134,6 +134,99 @@ void accum_scratch(int nx, int ny, int k1, int k2, int64_t* offset1, int64_t* of
 // n x n triangular factor in the worker scratch.
 int64_t tri_scratch_elems(int n) { return (int64_t)((n + 31) / 32) * 32 * 32; }
 
+int64_t h_op_check_words(const int* w, int n_words, int n_slots, std::string* why) {
+  auto fail = [&](const char* what) -> int64_t {
+    if (why) *why = what;
+    return -1;
+  };
+  if (n_words < 3 || w[0] != execute) return fail("not an execute entry");
+  const int op = w[2];
+  if (operationLength(op) != n_words) return fail("entry length differs from its operation's");
+  auto slot_ok = [&](int i) { return w[i] >= 0 && w[i] < n_slots; };
+  auto nonneg = [&](int i0, int i1) {
+    for (int i = i0; i <= i1; ++i)
+      if (w[i] < 0) return false;
+    return true;
+  };
+  auto flag_ok = [&](int i) { return w[i] == 0 || w[i] == 1; };
+  // a stored rows x cols block with pitch ld needs ld >= cols
+  auto pitch_ok = [](int ld, int cols) { return ld >= cols; };
+  switch (op) {
+    case nop:
+      return 0;
+    case getrf:  // [3] P, [4] off, [5] nx, [6] ny, [7] ld
+      if (!slot_ok(3) || !nonneg(4, 7) || !pitch_ok(w[7], w[5])) return fail("bad getrf operands");
+      return tri_scratch_elems(32);
+    case trsml:
+    case trsmr:
+    case trsmu: {  // [3] P(B), [4] P(F), [5,6] off, [7] nx_b, [8] ny_b, [9] n_f, [10,11] ld, [12] b_T
+      if (!slot_ok(3) || !slot_ok(4) || !nonneg(5, 11) || !flag_ok(12)) return fail("bad trsm operands");
+      const int nx_b = w[7], ny_b = w[8], nf = w[9];
+      const bool bt = w[12] != 0;
+      if (op == trsmu && bt) return fail("trsmu has no transposed form");
+      const int n = op == trsmr ? nx_b : ny_b;  // order of the triangular factor
+      if (nf != n || !pitch_ok(w[11], n) || !pitch_ok(w[10], bt ? ny_b : nx_b)) return fail("bad trsm shape");
+      return tri_scratch_elems(n);
+    }
+    case trmm:  // [3] P(B), [4] P(T), [5,6] off, [7] nx_b, [8] ny_b, [9] ld_b, [10] ld_t
+      if (!slot_ok(3) || !slot_ok(4) || !nonneg(5, 10) || !pitch_ok(w[9], w[7]) || !pitch_ok(w[10], w[8]))
+        return fail("bad trmm operands");
+      return (int64_t)w[7] * w[8];
+    case gemm:
+    case gemm_plus: {  // [3..5] P, [6..8] off, [9..11] m n k, [12..14] ld, [15,16] a_T b_T
+      if (!slot_ok(3) || !slot_ok(4) || !slot_ok(5) || !nonneg(6, 14) || !flag_ok(15) || !flag_ok(16))
+        return fail("bad gemm operands");
+      const int m = w[9], n = w[10], k = w[11];
+      if (!pitch_ok(w[12], n) || !pitch_ok(w[13], w[15] ? m : k) || !pitch_ok(w[14], w[16] ? k : n))
+        return fail("bad gemm pitch");
+      return 0;
+    }
+    case gemm_3x: {  // [3..6] P, [7..10] off, [11..14] m n k l, [15..18] ld, [19..21] T, [22] control
+      for (int i = 3; i <= 6; ++i)
+        if (!slot_ok(i)) return fail("bad gemm_3x slot");
+      if (!nonneg(7, 18) || !flag_ok(19) || !flag_ok(20) || !flag_ok(21) || !flag_ok(22))
+        return fail("bad gemm_3x operands");
+      const int64_t m = w[11], n = w[12], k = w[13], l = w[14];
+      if (!pitch_ok(w[15], (int)n) || !pitch_ok(w[16], (int)(w[19] ? m : k)) ||
+          !pitch_ok(w[17], (int)(w[20] ? k : l)) || !pitch_ok(w[18], (int)(w[21] ? l : n)))
+        return fail("bad gemm_3x pitch");
+      return w[22] ? m * l : k * n;
+    }
+    case gemm_4x: {  // [3..7] P, [8..12] off, [13..17] m n k l o, [18..22] ld, [23..26] T, [27] control, [28] offset
+      for (int i = 3; i <= 7; ++i)
+        if (!slot_ok(i)) return fail("bad gemm_4x slot");
+      if (!nonneg(8, 22) || !flag_ok(23) || !flag_ok(24) || !flag_ok(25) || !flag_ok(26) || w[27] < 0 || w[27] > 4 ||
+          w[28] < 0)
+        return fail("bad gemm_4x operands");
+      const int64_t m = w[13], n = w[14], k = w[15], l = w[16], o = w[17];
+      if (!pitch_ok(w[18], (int)n) || !pitch_ok(w[19], (int)(w[23] ? m : k)) ||
+          !pitch_ok(w[20], (int)(w[24] ? k : l)) || !pitch_ok(w[21], (int)(w[25] ? l : o)) ||
+          !pitch_ok(w[22], (int)(w[26] ? o : n)))
+        return fail("bad gemm_4x pitch");
+      // first and second temporaries of the five association orders
+      const int64_t t1[5] = {m * l, k * o, k * o, l * n, m * l};
+      const int64_t t2[5] = {m * o, m * o, k * n, k * n, l * n};
+      const int c = w[27];
+      if (w[28] < t1[c]) return fail("gemm_4x temporaries overlap");
+      return w[28] + t2[c];
+    }
+    case accum: {  // [3..6] P, [7..10] off, [11] nx, [12] ny, [13] k1, [14] k2, [15..18] ld, [19,20] offsets
+      for (int i = 3; i <= 6; ++i)
+        if (!slot_ok(i)) return fail("bad accum slot");
+      if (!nonneg(7, 20)) return fail("bad accum operands");
+      const int nx = w[11], ny = w[12], k1 = w[13], k2 = w[14];
+      if (k1 > 64) return fail("accum rank above 64");
+      if (!pitch_ok(w[15], k1) || !pitch_ok(w[16], k1) || !pitch_ok(w[17], k2) || !pitch_ok(w[18], k2))
+        return fail("bad accum pitch");
+      // Y [0, ny k1), Vn [offset1, + nx k1), Q [offset2, + ny k1), then the small area
+      if (w[19] < (int64_t)ny * k1 || w[20] < w[19] + (int64_t)nx * k1) return fail("accum temporaries overlap");
+      return align32(w[20] + (int64_t)ny * k1) + accum_small_elems(k1, k2);
+    }
+    default:
+      return fail("unknown operation");
+  }
+}
+
 void emit_getrf(h_ops_tree* tree, const dview& m) {
   h_op o = make_op(getrf, {m.ptr, off32(m.off), m.nx, m.ny, m.ld});
   o.data.addData(false, m.map());

@@ -31,6 +31,7 @@
 #pragma once
 
 #include <cstdint>
+#include <string>
 #include <vector>
 
 #include "../rts/ops.h"
@@ -166,3 +167,13 @@ void emit_accum(h_ops_tree* tree, const dview& u1, const dview& v1, const dview&
 int64_t accum_small_elems(int k1, int k2);
 void accum_scratch(int nx, int ny, int k1, int k2, int64_t* offset1, int64_t* offset2, int64_t* total);
 int64_t tri_scratch_elems(int n);
+
+// Checks the operand words of one execute entry (w[0..operationLength) with
+// w[2] the operation) the way the device decoder will read them: every slot
+// index in [0, n_slots), offsets, sizes and pitches non-negative, every pitch
+// at least the stored row width, transpose flags 0 / 1, a gemm_3x / gemm_4x
+// association order and accum's rank within the decoder's range, and the
+// worker-scratch offsets consistent with the temporaries they hold.  Returns
+// the per-worker scratch (elements) the device routine touches, or -1 with
+// the reason in *why.
+int64_t h_op_check_words(const int* w, int n_words, int n_slots, std::string* why);

@@ -213,8 +213,10 @@ std::unique_ptr<pp_program> decode_program(const ppw::section& s) {
   const int members = r.get<int32_t>();
   const int flags = r.get<int32_t>();
   (void)r.get<int32_t>();
+  // member i owns slots 3i (matrix), 3i + 1 (temps) and 3i + 2 (right-hand
+  // sides); the upload indexes the temp slot of every member
   if (p->workers < 1 || p->workers > (1 << 16) || p->n_slots < 0 || p->n_slots > (1 << 24) || n < 0 ||
-      members < 0 || members > p->n_slots || (flags & ~1) != 0)
+      members < 0 || (members > 0 && 3 * (int64_t)members - 2 >= p->n_slots) || (flags & ~1) != 0)
     throw wire_error("bad program header");
   p->zero_temps = (flags & 1) == 0;
   im->n_insts = n;
@@ -240,6 +242,12 @@ std::unique_ptr<pp_program> decode_program(const ppw::section& s) {
     if (op < (int)nop || op > (int)trmm || nw != operationLength(op) || o.words[0] != execute || o.words[2] != op ||
         o.scratch < 0)
       throw wire_error("bad instruction " + std::to_string(i));
+    // operand words as the device decoder reads them; the worker scratch the
+    // routine touches must fit in the program's per-worker scratch
+    std::string why;
+    const int64_t need = h_op_check_words(o.words.data(), nw, p->n_slots, &why);
+    if (need < 0) throw wire_error("instruction " + std::to_string(i) + ": " + why);
+    if (need > p->scratch) throw wire_error("instruction " + std::to_string(i) + " needs more scratch than the program");
     o.op = (operation_t)op;
     p->tree.push(std::move(o));
   }

@@ -687,14 +687,23 @@ __device__ __forceinline__ void wave_store_rows(T* G, int64_t ld, const PP_LDS T
 // Newton steps r <- r + r (1 - x r) (two in FP64, one in FP32; each doubles
 // the correct bits of the estimate, and the result is within an ulp of
 // 1 / x), a shorter dependent sequence than the correctly rounded division.
-// Zero and non-finite pivots give non-finite multipliers either way.
+// The estimate is only refined where it and 1 / x are normal numbers: for
+// |x| near the format's maximum 1 / x is subnormal and the estimate may be
+// flushed to 0 (which the Newton steps keep), and for infinite x the steps
+// turn rcp(inf) = 0 into NaN.  Pivots outside [2^-1000, 2^1000] (FP64) or
+// [2^-120, 2^120] (FP32), zero, infinite and NaN pivots therefore take the
+// division, so every pivot gives the multipliers 1 / x gives.
 template <typename T>
 __device__ __forceinline__ T pivot_recip(T x) {
   if constexpr (sizeof(T) == 8) {
+    const T ax = __builtin_fabs(x);
+    if (!(ax >= 0x1p-1000 && ax <= 0x1p1000)) return T(1) / x;
     T r = __builtin_amdgcn_rcp(x);
     r = __builtin_fma(r, __builtin_fma(-x, r, T(1)), r);
     return __builtin_fma(r, __builtin_fma(-x, r, T(1)), r);
   } else {
+    const T ax = __builtin_fabsf(x);
+    if (!(ax >= 0x1p-120f && ax <= 0x1p120f)) return T(1) / x;
     const T r = __builtin_amdgcn_rcpf(x);
     return __builtin_fmaf(r, __builtin_fmaf(-x, r, T(1)), r);
   }

@@ -64,6 +64,10 @@ struct pp_solver {
   void* d_tinv = nullptr;  // V^T x products of lplan / uplan, storage precision
   double factor_flops = 0;
   double last_factor_ms = 0;
+  // the factor arena holds factors of the current matrices: set by a
+  // factorization whose launch reported no VM error, or by loading factors;
+  // cleared when a factorization starts
+  bool factored = false;
   std::vector<double> member_res;
   bool use_graph = false;
   hipGraphExec_t factor_graph = nullptr;
@@ -93,6 +97,7 @@ int upload_plan(pp_solver* s, const std::vector<pp::MatvecBlock>& blocks, pp::Ma
   constexpr int kTileRows = 64;
   std::vector<pp::MatvecTile> tiles;
   std::vector<int> contrib;
+  std::vector<int64_t> rows_of(blocks.size(), 0);  // rows of the tiles each block contributes to
   for (const auto& r : ranges) {
     for (int64_t r0 = r.first; r0 < r.first + r.second; r0 += kTileRows) {
       pp::MatvecTile t{};
@@ -104,6 +109,7 @@ int upload_plan(pp_solver* s, const std::vector<pp::MatvecBlock>& blocks, pp::Ma
         if (b.row0 <= r0 && r0 + t.rows <= (int64_t)b.row0 + b.ny) {
           contrib.push_back((int)i);
           contrib.push_back(0);
+          rows_of[i] += t.rows;
         }
       }
       t.count = (int)(contrib.size() / 2) - t.first;
@@ -113,6 +119,11 @@ int upload_plan(pp_solver* s, const std::vector<pp::MatvecBlock>& blocks, pp::Ma
   int64_t covered = 0;
   for (const auto& t : tiles) covered += t.rows;
   if (covered != s->n_total) return set_error(PP_ERR_INVALID, "matvec plan: diagonal blocks do not tile the rows");
+  // a block whose rows start or end inside a row tile would be left out of
+  // that tile: every block must be covered by whole tiles
+  for (size_t i = 0; i < blocks.size(); ++i)
+    if (rows_of[i] != blocks[i].ny)
+      return set_error(PP_ERR_INVALID, "matvec plan: block " + std::to_string(i) + " is not aligned to the row tiles");
   pp::MatvecBlock* d_blocks = nullptr;
   pp::MatvecTile* d_tiles = nullptr;
   int *d_contrib = nullptr, *d_lr = nullptr;
@@ -479,12 +490,14 @@ int pp_solver_factor(pp_solver* s, void* hip_stream) {
   try {
     if (s == nullptr) return set_error(PP_ERR_INVALID, "null solver");
     hipStream_t st = pick(s->ctx, hip_stream);
+    s->factored = false;
     if (!s->use_graph) {
       int rc = enqueue_factor(s, st, true);
       if (rc) return rc;
       rc = pp_wait(s->ctx, s->fstream, 0, nullptr);
       if (rc) return rc;
       s->last_factor_ms = pp_stream_last_ms(s->fstream);
+      s->factored = true;
       return PP_OK;
     }
     // capture once (restore + temp fills + epoch bump + VM), replay after
@@ -514,7 +527,9 @@ int pp_solver_factor(pp_solver* s, void* hip_stream) {
     float ms = 0.f;
     HIP_TRY(hipEventElapsedTime(&ms, s->fstream->ev0, s->fstream->ev1));
     s->last_factor_ms = ms;
-    return check_status(s->fstream);
+    const int rc = check_status(s->fstream);
+    s->factored = rc == PP_OK;
+    return rc;
   } catch (...) {
     return from_exception();
   }
@@ -535,6 +550,7 @@ int pp_solver_solve(pp_solver* s, const double* b, double* x, int nrhs, int ld, 
   try {
     if (s == nullptr || b == nullptr || x == nullptr || nrhs < 1 || nrhs > s->nrhs_max || ld < nrhs || max_iter < 0)
       return set_error(PP_ERR_INVALID, "bad solve args");
+    if (!s->factored) return set_error(PP_ERR_INVALID, "solve needs a successful factorization (or loaded factors)");
     hipStream_t st = pick(s->ctx, hip_stream);
     const auto t0 = std::chrono::steady_clock::now();
     const int eb = elem_bytes(s->precision);
@@ -711,7 +727,9 @@ int pp_solver_decode_factors(pp_solver* s, const uint8_t* buf, uint64_t len) {
     const int rc = fp64_arena_sum(s, &here);
     if (rc) return rc;
     if (sum != here) throw ppw::wire_error("factors were computed from other matrix entries");
+    s->factored = false;
     HIP_TRY(hipMemcpy(s->d_F, r.here(), (size_t)n * elem_bytes(s->precision), hipMemcpyHostToDevice));
+    s->factored = true;
     return PP_OK;
   } catch (const ppw::wire_error& e) {
     return set_error(PP_ERR_INVALID, e.what());

@@ -180,9 +180,18 @@ class Server:
                 req.x = dx.cpu().numpy()
             except Exception as e:  # reported per request; the server keeps running
                 req.error = e
+                # the entry is this request's alone (in_use == 1): after any
+                # failure its factors cannot be trusted, so it is evicted and
+                # a later request for the key builds and factors afresh
                 with self.lock:
-                    if res.solver is None:
-                        self.resident.pop(req.key, None)
+                    if res.solver is not None:
+                        try:
+                            res.solver.close()
+                        except Exception:
+                            pass
+                        res.solver = None
+                    if self.resident.get(req.key) is res:
+                        del self.resident[req.key]
                         self.bytes_resident -= res.bytes
             req.done = time.perf_counter()
             with self.lock:

@@ -61,6 +61,25 @@ def diag_dominant(rng, n, m=None):
     return a
 
 
+@pytest.mark.parametrize("prec,scale", [(FP64, 2.0 ** 1012), (FP64, 2.0 ** -1010), (FP32, 2.0 ** 118)])
+def test_getrf_extreme_pivots(ctx, orc, prec, scale):
+    """Pivots near the format's largest and smallest normal numbers: the
+    panel takes the division path where the reciprocal estimate would be
+    flushed or overflow, and the factors equal the oracle's (which divides)."""
+    rng = np.random.default_rng(7)
+    n = 96
+    a = diag_dominant(rng, n) * scale
+    d = dev(a, prec)
+    run(ctx, prec, [1, 0, 0, n, n, n], [d])
+    got = host(d)
+    want = orc.getrf(a.astype(np.float32).astype(np.float64) if prec == FP32 else a)
+    assert np.all(np.isfinite(got))
+    # L (below the diagonal) is scale free; U carries the scale
+    lo = np.tril(np.ones((n, n), bool), -1)
+    assert np.max(np.abs(got[lo] - want[lo])) < TOL[prec] * 10 * np.max(np.abs(want[lo]))
+    assert rel(np.where(lo, 0, got), np.where(lo, 0, want)) < TOL[prec] * 10
+
+
 @pytest.mark.parametrize("prec", [FP64, FP32])
 @pytest.mark.parametrize("n,m", [(256, 256), (128, 128), (200, 200), (256, 100), (90, 300), (100, 100), (128, 64),
                                  (64, 128), (180, 180)])

@@ -46,7 +46,9 @@ class FakeCtx:
 
 class FakeSolver:
     fail_keys = set()
+    fail_factor_keys = set()
     factor_calls = 0
+    closed = 0
     size_fn = staticmethod(lambda hm: hm.storage * 12)
 
     def __init__(self, ctx, hm, ptr, precision, workers, wavefronts, nrhs_max):
@@ -59,12 +61,15 @@ class FakeSolver:
     def factor(self, stream=None):
         FakeSolver.factor_calls += 1
         time.sleep(0.002)
+        if self.hm.n in FakeSolver.fail_factor_keys:
+            raise RuntimeError("VM watchdog")
 
     def solve(self, b_ptr, x_ptr, nrhs, tol, max_iter, hip_stream=None):
         time.sleep(0.001)
         return {"iterations": 1, "converged": 1, "residual": 1e-12}
 
     def close(self):
+        FakeSolver.closed += 1
         self.ctx.release(self.bytes)
 
 
@@ -194,3 +199,27 @@ def test_poisson_trace_shape():
     assert all(1 <= t[2] <= 64 for t in tr)
     frac = sum(t[3] for t in tr) / len(tr)
     assert 0.2 < frac < 0.4
+
+
+def test_failed_factorization_evicts_the_entry():
+    """A factorization that raises after its solver was built leaves nothing
+    resident: the solver is closed and its bytes released, and a later reuse
+    request for the same key builds and factors again instead of solving
+    with invalid factors."""
+    FakeSolver.fail_keys = set()
+    FakeSolver.fail_factor_keys = {1000}
+    FakeSolver.factor_calls = 0
+    FakeSolver.closed = 0
+    srv, ctx, mats = make_server(budget=1 << 40, slots=1)
+    bad = serve.Request(0, "m1000", np.ones((5, 1)))
+    srv.submit(bad)
+    assert bad.event.wait(10)
+    assert bad.error is not None and "m1000" not in srv.resident
+    assert ctx.used == 0 and FakeSolver.closed == 1 and srv.bytes_resident == 0
+    FakeSolver.fail_factor_keys = set()
+    again = serve.Request(1, "m1000", np.ones((5, 1)), reuse=True)
+    srv.submit(again)
+    assert again.event.wait(10)
+    srv.close()
+    assert again.error is None and again.factored
+    assert FakeSolver.factor_calls == 2

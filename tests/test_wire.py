@@ -227,3 +227,64 @@ def test_rejects_inconsistent_program(pp):
     struct.pack_into("<i", bad, last, 2)
     with pytest.raises(pp.PastelError):
         pp.Program.from_bytes(reseal(bad))
+
+
+def _instruction_at(data, at, words):
+    """Byte offset of the instruction list entry whose words (from the
+    operation on) are `words` (the first match: instructions precede the
+    streams in the payload)."""
+    pat = struct.pack(f"<{len(words) - 2}i", *[int(x) for x in words[2:]])
+    i = data.find(pat, at)
+    assert i > 0
+    return i - 8  # back to words[0] (execute, id)
+
+
+def test_rejects_bad_operands_and_slots(pp):
+    """A resealed PROG string whose instruction words address a slot outside
+    the pointer table, carry a pitch narrower than the block or a negative
+    size, or whose header has fewer slots than its members use, is rejected
+    before anything is uploaded."""
+    hm = pp.HMatrix("hodlr", 256, levels=2, rank=4)
+    prog = pp.Program.factor(hm, 3)
+    data = prog.to_bytes()
+    (tag, at, n) = sections(data)[0]
+    w0 = prog.inst_words(0)
+    assert w0[2] == 1  # getrf [ex, id, getrf, P, off, nx, ny, ld]
+    pos = _instruction_at(data, at, w0)
+    for word, value, why in ((3, 3, "bad getrf operands"),      # slot 3 of a 3-slot program
+                             (3, -1, "bad getrf operands"),
+                             (7, int(w0[5]) - 1, "bad getrf operands"),  # pitch below the row width
+                             (6, -5, "bad getrf operands")):
+        bad = bytearray(data)
+        struct.pack_into("<i", bad, pos + 4 * word, value)
+        with pytest.raises(pp.PastelError) as e:
+            pp.Program.from_bytes(reseal(bad))
+        assert why in str(e.value), (word, value, str(e.value))
+    # header: n_slots = 1 while member 0's temp slot is 1
+    assert struct.unpack_from("<ii", data, at) == (3, 3)
+    bad = bytearray(data)
+    struct.pack_into("<i", bad, at + 4, 1)
+    with pytest.raises(pp.PastelError) as e:
+        pp.Program.from_bytes(reseal(bad))
+    assert "bad program header" in str(e.value)
+
+
+def test_rejects_scratch_beyond_program(pp):
+    """An accum whose scratch offsets reach past the program's per-worker
+    scratch, or make its temporaries overlap, is rejected."""
+    hm = pp.HMatrix("hodlr", 1024, levels=3, rank=8)
+    prog = pp.Program.factor(hm, 4)
+    data = prog.to_bytes()
+    (tag, at, n) = sections(data)[0]
+    acc = next(i for i in range(prog.info()["n_insts"]) if prog.inst_words(i)[2] == 8)
+    w = prog.inst_words(acc)
+    pos = _instruction_at(data, at, w)
+    scratch = prog.info()["scratch_elems"]
+    for word, value, why in ((20, scratch, "needs more scratch"),
+                             (19, 0, "accum temporaries overlap"),
+                             (13, 65, "accum rank above 64")):
+        bad = bytearray(data)
+        struct.pack_into("<i", bad, pos + 4 * word, value)
+        with pytest.raises(pp.PastelError) as e:
+            pp.Program.from_bytes(reseal(bad))
+        assert why in str(e.value), (word, value, str(e.value))

@@ -181,6 +181,9 @@ def main():
     ap.add_argument("--only", choices=["fp64", "mixed", "both"], default="both")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run only a plain 256 MiB device copy (PMC byte-counter calibration)")
+    ap.add_argument("--calibrate-mfma", action="store_true",
+                    help="run only three FP64 and three FP32 4096^3 library matrix products (PMC MFMA-counter "
+                         "calibration: 3 x 137.4 GFLOP each)")
     ap.add_argument("--workload", choices=["hlu", "batch", "serve", "build"], default="hlu",
                     help="hlu: the N=16384 factor+solve step (default); batch: many matrices in one launch "
                          "with autotuned leaves; serve: Poisson request stream; build: H-matrix construction "
@@ -203,6 +206,18 @@ def main():
             dst.copy_(src)
         torch.cuda.synchronize()
         print(json.dumps({"copy_bytes_read": src.numel() * 8, "copy_bytes_written": dst.numel() * 8}))
+        return
+    if args.calibrate_mfma:
+        out = {}
+        for dt in (torch.float64, torch.float32):
+            a = torch.ones(4096, 4096, dtype=dt, device="cuda")
+            b = torch.ones(4096, 4096, dtype=dt, device="cuda")
+            t0 = time.perf_counter()
+            for _ in range(3):
+                c = a @ b
+            torch.cuda.synchronize()
+            out[str(dt)] = {"flops": 3 * 2 * 4096 ** 3, "seconds": time.perf_counter() - t0, "c00": float(c[0, 0])}
+        print(json.dumps(out))
         return
     pp = importlib.import_module("pastel-palettes_amd")
     ctx = pp.Context(0)

@@ -49,6 +49,20 @@ int pp_rts_check_overlap(uint64_t ptr_a, uint64_t pitch_a, uint64_t width_a, uin
 int pp_rts_check_dependency(int n_from, const uint64_t* maps_from, const int32_t* ro_from, int n_to,
                             const uint64_t* maps_to, const int32_t* ro_to);
 
+/* dependency_table (rts/dependency.h:24-53, rts/dependency.cpp:11-76): the
+ * edge set with its from / to lists and mask per edge.  update with mask 0
+ * (dependency_t::no) removes the edge; lists come in insertion order
+ * (to != 0: predecessors of inst, else successors); list returns the count
+ * and fills at most cap entries of out (may be NULL). */
+typedef struct pp_dep_table pp_dep_table;
+int pp_dep_table_create(pp_dep_table** out);
+void pp_dep_table_destroy(pp_dep_table* t);
+int pp_dep_table_update(pp_dep_table* t, int64_t inst_from, int64_t inst_to, int dep);
+int pp_dep_table_lookup(const pp_dep_table* t, int64_t inst_from, int64_t inst_to);
+int pp_dep_table_list(const pp_dep_table* t, int64_t inst, int to, int64_t* out, int cap);
+int64_t pp_dep_table_count(const pp_dep_table* t, int64_t inst, int to);
+int pp_dep_table_clear(pp_dep_table* t);
+
 /* instructions_scheduler (rts/scheduler.h:73-126) over an explicit DAG:
  * n instructions with FLOP estimates, n_edges edges (edges[2e] -> edges[2e+1])
  * with masks.  From- and to-lists are taken in ascending order. */

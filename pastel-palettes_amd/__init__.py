@@ -44,6 +44,13 @@ def lib():
         "pp_min_inst_flops": (i64, []),
         "pp_rts_check_overlap": (C.c_int, [u64] * 8),
         "pp_rts_check_dependency": (C.c_int, [C.c_int, P(u64), P(i32), C.c_int, P(u64), P(i32)]),
+        "pp_dep_table_create": (C.c_int, [P(vp)]),
+        "pp_dep_table_destroy": (None, [vp]),
+        "pp_dep_table_update": (C.c_int, [vp, i64, i64, C.c_int]),
+        "pp_dep_table_lookup": (C.c_int, [vp, i64, i64]),
+        "pp_dep_table_list": (C.c_int, [vp, i64, C.c_int, P(i64), C.c_int]),
+        "pp_dep_table_count": (i64, [vp, i64, C.c_int]),
+        "pp_dep_table_clear": (C.c_int, [vp]),
         "pp_schedule_create": (C.c_int, [C.c_int, P(i64), C.c_int, P(i32), P(i32), C.c_int, P(vp)]),
         "pp_schedule_worker_length": (C.c_int, [vp, C.c_int]),
         "pp_schedule_worker_entries": (C.c_int, [vp, C.c_int, P(i32), P(i32), P(i64)]),
@@ -197,6 +204,47 @@ def check_dependency(ops_from, ops_to):
 
 def min_inst_flops():
     return int(lib().pp_min_inst_flops())
+
+
+class DepTable:
+    """dependency_table: edges (from, to) with a 3-bit mask each."""
+
+    def __init__(self):
+        h = C.c_void_p()
+        check(lib().pp_dep_table_create(C.byref(h)))
+        self.h = h
+
+    def update(self, f, t, dep):
+        check(lib().pp_dep_table_update(self.h, f, t, dep))
+
+    def lookup(self, f, t):
+        return check(lib().pp_dep_table_lookup(self.h, f, t))
+
+    def _list(self, inst, to):
+        n = check(lib().pp_dep_table_list(self.h, inst, to, None, 0))
+        a = np.zeros(max(n, 1), np.int64)
+        check(lib().pp_dep_table_list(self.h, inst, to, _p(a, C.c_int64), n))
+        return a[:n].tolist()
+
+    def dep_from(self, f):
+        return self._list(f, 0)
+
+    def dep_to(self, t):
+        return self._list(t, 1)
+
+    def count_from(self, f):
+        return int(check(lib().pp_dep_table_count(self.h, f, 0)))
+
+    def count_to(self, t):
+        return int(check(lib().pp_dep_table_count(self.h, t, 1)))
+
+    def clear(self):
+        check(lib().pp_dep_table_clear(self.h))
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.pp_dep_table_destroy(self.h)
+            self.h = None
 
 
 class Schedule:

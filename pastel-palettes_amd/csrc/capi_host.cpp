@@ -88,6 +88,61 @@ int pp_rts_check_dependency(int n_from, const uint64_t* maps_from, const int32_t
   }
 }
 
+struct pp_dep_table {
+  dependency_table t;
+};
+
+int pp_dep_table_create(pp_dep_table** out) {
+  try {
+    if (out == nullptr) return set_error(PP_ERR_INVALID, "null out");
+    *out = new pp_dep_table();
+    return PP_OK;
+  } catch (...) {
+    return from_exception();
+  }
+}
+
+void pp_dep_table_destroy(pp_dep_table* t) { delete t; }
+
+int pp_dep_table_update(pp_dep_table* t, int64_t inst_from, int64_t inst_to, int dep) {
+  try {
+    if (t == nullptr || dep < 0 || dep > 7) return set_error(PP_ERR_INVALID, "bad table update");
+    t->t.update(inst_from, inst_to, static_cast<dependency_t>(dep));
+    return PP_OK;
+  } catch (...) {
+    return from_exception();
+  }
+}
+
+int pp_dep_table_lookup(const pp_dep_table* t, int64_t inst_from, int64_t inst_to) {
+  if (t == nullptr) return set_error(PP_ERR_INVALID, "null table");
+  return static_cast<int>(t->t.lookup(inst_from, inst_to));
+}
+
+int pp_dep_table_list(const pp_dep_table* t, int64_t inst, int to, int64_t* out, int cap) {
+  try {
+    if (t == nullptr || cap < 0) return set_error(PP_ERR_INVALID, "bad table query");
+    std::vector<int64_t> v;
+    if (to) t->t.getDepTo(inst, v);
+    else t->t.getDepFrom(inst, v);
+    for (int i = 0; i < (int)v.size() && i < cap && out != nullptr; ++i) out[i] = v[i];
+    return (int)v.size();
+  } catch (...) {
+    return from_exception();
+  }
+}
+
+int64_t pp_dep_table_count(const pp_dep_table* t, int64_t inst, int to) {
+  if (t == nullptr) return set_error(PP_ERR_INVALID, "null table");
+  return (int64_t)(to ? t->t.countDepTo(inst) : t->t.countDepFrom(inst));
+}
+
+int pp_dep_table_clear(pp_dep_table* t) {
+  if (t == nullptr) return set_error(PP_ERR_INVALID, "null table");
+  t->t.clear();
+  return PP_OK;
+}
+
 int pp_schedule_create(int n, const int64_t* flops, int n_edges, const int32_t* edges, const int32_t* masks,
                        int workers, pp_schedule** out) {
   try {

@@ -195,3 +195,117 @@ def test_schedule_is_valid(pp):
                 ptr[w] += 1
                 progress = True
     assert len(done) == n
+
+
+@pytest.mark.skipif(not __import__("oracle").ref_available(), reason="reference rts library not built here")
+@pytest.mark.parametrize("seed", range(6))
+def test_dependency_table_matches_reference(pp, orc, seed):
+    """The library's dependency_table and the reference's own compiled
+    rts/dependency.cpp agree on random insert / retype sequences: lookups,
+    successor and predecessor lists (insertion order) and counts
+    (rts/dependency.cpp:11-65).  Removals and clear() are left out here: the
+    reference's removal erases inside its equal_range loop (undefined
+    behaviour) and its clear() keeps the masks; both are checked against a
+    model below."""
+    import ctypes as C
+    R = orc.ref()
+    rng = np.random.default_rng(seed)
+    ref = R.ref_table_new()
+    mine = pp.DepTable()
+    n = int(rng.integers(4, 40))
+    for _ in range(int(rng.integers(20, 400))):
+        f, t, m = int(rng.integers(0, n)), int(rng.integers(0, n)), int(rng.integers(1, 8))
+        R.ref_table_update(ref, f, t, m)
+        mine.update(f, t, m)
+    buf = (C.c_int64 * 4096)()
+    for i in range(n):
+        k = R.ref_table_from(ref, i, buf, 4096)
+        assert mine.dep_from(i) == list(buf[:k])
+        k = R.ref_table_to(ref, i, buf, 4096)
+        assert mine.dep_to(i) == list(buf[:k])
+        assert mine.count_from(i) == R.ref_table_count_from(ref, i)
+        assert mine.count_to(i) == R.ref_table_count_to(ref, i)
+        for j in range(n):
+            assert mine.lookup(i, j) == R.ref_table_lookup(ref, i, j)
+    R.ref_table_free(ref)
+
+
+def test_dependency_table_removal_and_clear(pp):
+    """update(no) removes exactly one edge from the edge set and both lists,
+    keeping the order of the others; clear() empties every container, so a
+    re-inserted edge carries its new mask."""
+    rng = np.random.default_rng(5)
+    t = pp.DepTable()
+    model = {}  # (f, t) -> mask, insertion ordered
+    for _ in range(600):
+        f, to = int(rng.integers(0, 12)), int(rng.integers(0, 12))
+        m = int(rng.integers(0, 8)) if rng.random() < 0.7 else 0
+        t.update(f, to, m)
+        if m == 0:
+            model.pop((f, to), None)
+        elif (f, to) in model:
+            model[(f, to)] = m
+        else:
+            model[(f, to)] = m
+    for i in range(12):
+        assert t.dep_from(i) == [b for (a, b) in model if a == i]
+        assert t.dep_to(i) == [a for (a, b) in model if b == i]
+        assert t.count_from(i) == sum(a == i for (a, b) in model)
+        for j in range(12):
+            assert t.lookup(i, j) == model.get((i, j), 0)
+    t.clear()
+    assert all(t.count_from(i) == 0 and t.count_to(i) == 0 for i in range(12))
+    t.update(3, 4, 2)
+    assert t.lookup(3, 4) == 2 and t.dep_from(3) == [4]
+
+
+# Schedules traced by hand through rts/scheduler.cpp, FLOPs in units of F =
+# _MIN_INST_FLOPS, two workers.  Each entry is (instruction, execute?, the
+# worker's FLOP clock at the entry).
+HAND_TRACED = {
+    # 0 -> 1, 2, 3; 4 alone.  Step 1 takes i0 over the head i4 (more
+    # successors, rts/scheduler.cpp:173-179); its three successors are
+    # prepended into one fresh list (equal max_sync, :343-344) which hookup
+    # merges behind the head (:142-161); i2 finds no worker with load <= its
+    # sync and waits for i0 on worker 1; i1 falls back to worker 0 (the
+    # initial earliest_after_sync, :270-286).
+    "fanout": (5, [1, 1, 1, 1, 1], [(0, 1), (0, 2), (0, 3)],
+               [[(0, True, 0), (3, True, 1), (1, True, 2)],
+                [(4, True, 0), (0, False, 1), (2, True, 1)]], 1),
+    # 0 (3F) -> 3, 5, 6; 1 -> 2, 4; 2 -> 5; 4 -> 6.  Step 1 takes i0 (3
+    # successors) over the head i1; i4 and i2 become ready behind the head
+    # i3 although their max_sync is smaller (hookup only inserts after the
+    # head); step 3 takes the head i3 because its max_sync exceeds the
+    # smallest load (:165-171); step 4 keeps the head i4 on a num_deps tie
+    # (strict >); i2 goes to the earliest worker after its sync; i6 and i5
+    # each wait once.
+    "head": (7, [3, 1, 1, 1, 1, 1, 1], [(0, 3), (0, 5), (0, 6), (1, 2), (1, 4), (2, 5), (4, 6)],
+             [[(0, True, 0), (3, True, 3), (2, False, 4), (5, True, 4)],
+              [(1, True, 0), (4, True, 1), (2, True, 2), (0, False, 3), (6, True, 3)]], 2),
+    # 0 (2F) -> 3, 4; 1 -> 2, 5; 2 -> 3; 3 -> 4.  i5 is prepended before i2;
+    # steps 3 and 4 take the second node (more successors) instead of the
+    # head i5; i3 waits for i0 (worker 1, clock 2F); i5 runs on the worker
+    # after its sync and waits for i1; i4's sync with i0 is dropped because
+    # worker 0 already waited for worker 1 at clock 2F (eliminateExtraSync,
+    # :289-304).
+    "elim": (6, [2, 1, 1, 1, 1, 1], [(0, 3), (0, 4), (1, 2), (1, 5), (2, 3), (3, 4)],
+             [[(1, True, 0), (2, True, 1), (0, False, 2), (3, True, 2), (4, True, 3)],
+              [(0, True, 0), (1, False, 2), (5, True, 2)]], 2),
+}
+
+
+@pytest.mark.parametrize("name", sorted(HAND_TRACED))
+def test_schedules_traced_by_hand(pp, name):
+    """The Python restatement and the library's scheduler both reproduce
+    schedules derived by hand from rts/scheduler.cpp, including hookup of
+    several fresh entries, num_deps ties, the head-only insertion and
+    eliminated syncs."""
+    n, units, edges, want, waits = HAND_TRACED[name]
+    F = pp.min_inst_flops()
+    flops = [u * F for u in units]
+    want = [[(i, ex, c * F) for i, ex, c in w] for w in want]
+    succ, pred = _lists(n, [(a, b, 1) for a, b in edges])
+    assert sref.schedule(n, flops, succ, pred, 2, F) == want
+    s = pp.Schedule(flops, edges, [1] * len(edges), 2)
+    assert [s.worker(w) for w in range(2)] == want
+    assert s.comm_waits() == waits

@@ -295,6 +295,12 @@ int pp_solver_num_members(const pp_solver* s);
 int64_t pp_solver_rows(const pp_solver* s);
 /* Row offset, size and last refined residual of member i. */
 int pp_solver_member(const pp_solver* s, int i, int64_t* row_off, int* n, double* last_residual);
+/* Element offset of member i's arena (its pp_hmatrix_storage elements, then
+ * its inverse area) within the factor arena of pp_solver_factor_arena and
+ * within the solver's FP64 copy of the matrices. */
+int pp_solver_member_arena(const pp_solver* s, int i, int64_t* arena_off);
+/* Elements of the factor arena (every member's arena and inverse area). */
+int64_t pp_solver_storage_elems(const pp_solver* s);
 /* Factor through a captured hipGraph (captured on first use per stream). */
 int pp_solver_use_graph(pp_solver* s, int enable);
 /* Factor in place (restores the working copy from the FP64 matrix first). */

@@ -130,6 +130,8 @@ def lib():
         "pp_solver_num_members": (C.c_int, [vp]),
         "pp_solver_rows": (i64, [vp]),
         "pp_solver_member": (C.c_int, [vp, C.c_int, P(i64), P(C.c_int), P(dp)]),
+        "pp_solver_member_arena": (C.c_int, [vp, C.c_int, P(i64)]),
+        "pp_solver_storage_elems": (i64, [vp]),
         "pp_solver_use_graph": (C.c_int, [vp, C.c_int]),
         "pp_solver_last_factor_ms": (dp, [vp]),
         "pp_hmatrix_build_kernel": (C.c_int, [vp, vp, C.c_int, vp, dp, dp, vp, vp, vp]),
@@ -701,10 +703,11 @@ class Solver:
 
     def members(self):
         out = []
-        ro, n, r = C.c_int64(), C.c_int(), C.c_double()
+        ro, n, r, ao = C.c_int64(), C.c_int(), C.c_double(), C.c_int64()
         for i in range(check(lib().pp_solver_num_members(self.h))):
             check(lib().pp_solver_member(self.h, i, C.byref(ro), C.byref(n), C.byref(r)))
-            out.append(dict(row_off=ro.value, n=n.value, residual=r.value))
+            check(lib().pp_solver_member_arena(self.h, i, C.byref(ao)))
+            out.append(dict(row_off=ro.value, n=n.value, residual=r.value, arena_off=ao.value))
         return out
 
     def use_graph(self, on=True):
@@ -731,6 +734,16 @@ class Solver:
         eb = C.c_int()
         check(lib().pp_solver_factor_arena(self.h, C.byref(p), C.byref(eb)))
         return p.value, eb.value
+
+    def factor_arena_host(self):
+        """The whole factor arena (all members, inverse areas included) as
+        float64 on the host."""
+        import ctypes as _C
+        ptr, eb = self.factor_arena()
+        total = int(lib().pp_solver_storage_elems(self.h))
+        raw = np.empty(total, np.float64 if eb == 8 else np.float32)
+        check(lib().pp_copy_d2h(self.ctx.h, raw.ctypes.data_as(_C.c_void_p), _C.c_void_p(ptr), eb * total))
+        return raw.astype(np.float64)
 
     def factor_stream_ms(self):
         return float(lib().pp_stream_last_ms(lib().pp_solver_factor_stream(self.h)))

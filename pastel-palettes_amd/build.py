@@ -30,7 +30,8 @@ HOST_SOURCES = [
     "runtime/build.cpp",
     "runtime/pivot.cpp",
 ]
-HIP_SOURCES = ["kernels/vm.hip", "kernels/build.hip", "kernels/pivot.hip"]
+HIP_SOURCES = ["kernels/vm_fp64.hip", "kernels/vm_mixed.hip", "kernels/vm_fp32.hip", "kernels/vm.hip", "kernels/build.hip",
+               "kernels/pivot.hip"]
 
 CXXFLAGS = ["-O2", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-function", f"-I{ROCM}/include",
             "-D__HIP_PLATFORM_AMD__"]
@@ -42,25 +43,34 @@ def _obj(src):
     return os.path.join(OBJ, src.replace("/", "__") + ".o")
 
 
-def _newest_header():
-    t = 0.0
-    for root in (CSRC, os.path.join(os.path.dirname(HERE), "include")):
-        for d, _, files in os.walk(root):
-            for f in files:
-                if f.endswith(".h"):
-                    t = max(t, os.path.getmtime(os.path.join(d, f)))
-    return t
+def _deps(out):
+    """Files the object was built from (its -MMD dependency list), or None."""
+    try:
+        with open(out + ".d") as f:
+            text = f.read().replace("\\\n", " ")
+    except OSError:
+        return None
+    return text.split(":", 1)[1].split() if ":" in text else None
 
 
-def _compile(src, hip, incremental=False, headers=0.0):
+def _up_to_date(out):
+    deps = _deps(out)
+    if not deps or not os.path.exists(out):
+        return False
+    t = os.path.getmtime(out)
+    return all(os.path.exists(d) and os.path.getmtime(d) <= t for d in deps)
+
+
+def _compile(src, hip, incremental=False):
     out = _obj(src)
     path = os.path.join(CSRC, src)
-    if incremental and os.path.exists(out) and os.path.getmtime(out) >= max(os.path.getmtime(path), headers):
+    if incremental and _up_to_date(out):
         return out
+    dep = ["-MMD", "-MF", out + ".d"]
     if hip:
-        cmd = [os.path.join(ROCM, "bin", "hipcc")] + HIPFLAGS + ["-c", path, "-o", out]
+        cmd = [os.path.join(ROCM, "bin", "hipcc")] + HIPFLAGS + dep + ["-c", path, "-o", out]
     else:
-        cmd = ["g++"] + CXXFLAGS + ["-c", path, "-o", out]
+        cmd = ["g++"] + CXXFLAGS + dep + ["-c", path, "-o", out]
     r = subprocess.run(cmd, capture_output=True, text=True)
     err = "\n".join(l for l in r.stderr.splitlines() if "packed-fp32-ops" not in l)
     if r.returncode != 0:
@@ -69,13 +79,12 @@ def _compile(src, hip, incremental=False, headers=0.0):
 
 
 def build(jobs=8, verbose=False, incremental=False):
-    """Compiles every source (incremental: only sources newer than their
-    object or older than the newest header) and links libpastel.so."""
+    """Compiles every source (incremental: only those whose object is older
+    than a file it was built from) and links libpastel.so."""
     os.makedirs(OBJ, exist_ok=True)
-    headers = _newest_header() if incremental else 0.0
     work = [(s, True) for s in HIP_SOURCES] + [(s, False) for s in HOST_SOURCES]
     with cf.ThreadPoolExecutor(max_workers=max(1, jobs)) as ex:
-        objs = list(ex.map(lambda a: _compile(a[0], a[1], incremental, headers), work))
+        objs = list(ex.map(lambda a: _compile(a[0], a[1], incremental), work))
     objs = objs[len(HIP_SOURCES):] + objs[:len(HIP_SOURCES)]
     tmp = LIB + ".tmp"
     cmd = [os.path.join(ROCM, "bin", "hipcc"), "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs + [

@@ -40,7 +40,7 @@ enum : int { MODE_FP64 = 0, MODE_FP32 = 1, MODE_BF16 = 2 };
 // namespace scope so that every routine addresses it as LDS (ds_* instructions)
 // rather than through generic pointers.
 constexpr int kLdsBytes = 144 * 1024;
-__shared__ __attribute__((aligned(16))) char pp_lds[kLdsBytes];
+static __shared__ __attribute__((aligned(16))) char pp_lds[kLdsBytes];
 // Pointers into the arena carry the LDS address space explicitly (PP_LDS), so
 // accesses compile to ds_* instructions even through selects and loops.
 #define PP_LDS __attribute__((address_space(3)))

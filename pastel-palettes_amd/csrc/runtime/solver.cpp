@@ -480,6 +480,15 @@ int pp_solver_member(const pp_solver* s, int i, int64_t* row_off, int* n, double
   return PP_OK;
 }
 
+int pp_solver_member_arena(const pp_solver* s, int i, int64_t* arena_off) {
+  if (s == nullptr || arena_off == nullptr || i < 0 || i >= (int)s->hms.size())
+    return set_error(PP_ERR_INVALID, "bad member");
+  *arena_off = s->arena_off[i];
+  return PP_OK;
+}
+
+int64_t pp_solver_storage_elems(const pp_solver* s) { return s ? s->storage_total : PP_ERR_INVALID; }
+
 int pp_solver_use_graph(pp_solver* s, int enable) {
   if (s == nullptr) return set_error(PP_ERR_INVALID, "null solver");
   s->use_graph = enable != 0;

@@ -106,7 +106,7 @@ def cpu_baseline(pp, orc, hm, arena, b, threads, tol, min_seconds=10.0, max_reps
     info = prog.info()
     sprog = pp.Program.solve(hm, b.shape[1], threads)
     sinfo = sprog.info()
-    rnd = pp.sketch(pp.SKETCH_SEED, 1 << 17)
+    rnd = orc.sketch(orc.sketch_need(prog.streams()))
     leaves = hm.leaves()
     bn = np.linalg.norm(b, axis=0)
     t0 = time.perf_counter()

@@ -178,18 +178,45 @@ def accum(u1, v1, u2, v2, rnd=None):
     return u1, v1
 
 
+_ENTRY_WORDS = {0: 3, 1: 8, 2: 13, 3: 13, 4: 17, 5: 17, 6: 23, 7: 29, 8: 21, 9: 13, 10: 11}
+
+
+def sketch_need(streams):
+    """Sketch values the streams' accum entries read: max of nx * k1."""
+    need = 1
+    for st in streams:
+        i, n = 0, len(st)
+        while i < n:
+            op = int(st[i])
+            if op == 0:
+                break
+            if op == 2:
+                i += 3
+                continue
+            o = int(st[i + 2])
+            if o == 8:
+                need = max(need, int(st[i + 11]) * int(st[i + 13]))
+            i += _ENTRY_WORDS[o]
+    return need
+
+
 def run_streams(streams, slots, n_insts, scratch_elems, rnd=None, threads=1):
     """Execute per-worker instruction streams (lists of int32 arrays) on host
-    arrays `slots` (float64, modified in place)."""
+    arrays `slots` (float64, modified in place).  rnd: the sketch values
+    (default: the library's sketch, as many values as the accum entries
+    read); a shorter array is refused rather than read past its end."""
     W = len(streams)
     sarr = [np.ascontiguousarray(s, np.int32) for s in streams]
+    need = sketch_need(sarr)
+    if rnd is None:
+        rnd = sketch(need)
+    rnd = np.ascontiguousarray(rnd, np.float64)
+    if rnd.size < need:
+        raise ValueError(f"sketch of {rnd.size} values; the accum entries read {need}")
     sp = (C.POINTER(C.c_int32) * W)(*[s.ctypes.data_as(C.POINTER(C.c_int32)) for s in sarr])
     pp = (C.POINTER(C.c_double) * len(slots))(*[_d(s) if s is not None else None for s in slots])
     stride = max(int(scratch_elems), 64)
     scratch = np.zeros(stride * W)
-    if rnd is None:
-        rnd = sketch(1 << 16)
-    rnd = np.ascontiguousarray(rnd, np.float64)
     rc = lib().or_run_streams(sp, W, pp, _d(scratch), stride, _d(rnd), n_insts, threads)
     if rc != 0:
         raise RuntimeError("oracle stream run failed (deadlock or bad entry)")

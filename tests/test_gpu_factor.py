@@ -46,8 +46,7 @@ def oracle_factor(pp, orc, prog, arena):
     info = prog.info()
     fac = arena.copy()
     tmp = np.zeros(max(info["temp_elems"], 1))
-    orc.run_streams(prog.streams(), [fac, tmp, None], info["n_insts"], info["scratch_elems"],
-                    rnd=pp.sketch(pp.SKETCH_SEED, 1 << 17))
+    orc.run_streams(prog.streams(), [fac, tmp, None], info["n_insts"], info["scratch_elems"])
     return fac
 
 

@@ -69,7 +69,7 @@ def test_factor_strings_rejected_elsewhere(pp, ctx):
     s1.close()
 
 
-def test_decoded_program_runs(pp, ctx):
+def test_decoded_program_runs(pp, orc, ctx):
     print("[case] decoded factor program n=2048", flush=True)
     hm, kern, arena = pp.build_config(1, 2048, levels=3, rank=16)
     prog = pp.Program.factor(hm, 16)
@@ -85,3 +85,7 @@ def test_decoded_program_runs(pp, ctx):
         out.append(d.cpu().numpy())
         s.close()
     assert np.array_equal(out[0], out[1])
+    from tests import hlu_host as H
+    worst = H.worst_leaf_diff(hm, out[1], H.oracle_factor(pp, orc, hm, arena))
+    print(f"decoded program vs oracle: worst leaf difference {worst:.2e}")
+    assert worst < 1e-9

@@ -244,6 +244,20 @@ int pp_exec_single(pp_ctx* ctx, int precision, const int32_t* words, int n_words
  * *ms receives the mean kernel time per execution (per-op latency). */
 int pp_exec_single_timed(pp_ctx* ctx, int precision, const int32_t* words, int n_words, const void* const* ptr_table,
                          int n_ptrs, uint64_t scratch_bytes, uint64_t seed, int wavefronts, int reps, double* ms);
+/* pp_exec_single without any allocation (no counterpart in the reference):
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  The
+ * worker scratch (scratch_bytes), the accum sketch and the status word live
+ * in one static device array per GPU, resolved once per device by
+ * pp_ctx_create; a call whose scratch, sketch and status need more than
+ * pp_exec_single_noalloc_max_bytes() (4 MiB) fails with PP_ERR_INVALID.
+ * ptr_table holds at most 16 device addresses.  Synchronous on hip_stream
+ * (NULL = the context's stream).  Every call on one GPU shares that array
+ * and leaves its contents there: calls that overlap on one GPU race. */
+int pp_exec_single_noalloc(pp_ctx* ctx, int precision, const int32_t* words, int n_words,
+                           const void* const* ptr_table, int n_ptrs, uint64_t scratch_bytes, uint64_t seed,
+                           void* hip_stream);
+uint64_t pp_exec_single_noalloc_max_bytes(void);
 /* Fill a device buffer with the sketch matrix used by accum (count values,
  * uniform in [-1, 1) from the in-library generator). */
 int pp_fill_sketch(pp_ctx* ctx, void* dptr, int64_t count, int precision, uint64_t seed);

@@ -112,6 +112,8 @@ def lib():
         "pp_exec_single_timed": (C.c_int, [vp, C.c_int, P(i32), C.c_int, P(vp), C.c_int, u64, u64, C.c_int, C.c_int,
                                            P(dp)]),
         "pp_fill_sketch": (C.c_int, [vp, vp, i64, C.c_int, u64]),
+        "pp_exec_single_noalloc": (C.c_int, [vp, C.c_int, P(i32), C.c_int, P(vp), C.c_int, u64, u64, vp]),
+        "pp_exec_single_noalloc_max_bytes": (u64, []),
         "pp_mfma_probe": (C.c_int, [vp, C.c_int, P(dp), C.c_int, P(dp), C.c_int, P(dp)]),
         "pp_solver_create": (C.c_int, [vp, vp, P(dp), C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, P(vp)]),
         "pp_solver_create_device": (C.c_int, [vp, vp, vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, P(vp)]),

@@ -31,7 +31,7 @@ HOST_SOURCES = [
     "runtime/pivot.cpp",
 ]
 HIP_SOURCES = ["kernels/vm_fp64.hip", "kernels/vm_mixed.hip", "kernels/vm_fp32.hip", "kernels/vm.hip", "kernels/build.hip",
-               "kernels/pivot.hip"]
+               "kernels/pivot.hip", "kernels/noalloc.hip"]
 
 CXXFLAGS = ["-O2", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-function", f"-I{ROCM}/include",
             "-D__HIP_PLATFORM_AMD__"]

@@ -136,7 +136,12 @@ hipError_t launch_happly(const MatvecPlanDev& p, int elem_bytes, const void* A, 
 hipError_t launch_segnorm2(const double* r, int ldr, const int* seg, int nseg, int nrhs, double* out, hipStream_t s);
 hipError_t launch_colnorm2(const double* r, int ldr, int n, int nrhs, double* out, hipStream_t s);
 hipError_t launch_mfma_probe(int mode, const double* a, const double* b, double* out, hipStream_t s);
-hipError_t launch_single_op(int precision, int threads, int reps, const int32_t* words, void* const* ptrs,
-                            void* scratch, const void* rnd, int32_t* status, hipStream_t s);
+hipError_t launch_single_op(int precision, int threads, int reps, const SingleOp& op, void* scratch, const void* rnd,
+                            int32_t* status, hipStream_t s);
+// Sketch values [-1, 1) of the library's generator (splitmix64 from `seed`,
+// value i from state seed + (i + 1) * 0x9E3779B97F4A7C15), as FP64
+// (elem_bytes 8) or rounded to FP32 (4), computed on the device.
+hipError_t launch_sketch_fill(void* out, int64_t count, int elem_bytes, uint64_t seed, hipStream_t s);
+hipError_t launch_clear_i32(int32_t* p, int n, hipStream_t s);
 
 }  // namespace pp

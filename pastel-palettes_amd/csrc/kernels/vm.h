@@ -40,6 +40,14 @@ struct VmArgs {
   int32_t n_insts;
 };
 
+// One execute entry for the single-op kernel, passed by value (no device
+// copy of the words or the pointer table is needed).
+constexpr int kSingleMaxWords = 32, kSingleMaxPtrs = 16;
+struct SingleOp {
+  int32_t words[kSingleMaxWords];
+  void* ptrs[kSingleMaxPtrs];
+};
+
 // status codes
 enum : int32_t { VM_OK = 0, VM_TIMEOUT = 1, VM_BAD_OPCODE = 2, VM_BAD_OPERATION = 3, VM_ABORTED = 4 };
 

@@ -282,6 +282,36 @@ hipError_t launch_colnorm2(const double* r, int ldr, int n, int nrhs, double* ou
   return hipGetLastError();
 }
 
+// Sketch values of the library's generator, one thread per value:
+// splitmix64's i-th output mixes seed + (i + 1) * golden.
+__global__ void sketch_fill_kernel(void* out, int64_t count, int elem_bytes, uint64_t seed) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < count; i += (int64_t)gridDim.x * blockDim.x) {
+    uint64_t z = seed + (uint64_t)(i + 1) * 0x9E3779B97F4A7C15ULL;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+    z ^= z >> 31;
+    const double v = 2.0 * ((double)(z >> 11) * 0x1.0p-53) - 1.0;
+    if (elem_bytes == 8) static_cast<double*>(out)[i] = v;
+    else static_cast<float*>(out)[i] = (float)v;
+  }
+}
+
+__global__ void clear_i32_kernel(int32_t* p, int n) {
+  if ((int)threadIdx.x < n) p[threadIdx.x] = 0;
+}
+
+hipError_t launch_sketch_fill(void* out, int64_t count, int elem_bytes, uint64_t seed, hipStream_t s) {
+  if (count <= 0) return hipSuccess;
+  const int g = (int)std::min<int64_t>((count + 255) / 256, 1024);
+  hipLaunchKernelGGL(sketch_fill_kernel, dim3(g), dim3(256), 0, s, out, count, elem_bytes, seed);
+  return hipGetLastError();
+}
+
+hipError_t launch_clear_i32(int32_t* p, int n, hipStream_t s) {
+  hipLaunchKernelGGL(clear_i32_kernel, dim3(1), dim3(64), 0, s, p, n);
+  return hipGetLastError();
+}
+
 // MFMA lane-layout probe: D = A B for one 16x16xK tile with A[i][k] = a[i*K+k],
 // B[k][j] = b[k*16+j]; lane l writes its 4 accumulator values to out[4l..].
 __global__ void mfma_probe_kernel(int mode, const double* a, const double* b, double* out) {
@@ -312,11 +342,11 @@ hipError_t launch_mfma_probe(int mode, const double* a, const double* b, double*
   return hipGetLastError();
 }
 
-hipError_t launch_single_op(int precision, int threads, int reps, const int32_t* words, void* const* ptrs,
-                            void* scratch, const void* rnd, int32_t* status, hipStream_t s) {
-  if (precision == 0) return vm_single_fp64(threads, reps, words, ptrs, scratch, rnd, status, s);
-  if (precision == 1) return vm_single_fp32(threads, reps, words, ptrs, scratch, rnd, status, s);
-  return vm_single_mixed(threads, reps, words, ptrs, scratch, rnd, status, s);
+hipError_t launch_single_op(int precision, int threads, int reps, const SingleOp& op, void* scratch, const void* rnd,
+                            int32_t* status, hipStream_t s) {
+  if (precision == 0) return vm_single_fp64(threads, reps, op, scratch, rnd, status, s);
+  if (precision == 1) return vm_single_fp32(threads, reps, op, scratch, rnd, status, s);
+  return vm_single_mixed(threads, reps, op, scratch, rnd, status, s);
 }
 
 }  // namespace pp

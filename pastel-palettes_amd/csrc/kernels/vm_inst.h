@@ -13,8 +13,8 @@ namespace pp {
 #define PP_VM_DECLARE(name)                                                                                 \
   hipError_t vm_launch_##name(const VmArgs& a, int workers, int threads, bool factor, hipStream_t s);      \
   int vm_blocks_##name(int threads);                                                                        \
-  hipError_t vm_single_##name(int threads, int reps, const int32_t* words, void* const* ptrs, void* scratch, \
-                              const void* rnd, int32_t* status, hipStream_t s);
+  hipError_t vm_single_##name(int threads, int reps, const SingleOp& op, void* scratch, const void* rnd,      \
+                              int32_t* status, hipStream_t s);
 PP_VM_DECLARE(fp64)
 PP_VM_DECLARE(fp32)
 PP_VM_DECLARE(mixed)

@@ -130,24 +130,30 @@ static int max_blocks_t() {
 }
 
 // Single-op harness: one workgroup runs one execute entry `reps` times
-// (tests use reps = 1; latency measurements use more, in place).
+// (tests use reps = 1; latency measurements use more, in place).  The words
+// and the pointer table arrive by value and are read from LDS.
 template <typename T, int FMODE, int GMODE, int NT>
-__global__ __launch_bounds__(NT) void single_op_kernel(const int32_t* words, void* const* ptrs, void* scratch,
-                                                       const void* rnd, int32_t* status, int reps) {
+__global__ __launch_bounds__(NT) void single_op_kernel(SingleOp op, void* scratch, const void* rnd, int32_t* status,
+                                                       int reps) {
+  __shared__ int32_t w[kSingleMaxWords];
+  __shared__ void* p[kSingleMaxPtrs];
+  if (threadIdx.x < kSingleMaxWords) w[threadIdx.x] = op.words[threadIdx.x];
+  if (threadIdx.x < kSingleMaxPtrs) p[threadIdx.x] = op.ptrs[threadIdx.x];
+  __syncthreads();
   int len = -1;
   for (int r = 0; r < reps; ++r) {
-    len = exec_entry<T, FMODE, GMODE>(words, ptrs, rnd, reinterpret_cast<T*>(scratch));
+    len = exec_entry<T, FMODE, GMODE>(w, p, rnd, reinterpret_cast<T*>(scratch));
     __syncthreads();
   }
   if (threadIdx.x == 0) status[0] = len;
 }
 
 template <typename T, int FMODE, int GMODE>
-static void launch_single_t(int threads, const int32_t* words, void* const* ptrs, void* scratch, const void* rnd,
-                            int32_t* status, int reps, hipStream_t s) {
+static void launch_single_t(int threads, const SingleOp& op, void* scratch, const void* rnd, int32_t* status, int reps,
+                            hipStream_t s) {
   (void)threads;  // one workgroup shape: 256 threads (4 wavefronts)
-  hipLaunchKernelGGL((single_op_kernel<T, FMODE, GMODE, 256>), dim3(1), dim3(256), 0, s, words, ptrs, scratch, rnd,
-                     status, reps);
+  hipLaunchKernelGGL((single_op_kernel<T, FMODE, GMODE, 256>), dim3(1), dim3(256), 0, s, op, scratch, rnd, status,
+                     reps);
 }
 
 }  // namespace pp
@@ -159,9 +165,9 @@ static void launch_single_t(int threads, const int32_t* words, void* const* ptrs
     return launch_vm_nt<T, FMODE, GMODE>(a, workers, threads, factor, s);                                     \
   }                                                                                                           \
   int vm_blocks_##name(int threads) { return threads == 256 ? max_blocks_t<T, FMODE, GMODE, 256>() : 0; }     \
-  hipError_t vm_single_##name(int threads, int reps, const int32_t* words, void* const* ptrs, void* scratch,   \
-                              const void* rnd, int32_t* status, hipStream_t s) {                              \
-    launch_single_t<T, FMODE, GMODE>(threads, words, ptrs, scratch, rnd, status, reps, s);                    \
+  hipError_t vm_single_##name(int threads, int reps, const SingleOp& op, void* scratch, const void* rnd,        \
+                              int32_t* status, hipStream_t s) {                                               \
+    launch_single_t<T, FMODE, GMODE>(threads, op, scratch, rnd, status, reps, s);                             \
     return hipGetLastError();                                                                                 \
   }                                                                                                           \
   }

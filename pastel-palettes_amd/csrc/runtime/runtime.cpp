@@ -74,6 +74,24 @@ int enqueue_launch(pp_ctx* ctx, pp_stream* s, int wavefronts_per_inst, hipStream
   return PP_OK;
 }
 
+int single_op_args(int precision, const int32_t* words, int n_words, const void* const* ptr_table, int n_ptrs,
+                   pp::SingleOp* op) {
+  if (words == nullptr || n_words < 3 || ptr_table == nullptr || n_ptrs < 1 || n_ptrs > pp::kSingleMaxPtrs ||
+      precision < 0 || precision > 2)
+    return set_error(PP_ERR_INVALID, "bad single-op args");
+  const int len = operationLength(words[2]);
+  if (words[0] != execute || len <= 0 || n_words < len || len > pp::kSingleMaxWords)
+    return set_error(PP_ERR_INVALID, "bad instruction words");
+  *op = pp::SingleOp{};
+  for (int i = 0; i < len; ++i) op->words[i] = words[i];
+  for (int i = 0; i < n_ptrs; ++i) op->ptrs[i] = const_cast<void*>(ptr_table[i]);
+  return PP_OK;
+}
+
+int64_t single_op_sketch(const int32_t* words) {
+  return words[2] == accum ? std::max<int64_t>((int64_t)words[11] * words[13], 1) : 1;
+}
+
 }  // namespace ppr
 
 namespace {
@@ -192,6 +210,12 @@ int pp_ctx_create(int device, uint64_t budget_bytes, pp_ctx** out) {
     HIP_TRY(hipGetDeviceProperties(&prop, device));
     c->cus = prop.multiProcessorCount;
     HIP_TRY(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    // the allocation-free entry point's scratch on this device (once per device)
+    const int nrc = noalloc_init(device);
+    if (nrc != PP_OK) {
+      (void)hipStreamDestroy(c->stream);
+      return nrc;
+    }
     *out = c.release();
     return PP_OK;
   } catch (...) {
@@ -429,43 +453,26 @@ int pp_fill_sketch(pp_ctx* ctx, void* dptr, int64_t count, int precision, uint64
 
 int pp_exec_single_timed(pp_ctx* ctx, int precision, const int32_t* words, int n_words, const void* const* ptr_table,
                          int n_ptrs, uint64_t scratch_bytes, uint64_t seed, int wavefronts, int reps, double* ms) {
-  if (ctx == nullptr || words == nullptr || n_words < 3 || ptr_table == nullptr || n_ptrs < 1 || reps < 1 ||
-      wavefronts != 4)
-    return set_error(PP_ERR_INVALID, "bad args");
-  const int len = operationLength(words[2]);
-  if (words[0] != execute || len <= 0 || n_words < len) return set_error(PP_ERR_INVALID, "bad instruction words");
-  int32_t* d_w = nullptr;
-  void** d_p = nullptr;
+  pp::SingleOp op{};
+  int rc = single_op_args(precision, words, n_words, ptr_table, n_ptrs, &op);
+  if (ctx == nullptr || reps < 1 || wavefronts != 4) return set_error(PP_ERR_INVALID, "bad args");
+  if (rc) return rc;
   void* d_s = nullptr;
   void* d_r = nullptr;
   int32_t* d_st = nullptr;
   hipEvent_t e0 = nullptr, e1 = nullptr;
-  int rc = PP_OK;
   const int eb = elem_bytes(precision);
-  int64_t need = 1;
-  if (words[2] == accum) need = (int64_t)words[11] * words[13];
-  std::vector<double> rv;
-  sketch_values(rv, need, seed);
+  const int64_t need = single_op_sketch(words);
   do {
-    if ((rc = ctx_alloc_t(ctx, n_words, &d_w))) break;
-    if ((rc = ctx_alloc_t(ctx, n_ptrs, &d_p))) break;
     if ((rc = ctx_alloc(ctx, std::max<uint64_t>(scratch_bytes, 256), &d_s))) break;
     if ((rc = ctx_alloc(ctx, need * eb, &d_r))) break;
     if ((rc = ctx_alloc_t(ctx, 4, &d_st))) break;
-    hipError_t e = hipMemcpy(d_w, words, 4 * n_words, hipMemcpyHostToDevice);
-    if (e == hipSuccess) e = hipMemcpy(d_p, ptr_table, sizeof(void*) * n_ptrs, hipMemcpyHostToDevice);
-    if (e == hipSuccess) {
-      if (precision == PP_FP64) e = hipMemcpy(d_r, rv.data(), need * 8, hipMemcpyHostToDevice);
-      else {
-        std::vector<float> f(rv.begin(), rv.end());
-        e = hipMemcpy(d_r, f.data(), need * 4, hipMemcpyHostToDevice);
-      }
-    }
+    hipError_t e = pp::launch_sketch_fill(d_r, need, eb, seed, ctx->stream);
+    if (e == hipSuccess) e = pp::launch_clear_i32(d_st, 4, ctx->stream);
     if (e == hipSuccess) e = hipEventCreate(&e0);
     if (e == hipSuccess) e = hipEventCreate(&e1);
     if (e == hipSuccess) e = hipEventRecord(e0, ctx->stream);
-    if (e == hipSuccess)
-      e = pp::launch_single_op(precision, 64 * wavefronts, reps, d_w, d_p, d_s, d_r, d_st, ctx->stream);
+    if (e == hipSuccess) e = pp::launch_single_op(precision, 64 * wavefronts, reps, op, d_s, d_r, d_st, ctx->stream);
     if (e == hipSuccess) e = hipEventRecord(e1, ctx->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
     int32_t st = 0;
@@ -477,12 +484,10 @@ int pp_exec_single_timed(pp_ctx* ctx, int precision, const int32_t* words, int n
       break;
     }
     if (ms) *ms = t / reps;
-    if (st != len) rc = set_error(PP_ERR_DEVICE, "device rejected the instruction");
+    if (st != operationLength(words[2])) rc = set_error(PP_ERR_DEVICE, "device rejected the instruction");
   } while (false);
   if (e0) (void)hipEventDestroy(e0);
   if (e1) (void)hipEventDestroy(e1);
-  ctx_free(ctx, d_w);
-  ctx_free(ctx, d_p);
   ctx_free(ctx, d_s);
   ctx_free(ctx, d_r);
   ctx_free(ctx, d_st);

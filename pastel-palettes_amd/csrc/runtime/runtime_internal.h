@@ -98,6 +98,16 @@ int ctx_alloc_t(pp_ctx* c, int64_t count, T** p) {
 
 inline hipStream_t pick(pp_ctx* c, void* s) { return s ? static_cast<hipStream_t>(s) : c->stream; }
 
+// Checks one execute entry for the single-op kernel and packs its words and
+// pointer table (at most kSingleMaxPtrs slots) by value.
+int single_op_args(int precision, const int32_t* words, int n_words, const void* const* ptr_table, int n_ptrs,
+                   pp::SingleOp* op);
+// Sketch values the entry reads (nx * k1 for accum, else 1).
+int64_t single_op_sketch(const int32_t* words);
+// Resolves this device's allocation-free scratch (the device must be
+// current); called once per device by pp_ctx_create.
+int noalloc_init(int device);
+
 // Enqueue the launch sequence of a stream (temp fills, epoch bump, VM) on st
 // without waiting; usable under hipGraph capture.
 int enqueue_launch(pp_ctx* ctx, pp_stream* s, int wavefronts_per_inst, hipStream_t st, bool record_events);

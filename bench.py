@@ -184,7 +184,7 @@ def main():
     ap.add_argument("--calibrate-mfma", action="store_true",
                     help="run only three FP64 and three FP32 4096^3 library matrix products (PMC MFMA-counter "
                          "calibration: 3 x 137.4 GFLOP each)")
-    ap.add_argument("--workload", choices=["hlu", "batch", "serve", "build"], default="hlu",
+    ap.add_argument("--workload", choices=["hlu", "batch", "serve", "build", "trace", "ops"], default="hlu",
                     help="hlu: the N=16384 factor+solve step (default); batch: many matrices in one launch "
                          "with autotuned leaves; serve: Poisson request stream; build: H-matrix construction "
                          "on the device")
@@ -227,6 +227,10 @@ def main():
         if args.workload == "batch":
             out = wl.run_batch(pp, torch, ctx, count=args.batch_count, steps=args.steps, warmup=args.warmup,
                                log=log)
+        elif args.workload == "trace":
+            out = wl.run_trace(pp, torch, ctx, n=args.n, levels=args.levels, rank=args.rank, log=log)
+        elif args.workload == "ops":
+            out = wl.run_ops(pp, torch, ctx, log=log)
         elif args.workload == "build":
             out = wl.run_build(pp, torch, ctx, n=args.n, levels=args.levels, rank=args.rank, steps=args.steps,
                                warmup=args.warmup, log=log)

@@ -280,3 +280,96 @@ def run_build(pp, torch, ctx, n=16384, levels=6, rank=32, steps=5, warmup=1, sam
         "sampled_rows": sample_rows,
         "compression_error_device": err(dev), "compression_error_host": err(host),
     }
+
+
+def run_trace(pp, torch, ctx, n=16384, levels=6, rank=32, precision=None, workers=None, log=print):
+    """One traced factorization launch of the benchmark matrix (the solver's
+    factor program: H-LU plus the inverse construction) per precision:
+    per-entry wall-clock stamps of every worker (reference: clock64() per
+    instruction, src/kernel.cu:1389-1390, read by analyzeClocks,
+    rts/scheduler.cpp:494-544), reduced to the critical path -- the chain of
+    instructions that set the launch's end -- with per-operation time on it
+    and the signalling gaps between its entries."""
+    from . import trace as tr
+    precs = {"fp64": pp.FP64, "mixed": pp.MIXED} if precision is None else {precision: pp.PRECISIONS[precision]}
+    hm, kern, arena = pp.build_config(1, n, levels=levels, rank=rank)
+    workers = workers or min(ctx.cu_count, 256)
+    out = {"workload": f"traced factorization launch, hodlr n={n} levels={levels} rank={rank}, {workers} workers"}
+    for name, prec in precs.items():
+        solver = pp.Solver(ctx, hm, arena, precision=prec, workers=workers, nrhs_max=1)
+        fs = pp.lib().pp_solver_factor_stream(solver.h)
+        solver.factor()  # warm
+        pp.check(pp.lib().pp_stream_enable_trace(ctx.h, fs, 1))
+        solver.factor()
+        ms = solver.factor_stream_ms()
+        n_vals = pp.check(pp.lib().pp_read_trace(ctx.h, fs, None, 0))
+        stamps = np.zeros(n_vals, np.uint64)
+        import ctypes as C
+        pp.lib().pp_read_trace(ctx.h, fs, stamps.ctypes.data_as(C.POINTER(C.c_uint64)), n_vals)
+        prog = _factor_program_of(pp, solver, hm, workers)
+        summ = tr.summary(prog, stamps)
+        path, by_op = tr.critical_path(prog, stamps)
+        gaps = sum(max(0.0, b[2] - a[3]) for a, b in zip(path, path[1:]))
+        counts = {}
+        for inst, op, t0, t1 in path:
+            k = tr.OP_NAMES.get(op, str(op))
+            counts[k] = counts.get(k, 0) + 1
+        out[name] = {
+            "launch_ms": ms, "span_us": summ["span_us"], "busy_fraction": summ["busy_fraction"],
+            "critical_path": {"entries": len(path), "exec_us": sum(by_op.values()), "gap_us": gaps,
+                              "by_op": {k: {"count": counts[k], "total_us": v, "mean_us": v / counts[k]}
+                                        for k, v in sorted(by_op.items(), key=lambda kv: -kv[1])}},
+            "ops": summ["ops"],
+        }
+        solver.close()
+    return out
+
+
+def _factor_program_of(pp, solver, hm, workers):
+    """The factor program a single-matrix solver launches (factorization
+    followed by the hierarchical-inverse construction), rebuilt the same way
+    so that its streams match the traced launch's."""
+    return pp.Program.factor_inv_batch([hm], workers)
+
+
+def run_ops(pp, torch, ctx, reps=50, log=print):
+    """Latency of the operations on the benchmark's critical path, one
+    workgroup each, executed `reps` times in place (pp_exec_single_timed):
+    a 256 x 256 diagonal-leaf getrf, the 256 x 32 trsm of a low-rank
+    factor (left, and right with transposed storage), and the update of a
+    leaf by a rank-32 product (gemm_4x, and the 256 x 256 x 32 gemm_plus)."""
+    rng = np.random.default_rng(1)
+    res = {}
+    for name, prec in (("fp64", pp.FP64), ("fp32", pp.FP32), ("mixed", pp.MIXED)):
+        dt = torch.float64 if prec == pp.FP64 else torch.float32
+        d = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dt).cuda()
+        n, r = 256, 32
+        a = rng.standard_normal((n, n)) + 2 * n * np.eye(n)
+        lu = d(a)
+        row = {}
+        # getrf in place on a fresh copy per rep is not possible; the timed
+        # reps refactor the factors, the same work at the same shape
+        row["getrf_256"] = 1e3 * ctx.time_single(prec, [1, 0, 1, 0, 0, n, n, n], [lu.data_ptr()], reps=reps)
+        L = d(np.tril(rng.standard_normal((n, n)) / n, -1) + np.eye(n))
+        B = d(rng.standard_normal((n, r)))
+        row["trsml_256x32"] = 1e3 * ctx.time_single(prec, [1, 0, 2, 0, 1, 0, 0, r, n, n, r, n, 0],
+                                                    [B.data_ptr(), L.data_ptr()], reps=reps)
+        U = d(np.triu(rng.standard_normal((n, n)) / n) + 2 * np.eye(n))
+        Bt = d(rng.standard_normal((n, r)))  # V factor: B^T stored (n x r)
+        row["trsmr_t_256x32"] = 1e3 * ctx.time_single(prec, [1, 0, 3, 0, 1, 0, 0, n, r, n, r, n, 1],
+                                                      [Bt.data_ptr(), U.data_ptr()], reps=reps)
+        M = d(rng.standard_normal((n, n)))
+        Ua, Vb = d(rng.standard_normal((n, r))), d(rng.standard_normal((n, r)))
+        row["gemm_plus_256x256x32"] = 1e3 * ctx.time_single(
+            prec, [1, 0, 5, 0, 1, 2, 0, 0, 0, n, n, r, n, r, r, 0, 1], [M.data_ptr(), Ua.data_ptr(), Vb.data_ptr()],
+            reps=reps)
+        C1, C2 = d(rng.standard_normal((r, r))), d(rng.standard_normal((n, r)))
+        t1 = n * r
+        row["gemm_4x_256x256_r32"] = 1e3 * ctx.time_single(
+            prec, [1, 0, 7, 0, 1, 2, 3, 4, 0, 0, 0, 0, 0, n, n, r, r, r, n, r, r, r, r, 0, 0, 0, 1, 0,
+                   (t1 + 31) // 32 * 32],
+            [M.data_ptr(), Ua.data_ptr(), C1.data_ptr(), C1.data_ptr(), C2.data_ptr()], reps=reps)
+        res[name] = row
+        log(f"[ops] {name}: " + ", ".join(f"{k} {v:.1f} us" for k, v in row.items()))
+    return {"workload": "latency of one critical-path operation on one workgroup (microseconds, mean of "
+                        f"{reps} in-place executions)", **res}

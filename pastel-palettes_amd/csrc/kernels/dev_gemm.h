@@ -39,7 +39,7 @@ enum : int { MODE_FP64 = 0, MODE_FP32 = 1, MODE_BF16 = 2 };
 // LDS arena shared by all routines of one worker (bytes).  It is declared at
 // namespace scope so that every routine addresses it as LDS (ds_* instructions)
 // rather than through generic pointers.
-constexpr int kLdsBytes = 144 * 1024;
+constexpr int kLdsBytes = 152 * 1024;
 static __shared__ __attribute__((aligned(16))) char pp_lds[kLdsBytes];
 // Pointers into the arena carry the LDS address space explicitly (PP_LDS), so
 // accesses compile to ds_* instructions even through selects and loops.

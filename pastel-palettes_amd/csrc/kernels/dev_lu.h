@@ -752,12 +752,14 @@ __device__ __forceinline__ void panel_lu(T* M, int ld, int p, int pw, int ny, T*
     T x[NB];
 #pragma unroll
     for (int c = 0; c < NB; ++c) x[c] = (own && c < pw) ? row[c] : T(0);
-    T rinv = pivot_recip(x[0]);  // reciprocal of the next pivot (meaningful on its owner)
+    // reciprocal of the next pivot, formed from the broadcast pivot so that
+    // every lane takes the same branch of pivot_recip
+    T rinv = pivot_recip(bcast_lane(x[0], 0));
     T my_rinv = rinv;      // reciprocal of this lane's own pivot
 #pragma unroll
     for (int j = 0; j < NB; ++j) {
       if (j < pw) {
-        const T rj = bcast_lane(rinv, j);
+        const T rj = rinv;
         const bool act = own && l > j;
         // inactive lanes use a zero multiplier: x - 0 u = x for the finite
         // pivot rows, which avoids a select per element
@@ -769,7 +771,7 @@ __device__ __forceinline__ void panel_lu(T* M, int ld, int p, int pw, int ny, T*
         if (j + 1 < NB) {
           const T u = bcast_lane(x[j + 1], j);
           x[j + 1] -= lv * u;
-          rinv = pivot_recip(x[j + 1]);
+          rinv = pivot_recip(bcast_lane(x[j + 1], j + 1));
           if (l == j + 1) my_rinv = rinv;
         }
 #pragma unroll

@@ -12,6 +12,7 @@
 #pragma once
 
 #include "../h_ops/h_ops.h"
+#include "dev_lds_lu.h"
 #include "dev_lu.h"
 
 namespace pp {
@@ -32,7 +33,9 @@ enum : int {
   ST_COPY = 5,
   ST_CHOL = 6,
   ST_TSOLVE = 7,
-  ST_LUUPD = 8
+  ST_LUUPD = 8,
+  ST_LDSLU = 9,
+  ST_LDSLU2 = 10
 };
 
 // One primitive step.  Field use per kind:
@@ -47,6 +50,9 @@ enum : int {
 //   LUUPD    lu_update(M = C, ld = ldc, p = m, pw = n, nx = lda, ny = k, linv = A)
 //   TSOLVE   narrow_solve(W = C, ldw = ldc, w_trans = bT, m, w = n, F = A,
 //            ldf = lda, f_trans = aT, dinv = B, forward = k != 0)
+//   LDSLU    getrf_lds(M = C, ld = ldc, n = m): a square leaf factored in LDS
+//   LDSLU2   getrf_lds2(M = C, ld = ldc, n = m): the two-block form (FP32,
+//            128 < n <= 256)
 template <typename T>
 struct Step {
   int kind;
@@ -129,10 +135,20 @@ __device__ __forceinline__ int make_step(const int32_t* pc, void* const* P, int 
     case nop:
       return 0;
     case getrf: {
+      // square leaves that fit in LDS: one step (getrf_lds); otherwise
       // right-looking blocked LU: per 32-column panel, PANEL then LUUPD
       // (U12 = L11^{-1} A12 and A22 -= L21 U12)
       T* M = slot<T>(P, pc, 3) + word(pc, 4);
       const int nx = word(pc, 5), ny = word(pc, 6), ld = word(pc, 7);
+      if (nx == ny && nx <= (lds_lu_max<T>() > lds2_max<T>() ? lds_lu_max<T>() : lds2_max<T>())) {
+        if (s > 0 || nx == 0) return 0;
+        st.kind = nx <= lds_lu_max<T>() ? ST_LDSLU : ST_LDSLU2;
+        st.leaf = true;
+        st.C = M;
+        st.ldc = ld;
+        st.m = nx;
+        return 1;
+      }
       const int mn = nx < ny ? nx : ny, np = (mn + NB - 1) / NB;
       if (s >= 2 * np) return 0;
       const int q = s >> 1, r = s & 1, p = q * NB;
@@ -522,6 +538,12 @@ __device__ __forceinline__ void exec_step(const Step<T>& st) {
       break;
     case ST_TSOLVE:
       narrow_solve<T, FMODE>(st.C, st.ldc, st.bT, st.m, st.n, st.A, st.lda, st.aT, st.B, st.k != 0);
+      break;
+    case ST_LDSLU:
+      getrf_lds<T, FMODE>(st.C, st.ldc, st.m);
+      break;
+    case ST_LDSLU2:
+      if constexpr (lds2_max<T>() > 0) getrf_lds2<T, FMODE>(st.C, st.ldc, st.m);
       break;
     default:
       break;

@@ -335,7 +335,7 @@ def _factor_program_of(pp, solver, hm, workers):
 def run_ops(pp, torch, ctx, reps=50, log=print):
     """Latency of the operations on the benchmark's critical path, one
     workgroup each, executed `reps` times in place (pp_exec_single_timed):
-    a 256 x 256 diagonal-leaf getrf, the 256 x 32 trsm of a low-rank
+    256 x 256 (and 128, 160) diagonal-leaf getrfs, the 256 x 32 trsm of a low-rank
     factor (left, and right with transposed storage), and the update of a
     leaf by a rank-32 product (gemm_4x, and the 256 x 256 x 32 gemm_plus)."""
     rng = np.random.default_rng(1)
@@ -350,6 +350,9 @@ def run_ops(pp, torch, ctx, reps=50, log=print):
         # getrf in place on a fresh copy per rep is not possible; the timed
         # reps refactor the factors, the same work at the same shape
         row["getrf_256"] = 1e3 * ctx.time_single(prec, [1, 0, 1, 0, 0, n, n, n], [lu.data_ptr()], reps=reps)
+        for m in (128, 160):
+            a_m = d(rng.standard_normal((m, m)) + 2 * m * np.eye(m))
+            row[f"getrf_{m}"] = 1e3 * ctx.time_single(prec, [1, 0, 1, 0, 0, m, m, m], [a_m.data_ptr()], reps=reps)
         L = d(np.tril(rng.standard_normal((n, n)) / n, -1) + np.eye(n))
         B = d(rng.standard_normal((n, r)))
         row["trsml_256x32"] = 1e3 * ctx.time_single(prec, [1, 0, 2, 0, 1, 0, 0, r, n, n, r, n, 0],

@@ -879,6 +879,13 @@ __device__ __forceinline__ volatile PP_LDS int* chol_fail_flag() {
   return (volatile PP_LDS int*)(lds_base() + kLdsBytes - 16);
 }
 
+// Relative pivot below which a factorisation after the first drops a column
+// in mixed precision (chol_upper): after the first (shifted) pass the
+// sketch's independent columns have squared norms near 1 and the dependent
+// ones near the rounding level, which FP32 resolves only to about k u.
+template <typename T>
+__device__ __forceinline__ constexpr T chol_drop_tol() { return sizeof(T) == 8 ? T(1e-12) : T(1e-4); }
+
 // G (k x k, pitch k, symmetric positive semi-definite; k <= 64) <- R, the
 // upper Cholesky factor of G.  first: the first factorisation of an accum;
 // when its plain factorisation meets a non-positive pivot, G + s I is
@@ -886,19 +893,25 @@ __device__ __forceinline__ volatile PP_LDS int* chol_fail_flag() {
 // shifted CholeskyQR3, Fukaya et al., SIAM J. Sci. Comput. 42 (2020)); the
 // failure flag is reset there.  Later factorisations are not shifted (a
 // failure sets the flag and the caller orthonormalises by Householder) unless
-// `retry_all`: then every factorisation that meets a non-positive pivot is
-// retried with the shift.  The mixed-precision mode uses that: its sketches of
-// rank-48 updates are often too ill-conditioned for FP32 CholeskyQR3, the
-// Householder fallback costs several times the accum, and the factorization is
-// a preconditioner for FP64 refinement built from BF16 products anyway.
+// `retry_all` (mixed precision): then a later factorisation never fails on a
+// small pivot.  A pivot at or below chol_drop_tol times the largest diagonal
+// entry marks a column that depends on the earlier ones to within rounding:
+// its row of R is set to zero with an infinite diagonal (the inverses use
+// 1 / r_jj = 0), so that column of Q is zero and Q Q^T projects onto the
+// independent columns.  Shifting every pass instead (as the first) kept the
+// independent columns of rank-deficient sketches short of unit length, and
+// Q Q^T A came out about 10 % small; the Householder fallback costs several
+// times the accum, and the factorization is a preconditioner for FP64
+// refinement built from BF16 products anyway.
 // Entries below the diagonal are set to zero.
 template <typename T>
 __device__ __forceinline__ void chol_upper(T* G, int k, int rows, bool first, bool retry_all = false) {
   PP_LDS T* s = (PP_LDS T*)lds_base();  // k x (k + 1)
   const int sp = k + 1, tid = threadIdx.x, nt = blockDim.x;
   const T u = sizeof(T) == 8 ? T(1.1102230246251565e-16) : T(5.9604644775390625e-08);
+  const T inf = T(__builtin_inf());
   bool ok = false;
-  for (int attempt = 0; attempt < ((first || retry_all) ? 2 : 1) && !ok; ++attempt) {
+  for (int attempt = 0; attempt < (first ? 2 : 1) && !ok; ++attempt) {
     // eight loads in flight per thread (global address space, clamped
     // addresses: see narrow_solve's staging)
     for (int e0 = tid; e0 < k * k; e0 += 8 * nt) {
@@ -923,17 +936,25 @@ __device__ __forceinline__ void chol_upper(T* G, int k, int rows, bool first, bo
       for (int i = tid; i < k; i += nt) s[i * sp + i] += shift;
       __syncthreads();
     }
+    T drop = T(0);
+    if (retry_all && !first) {
+      T gmax = T(0);
+      for (int i = 0; i < k; ++i) gmax = fmax(gmax, s[i * sp + i]);
+      drop = chol_drop_tol<T>() * gmax;
+    }
     ok = true;
     for (int j = 0; j < k; ++j) {
       const T d = s[j * sp + j];  // every thread reads the same pivot
-      if (!(d > T(0)) || !(d < T(__builtin_inf()))) {
+      if (!(d < inf) || (drop == T(0) && !(d > T(0)))) {
         ok = false;
         break;
       }
-      const T r = sqrt(d), ri = T(1) / r;
+      const bool dep = drop > T(0) && !(d > drop);
+      const T r = dep ? inf : sqrt(d), ri = dep ? T(0) : T(1) / r;
       __syncthreads();
       for (int c = j + tid; c < k; c += nt) s[j * sp + c] = c == j ? r : s[j * sp + c] * ri;
       __syncthreads();
+      if (dep) continue;  // row j is zero right of the diagonal: no update
       const int m = k - j - 1;  // trailing upper triangle, row-major over (a, b)
       for (int e = tid; e < m * m; e += nt) {
         const int a = j + 1 + e / m, b = j + 1 + e % m;

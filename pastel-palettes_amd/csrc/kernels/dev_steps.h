@@ -12,6 +12,7 @@
 #pragma once
 
 #include "../h_ops/h_ops.h"
+#include "dev_accum.h"
 #include "dev_lds_lu.h"
 #include "dev_lu.h"
 
@@ -557,6 +558,16 @@ __device__ __forceinline__ int exec_entry(const int32_t* pc, void* const* ptrs, 
                                           int32_t* fallbacks = nullptr) {
   const int len = entry_length(word(pc, 2));
   if (len < 0) return -1;
+  if (word(pc, 2) == accum && accum_lds_ok<T>(word(pc, 11), word(pc, 12), word(pc, 13), word(pc, 14))) {
+    // a small block: the whole accumulation with the sketch in LDS; only
+    // when its factorisations fail does the step sequence below run, from
+    // the start (its Cholesky passes fail the same way, then Householder)
+    accum_lds<T, FMODE, GMODE>(slot<T>(ptrs, pc, 3) + word(pc, 7), word(pc, 15), slot<T>(ptrs, pc, 4) + word(pc, 8),
+                               word(pc, 16), slot<T>(ptrs, pc, 5) + word(pc, 9), word(pc, 17),
+                               slot<T>(ptrs, pc, 6) + word(pc, 10), word(pc, 18), word(pc, 11), word(pc, 12),
+                               word(pc, 13), word(pc, 14), reinterpret_cast<const T*>(rnd), GMODE == MODE_BF16);
+    if (*chol_fail_flag() == 0) return len;
+  }
   for (int s = 0;; ++s) {
     Step<T> st;
     const int r = make_step<T>(pc, ptrs, s, tmp, reinterpret_cast<const T*>(rnd), st);

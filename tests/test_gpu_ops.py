@@ -229,15 +229,20 @@ def test_gemm4(ctx, orc, prec, control):
 
 
 @pytest.mark.parametrize("prec", [FP64, FP32, MIXED])
-@pytest.mark.parametrize("nx,ny,k1,k2", [(256, 256, 32, 32), (300, 500, 16, 24), (1024, 2048, 32, 32),
-                                         (128, 96, 48, 8)])
-def test_accum(ctx, orc, pp, prec, nx, ny, k1, k2):
+@pytest.mark.parametrize("nx,ny,k1,k2,decay", [(256, 256, 32, 32, 0.5), (512, 512, 32, 32, 0.9),
+                                               (300, 500, 16, 24, 0.5), (64, 200, 32, 20, 0.9),
+                                               (1024, 2048, 32, 32, 0.5), (128, 96, 48, 8, 0.5)])
+def test_accum(ctx, orc, pp, prec, nx, ny, k1, k2, decay):
     """U1 V1^T after accum equals the oracle's (Givens-based) projection;
     the factors themselves may differ by column signs."""
     rng = np.random.default_rng(nx + ny + k1 + k2)
-    # a numerically rank-(k1) sum so the projection is well conditioned
+    # a numerically rank-(k1) sum so the projection is well conditioned: the
+    # k1-th singular value (decay^(k1-1)) stays above the 1e-3 update; with
+    # decay 0.5 and k1 = 32 that gap is absent and the projection onto the
+    # sketch's weakest directions is set by rounding (it holds to 1e-10 on
+    # the first shapes only)
     u1 = rng.standard_normal((ny, k1))
-    v1 = rng.standard_normal((nx, k1)) * (0.5 ** np.arange(k1))
+    v1 = rng.standard_normal((nx, k1)) * (decay ** np.arange(k1))
     u2 = rng.standard_normal((ny, k2)) * 1e-3
     v2 = rng.standard_normal((nx, k2))
     rnd = pp.sketch(pp.SKETCH_SEED, nx * k1)
@@ -253,6 +258,31 @@ def test_accum(ctx, orc, pp, prec, nx, ny, k1, k2):
     assert rel(got, want) < {FP64: 1e-10, FP32: 1e-4, MIXED: 5e-2}[prec]
     if prec == FP64:  # Q orthonormal
         assert np.max(np.abs(gu.T @ gu - np.eye(k1))) < 1e-12
+
+
+@pytest.mark.parametrize("prec", [FP64, FP32, MIXED])
+@pytest.mark.parametrize("nx,ny,k1,k2", [(256, 256, 32, 16), (512, 384, 32, 8), (256, 640, 32, 16)])
+def test_accum_rank_deficient(ctx, orc, pp, prec, nx, ny, k1, k2):
+    """The update cancels k2 of U1 V1^T's columns, so the sum has rank
+    k1 - k2 and the sketch's Gram matrix is singular: the shifted
+    factorisation (or the Householder fallback) must still give the
+    oracle's projection, which is the sum itself."""
+    rng = np.random.default_rng(7 * nx + ny)
+    u1 = rng.standard_normal((ny, k1))
+    v1 = rng.standard_normal((nx, k1))
+    u2, v2 = -u1[:, :k2].copy(), v1[:, :k2].copy()
+    rnd = pp.sketch(pp.SKETCH_SEED, nx * k1)
+    wu, wv = orc.accum(u1, v1, u2, v2, rnd)
+    want = wu @ wv.T
+    exact = u1[:, k2:] @ v1[:, k2:].T
+    assert rel(want, exact) < 1e-12  # the oracle recovers the sum
+    bufs = [dev(x, prec) for x in (u1, v1, u2, v2)]
+    o1 = (ny * k1 + 31) // 32 * 32
+    o2 = (o1 + nx * k1 + 31) // 32 * 32
+    words = [8, 0, 1, 2, 3, 0, 0, 0, 0, nx, ny, k1, k2, k1, k1, k2, k2, o1, o2]
+    run(ctx, prec, words, bufs, scratch_elems=o2 + ny * k1 + 4 * k1 * max(k1, k2) + 4096)
+    got = host(bufs[0]) @ host(bufs[1]).T
+    assert rel(got, want) < {FP64: 1e-10, FP32: 1e-4, MIXED: 5e-2}[prec]
 
 
 def test_single_rejects_unknown_operation(ctx, pp):

@@ -11,7 +11,13 @@ solve the 16 right-hand sides and refine in FP64 until the relative residual
 throughput; the plain FP64 path is measured beside it in the same run.
 The CPU baseline runs the same factorization and solve instruction streams
 with the CPU oracle executor on the host cores, refined in FP64 to the same
-tolerance (a bounded sample of about 10 s).
+tolerance (a bounded sample of about 10 s); its refined solution is the FP64
+oracle solution every GPU solution's error is stated against, and the
+reference's CPU configuration (n = 1024, all-dense, one right-hand side) is
+timed beside it.  The line also carries the reference's run and scheduler
+summaries, and two further records: the batched configuration (512 matrices
+in one persistent launch) and the serving configuration (a Poisson request
+stream, p50 / p99 latency at 80 % of the closed-loop peak).
 """
 import argparse
 import importlib
@@ -101,7 +107,7 @@ def cpu_baseline(pp, orc, hm, arena, b, threads, tol, min_seconds=10.0, max_reps
     executor runs the factorization and substitution-solve streams on
     `threads` workers, the residual uses a numpy H-matvec.  Repeated until
     `min_seconds` have passed (at most `max_reps` times); returns (seconds
-    per step, steps, worst residual, iterations)."""
+    per step, steps, worst residual, iterations, the refined solution)."""
     prog = pp.Program.factor(hm, threads)
     info = prog.info()
     sprog = pp.Program.solve(hm, b.shape[1], threads)
@@ -136,7 +142,7 @@ def cpu_baseline(pp, orc, hm, arena, b, threads, tol, min_seconds=10.0, max_reps
         worst, iters = max(worst, res), max(iters, it)
         if time.perf_counter() - t0 >= min_seconds or reps >= max_reps:
             break
-    return (time.perf_counter() - t0) / reps, reps, worst, iters
+    return (time.perf_counter() - t0) / reps, reps, worst, iters, x
 
 
 def cpu_model():
@@ -148,6 +154,35 @@ def cpu_model():
     except OSError:
         pass
     return "unknown"
+
+
+def measured_mfma_busy(precision):
+    """MFMA busy fraction of vm_factor_kernel from the committed PMC summary
+    (mean over its dispatches), or None when no summary is present."""
+    path = os.path.join(ROOT, "profiles", "vm_factor_kernel_mfma_hodlr_n16384_l6_r32.txt")
+    vals = []
+    try:
+        with open(path) as f:
+            for line in f:
+                parts = line.split()
+                if len(parts) == 7 and parts[0] == precision:
+                    vals.append(float(parts[6]))
+    except (OSError, ValueError):
+        return None
+    return sum(vals) / len(vals) if vals else None
+
+
+def summary_columns(pp, hm, prog, factor_flops, factor_ms):
+    """The reference's run summary (src/launcher.cu:174-212: actual and
+    dense-LU FLOPs, FLOPs compression ratio, GPU and equivalent dense-LU
+    rates) and scheduler summary (rts/scheduler.cpp:406-427)."""
+    tr = importlib.import_module("pastel-palettes_amd.trace")
+    dense = pp.dense_lu_flops(hm.n)
+    gflops = factor_flops / (factor_ms * 1e-3) / 1e9
+    return {"actual_flops": int(factor_flops), "dense_lu_flops": int(dense),
+            "flops_compression_pct": 100.0 * factor_flops / dense,
+            "gpu_gflops": gflops, "equivalent_dense_lu_tflops": dense / (factor_ms * 1e-3) / 1e12,
+            "scheduler": tr.schedule_summary(prog)}
 
 
 def measured_traffic(kernel):
@@ -189,9 +224,12 @@ def main():
                          "with autotuned leaves; serve: Poisson request stream; build: H-matrix construction "
                          "on the device")
     ap.add_argument("--batch-count", type=int, default=512)
-    ap.add_argument("--serve-seconds", type=float, default=20.0)
+    ap.add_argument("--serve-seconds", type=float, default=15.0)
     ap.add_argument("--serve-slots", type=int, default=4)
     ap.add_argument("--host-build", action="store_true", help="serve: build the matrices on the host")
+    ap.add_argument("--serve-policy", choices=["sjf", "fifo"], default="sjf")
+    ap.add_argument("--no-batch", action="store_true", help="hlu: skip the batched-configuration record")
+    ap.add_argument("--no-serve", action="store_true", help="hlu: skip the serving-configuration record")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path (run with --gpus 1)",
@@ -236,7 +274,7 @@ def main():
                                warmup=args.warmup, log=log)
         else:
             out = wl.run_serve(pp, torch, ctx, duration=args.serve_seconds, slots=args.serve_slots,
-                               device_build=not args.host_build, log=log)
+                               device_build=not args.host_build, policy=args.serve_policy, log=log)
         print(json.dumps(out), flush=True)
         return
 
@@ -269,17 +307,32 @@ def main():
     achieved = r["factor_flops"] / (r["factor_kernel_ms"] * 1e-3) / 1e12
     t_lb = lower_bound_ms(prog, head == "mixed")
 
-    cpu = None
+    cpu = cpu0 = None
+    errs = None
     if not args.no_cpu:
         import oracle
         threads = min(len(os.sched_getaffinity(0)), 16)
         log(f"[bench] cpu baseline on {threads} threads ({cpu_model()}, nproc {os.cpu_count()})")
-        sec, reps, cres, cit = cpu_baseline(pp, oracle, hm, arena, b, threads, args.tol)
+        sec, reps, cres, cit, xc = cpu_baseline(pp, oracle, hm, arena, b, threads, args.tol)
         cpu = {"value": 1.0 / sec, "unit": "factor+solve/s", "cores": threads, "kind": "port",
                "sample": f"{reps} x (FP64 factorization + {args.nrhs}-rhs solve + FP64 refinement to "
                          f"{args.tol:g}: {cit} iterations, residual {cres:.2e}) of the same n={args.n} "
                          f"HODLR streams by the oracle executor on {threads} threads; host {cpu_model()}, "
                          f"nproc {os.cpu_count()}"}
+        # every GPU solution against the FP64 oracle's (refined) solution
+        errs = {name: float(np.linalg.norm(xs[name] - xc) / np.linalg.norm(xc)) for name in xs}
+        # the reference's CPU configuration: n = 1024, 1-D exp kernel, all
+        # blocks dense (leaf 128), one right-hand side
+        hm0, _, arena0 = pp.build_config(0)
+        b0 = pp.rhs(0, hm0.n, 1)
+        sec0, reps0, cres0, cit0, _ = cpu_baseline(pp, oracle, hm0, arena0, b0, threads, args.tol, min_seconds=3.0,
+                                                   max_reps=50)
+        cpu0 = {"value": 1.0 / sec0, "unit": "factor+solve/s", "cores": threads, "kind": "port",
+                "flops": int(pp.dense_lu_flops(hm0.n)),
+                "gflops_factor_flops_per_step_time": pp.dense_lu_flops(hm0.n) / sec0 / 1e9,
+                "sample": f"{reps0} x (FP64 factorization + 1-rhs solve + FP64 refinement to {args.tol:g}: "
+                          f"{cit0} iterations, residual {cres0:.2e}) of n={hm0.n}, 1-D exp kernel, all-dense "
+                          f"leaf-128 streams by the oracle executor on {threads} threads"}
 
     line = {
         "metric": "H-LU factor+solve/sec & p99 latency at residual <=1e-10, mixed vs plain FP64",
@@ -310,7 +363,16 @@ def main():
                      "t_lb_note": "sum over the H-LU instructions of max(FLOPs / MFMA peak, algorithmic bytes / "
                                   "8 TB/s); the inverse construction in the same launch is not counted"},
         "cpu_baseline": cpu,
+        "cpu_baseline_n1024_dense": cpu0,
+        "error_vs_fp64_oracle": errs,
+        "summary": summary_columns(pp, hm, prog, r["factor_flops"], r["factor_kernel_ms"]),
     }
+    traffic = line["roofline"]["traffic"]
+    line["roofline"]["hbm_gbps"] = traffic / (r["factor_kernel_ms"] * 1e-3) / 1e9 if traffic else None
+    line["roofline"]["mfma_busy_frac"] = measured_mfma_busy(head)
+    line["roofline"]["counters_note"] = ("traffic and mfma_busy_frac are read from the committed rocprofv3 PMC "
+                                         "summaries of vm_factor_kernel (profiles/); hbm_gbps = traffic / the "
+                                         "factorization launch time measured here")
     if "fp64" in res:
         f = res["fp64"]
         line["fp64"] = {"value": args.steps / f["seconds"], "ms_per_step": f["per_step_ms"],
@@ -323,6 +385,20 @@ def main():
         line["mixed_over_fp64"] = line["value"] / line["fp64"]["value"]
         line["mixed_vs_fp64_solution_rel_diff"] = float(np.linalg.norm(xs["mixed"] - xs["fp64"]) /
                                                         np.linalg.norm(xs["fp64"]))
+    # the batched and serving configurations, each a bounded run of its own
+    # workload (bench.py --workload batch|serve runs them alone)
+    wl = importlib.import_module("pastel-palettes_amd.workloads")
+    if not args.no_batch:
+        log("[bench] batch record")
+        bt = wl.run_batch(pp, torch, ctx, count=args.batch_count, steps=2, warmup=1, log=log)
+        line["batch"] = {k: bt[k] for k in ("workload", "matrices_per_s", "ms_per_batch", "factor_ms", "factor_tflops",
+                                            "residual", "iterations", "converged", "leaf_choices", "solve_mode",
+                                            "tuned_buckets")}
+    if not args.no_serve:
+        log("[bench] serve record")
+        sv = wl.run_serve(pp, torch, ctx, duration=args.serve_seconds, slots=args.serve_slots,
+                          policy=args.serve_policy, log=log)
+        line["serve"] = {k: v for k, v in sv.items() if k not in ("sizes",)}
     print(json.dumps(line), flush=True)
 
 

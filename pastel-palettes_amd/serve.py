@@ -13,7 +13,19 @@ reuse them.
 
 The slot count times the workers per slot never exceeds the number of VM
 workgroups that fit co-resident on the device (the persistent kernels of
-all slots must be resident together).
+all slots must be resident together).  That product is the server's worker
+pool.  With a `share` function each request's factorization gets a worker
+count sized by its matrix instead of a whole slot (a small matrix's DAG has
+little parallelism to spread), and a request is admitted only while the
+workers of the requests on the device, its own included, fit in the pool,
+so their persistent launches are always co-resident.
+
+Admission order: "fifo" takes the first admissible request in arrival
+order; "sjf" takes the admissible request with the smallest estimated
+service time less `aging` times its time in the queue (shortest job first,
+and a long job's priority grows as it waits, so it is not starved).  The
+estimate of a key's factor + solve and of its solve alone starts from a size
+model and follows the measured service times (exponential average).
 
 Every request's arithmetic is independent of the others (static schedule,
 one workgroup per instruction, fixed reduction orders), so its result is the
@@ -48,22 +60,37 @@ class Request:
 
 
 class _Resident:
-    __slots__ = ("solver", "bytes", "in_use", "last_used", "key")
+    __slots__ = ("solver", "bytes", "in_use", "last_used", "key", "workers")
 
-    def __init__(self, key, solver, nbytes):
-        self.key, self.solver, self.bytes = key, solver, nbytes
+    def __init__(self, key, solver, nbytes, workers):
+        self.key, self.solver, self.bytes, self.workers = key, solver, nbytes, workers
         self.in_use = 0
         self.last_used = time.perf_counter()
+
+
+def default_share(n, leaf=256):
+    """Workers for a factorization of order n: four per diagonal leaf, at
+    least 16 and at most 128 (the DAG of a HODLR factorization has about as
+    many independent low-rank updates per step as leaves below the current
+    one)."""
+    return int(min(128, max(16, 4 * n // leaf)))
 
 
 class Server:
     """HBM-budgeted, multi-slot request loop over pp.Solver."""
 
     def __init__(self, pp, torch, ctx, matrices, precision, budget_bytes, slots=4, workers_per_slot=64,
-                 wavefronts=4, tol=1e-10, max_iter=60, nrhs_max=64):
+                 wavefronts=4, tol=1e-10, max_iter=60, nrhs_max=64, share=None, policy="fifo", aging=1.0):
         cap = ctx.max_workers(precision, wavefronts)
         if slots * workers_per_slot > cap:
             raise ValueError(f"{slots} slots x {workers_per_slot} workers exceed co-resident capacity {cap}")
+        if policy not in ("fifo", "sjf"):
+            raise ValueError(f"unknown admission policy {policy!r}")
+        self.pool = slots * workers_per_slot
+        self.free_workers = self.pool
+        self.share = share
+        self.policy, self.aging = policy, aging
+        self.est = {}  # (key, reuse) -> exponential average of the service time (s)
         self.pp, self.torch, self.ctx = pp, torch, ctx
         self.matrices = matrices  # key -> (hm, device FP64 arena tensor)
         self.precision, self.budget = precision, budget_bytes
@@ -99,14 +126,15 @@ class Server:
             r.solver.close()
         self.resident.clear()
 
-    def _estimate(self, key):
+    def _estimate(self, key, workers=None):
         hm, _ = self.matrices[key]
+        workers = self.workers if workers is None else workers
         eb = 8 if self.precision == self.pp.FP64 else 4
         n = hm.n
         # FP64 arena + factor arena (both with the leaf-inverse area) + rhs/residual
         # buffers + per-worker scratch (upper bound)
         elems = getattr(hm, "solver_elems", hm.storage)
-        return elems * (8 + eb) + n * self.nrhs_max * (eb + 8) + self.workers * 64 * 1024 * 1024 // 16
+        return elems * (8 + eb) + n * self.nrhs_max * (eb + 8) + workers * 64 * 1024 * 1024 // 16
 
     def _evict_for(self, need):
         """Evict idle resident factorizations (LRU) until `need` bytes fit."""
@@ -121,27 +149,60 @@ class Server:
             self.evictions += 1
         return fits()
 
+    def _workers_for(self, key):
+        if self.share is None:
+            return self.workers
+        return int(min(self.pool, max(1, self.share(self.matrices[key][0].n))))
+
+    def _service_estimate(self, req):
+        """Estimated service time (s) of a request: measured average of its
+        key (factor + solve, or solve alone for a resident factorization),
+        else a size model (a chain of diagonal leaves for the factorization)."""
+        res = self.resident.get(req.key)
+        solve_only = req.reuse and res is not None and res.solver is not None
+        k = (req.key, solve_only)
+        if k in self.est:
+            return self.est[k]
+        n = self.matrices[req.key][0].n
+        return 2e-3 + (0.0 if solve_only else 0.35e-3 * n / 256.0)
+
+    def _order(self):
+        """Queue positions in the order the policy offers them."""
+        if self.policy == "fifo":
+            return range(len(self.queue))
+        now = time.perf_counter()
+        pri = [self._service_estimate(q) - self.aging * (now - q.arrival) for q in self.queue]
+        return sorted(range(len(self.queue)), key=lambda i: pri[i])
+
     def _take(self):
         """Next admissible request and its resident entry (or None to build)."""
         while True:
             if self.stop and not self.queue:
                 return None, None
-            for i, req in enumerate(self.queue):
+            for i in self._order():
+                req = self.queue[i]
                 res = self.resident.get(req.key)
                 if res is not None and res.in_use == 0 and res.solver is not None:
+                    if res.workers > self.free_workers:
+                        break  # the first in order waits for workers; no one passes it
                     del self.queue[i]
                     res.in_use += 1
+                    self.free_workers -= res.workers
                     return req, res
                 if res is None:
-                    need = self._estimate(req.key)
+                    w = self._workers_for(req.key)
+                    if w > self.free_workers:
+                        break
+                    need = self._estimate(req.key, w)
                     if self._evict_for(need):
                         del self.queue[i]
                         # reserve the key so no other slot builds it concurrently
-                        placeholder = _Resident(req.key, None, need)
+                        placeholder = _Resident(req.key, None, need, w)
                         placeholder.in_use = 1
                         self.resident[req.key] = placeholder
                         self.bytes_resident += need
                         self.reserved += need
+                        self.free_workers -= w
                         return req, placeholder
             self.lock.wait(timeout=0.05)
 
@@ -161,7 +222,7 @@ class Server:
                     hm, arena = self.matrices[req.key]
                     try:
                         res.solver = pp.Solver(self.ctx, hm, arena.data_ptr(), precision=self.precision,
-                                               workers=self.workers, wavefronts=self.wavefronts,
+                                               workers=res.workers, wavefronts=self.wavefronts,
                                                nrhs_max=self.nrhs_max)
                     finally:
                         with self.lock:
@@ -197,6 +258,11 @@ class Server:
             with self.lock:
                 res.in_use -= 1
                 res.last_used = req.done
+                self.free_workers += res.workers
+                if req.error is None:
+                    k = (req.key, not req.factored)
+                    dt = req.done - req.start
+                    self.est[k] = dt if k not in self.est else 0.7 * self.est[k] + 0.3 * dt
                 self.completed.append(req)
                 self.lock.notify_all()
             req.event.set()

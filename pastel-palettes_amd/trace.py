@@ -125,3 +125,25 @@ def critical_path(program, trace):
     for inst, op, t0, t1 in path:
         by_op[OP_NAMES.get(op, str(op))] += t1 - t0
     return path, dict(by_op)
+
+
+def schedule_summary(program):
+    """The reference scheduler's summary columns for a program's streams
+    (instructions_scheduler, rts/scheduler.cpp:406-427): instruction count,
+    the heaviest worker's share of the FLOPs, utilisation (total FLOPs over
+    workers x heaviest worker's FLOPs) and cross-worker waits per
+    instruction."""
+    info = program.info()
+    per_worker = []
+    for s in program.streams():
+        per_worker.append(sum(program.inst_flops(i) for kind, i, _ in _entries(s) if kind == "exec"))
+    total = float(sum(per_worker))
+    heaviest = float(max(per_worker)) if per_worker else 0.0
+    n = info["n_insts"]
+    return {
+        "instructions": n,
+        "workers": info["workers"],
+        "heaviest_worker_pct": 100.0 * heaviest / total if total else 0.0,
+        "utilization_pct": 100.0 * total / heaviest / len(per_worker) if heaviest else 0.0,
+        "comm_per_inst": info["comm_waits"] / n if n else 0.0,
+    }

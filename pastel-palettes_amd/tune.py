@@ -58,8 +58,11 @@ class Autotuner:
         pp, torch = self.pp, self.torch
         levels = hodlr_levels(n, leaf)
         hm = pp.HMatrix("hodlr", n, levels=levels, rank=self.rank)
-        arena = pp.fill_arena(hm, pp.Kernel(pp.points_2d_morton(17, n), "inv2d", n / 4.0))
-        dev = torch.from_numpy(arena).cuda()
+        # the candidate matrix is built on the device (the timing does not
+        # depend on how its low-rank blocks were compressed)
+        pts = torch.from_numpy(np.ascontiguousarray(pp.points_2d_morton(17, n))).cuda()
+        dev = torch.empty(hm.storage, dtype=torch.float64, device="cuda")
+        hm.build_on_device(self.ctx, pp.KERNEL_INV2D, pts.data_ptr(), 1e-3, n / 4.0, dev.data_ptr())
         s = pp.Solver.batch(self.ctx, [hm] * self.copies, [dev.data_ptr()] * self.copies, precision=self.precision,
                             workers=self.workers, wavefronts=wavefronts, nrhs_max=self.nrhs)
         b = np.concatenate([pp.rhs(3, n, self.nrhs)] * self.copies, axis=0)

@@ -146,11 +146,26 @@ def run_batch(pp, torch, ctx, count=512, nrhs=4, steps=3, warmup=1, tol=1e-10, p
     return out
 
 
-def run_serve(pp, torch, ctx, keys=12, duration=20.0, load=0.8, burst=48, slots=4, tol=1e-10, budget_gb=200,
-              device_build=True, log=print):
+def _size_bucket(n):
+    return 1 << int(round(math.log2(n)))
+
+
+def run_serve(pp, torch, ctx, keys=32, duration=20.0, load=0.8, burst=64, slots=4, tol=1e-10, budget_gb=200,
+              device_build=True, policy="sjf", share=True, log=print):
+    """Serving benchmark: `keys` matrices (N log-uniform in [1024, 32768],
+    the largest set to 32768) built on the device; every matrix factored and
+    solved once alone (its alone service times, factor + solve and solve
+    alone); the closed-loop peak over a burst of the request mix; then a
+    Poisson stream at `load` times that peak for `duration` seconds.
+    Reports overall p50 / p99 latency, per-size-bucket latencies, and the
+    slowdown (latency over the request's alone service time), whose spread
+    is what the admission policy controls: the spread of the raw latencies is
+    bounded below by the mix's service times, which grow with N (a chain of
+    diagonal-leaf factorizations)."""
     from . import serve
     rng = np.random.default_rng(5000)
     sizes = log_uniform_sizes(rng, keys, 1024, 32768, 256)
+    sizes[int(np.argmax(sizes))] = 32768
     jobs = [(n, max(1, int(round(math.log2(n / 256)))), 32, 200 + i) for i, n in enumerate(sizes)]
     log(f"[serve] building {keys} matrices (N = {sizes}) on the {'device' if device_build else 'host'}")
     t0 = time.perf_counter()
@@ -168,20 +183,31 @@ def run_serve(pp, torch, ctx, keys=12, duration=20.0, load=0.8, burst=48, slots=
         for i, ((n, lv, r, _), a) in enumerate(zip(jobs, arenas)):
             matrices[f"m{i}"] = (pp.HMatrix("hodlr", n, levels=lv, rank=r), torch.from_numpy(a).cuda())
         del arenas
+    torch.cuda.synchronize()
     build_s = time.perf_counter() - t0
     log(f"[serve] built in {build_s:.1f} s")
     names = sorted(matrices)
     workers = min(ctx.cu_count, 256) // slots
     server = serve.Server(pp, torch, ctx, matrices, pp.MIXED, int(budget_gb * 1e9), slots=slots,
-                          workers_per_slot=workers, tol=tol)
+                          workers_per_slot=workers, tol=tol, policy=policy,
+                          share=serve.default_share if share else None)
 
     def make_rhs(key, nrhs, k):
         return pp.rhs(7 + k % 97, matrices[key][0].n, nrhs)
 
     try:
-        # warm: every matrix factored once and resident
+        # every factorization made resident (solver construction included),
+        # then each matrix's alone service times: factor + solve, and solve
+        # with the resident factors, one request on the device at a time
+        t0 = time.perf_counter()
         serve.replay(server, [(0.0, k, 1, False) for k in names], make_rhs, realtime=False)
-        log("[serve] all matrices resident")
+        alone = {}
+        for k in names:
+            for reuse in (False, True):
+                r, _ = serve.replay(server, [(0.0, k, 1, reuse)], make_rhs, realtime=False)
+                alone[(k, reuse)] = r[0].done - r[0].start
+        setup_s = time.perf_counter() - t0
+        log(f"[serve] all matrices resident ({setup_s:.1f} s)")
         # peak: a closed burst (all requests queued at once) of the same mix
         tr = serve.poisson_trace(1e9, burst / 1e9, names, seed=11)[:burst]
         reqs, wall = serve.replay(server, [(0.0, k, nr, ru) for _, k, nr, ru in tr], make_rhs, realtime=False)
@@ -191,18 +217,21 @@ def run_serve(pp, torch, ctx, keys=12, duration=20.0, load=0.8, burst=48, slots=
         trace = serve.poisson_trace(rate, duration, names, seed=12)
         reqs, wall = serve.replay(server, trace, make_rhs, realtime=True)
         s = serve.summarize(reqs, wall)
+        ok = [q for q in reqs if q.error is None]
         by_size = {}
-        for q in reqs:
-            if q.error is None:
-                by_size.setdefault(matrices[q.key][0].n, []).append(q.latency)
+        for q in ok:
+            by_size.setdefault(_size_bucket(matrices[q.key][0].n), []).append(q.latency)
         per_n = {str(n): {"count": len(v), "p50_ms": 1e3 * float(np.percentile(v, 50)),
                           "p99_ms": 1e3 * float(np.percentile(v, 99))} for n, v in sorted(by_size.items())}
+        slow = np.array([q.latency / alone[(q.key, not q.factored)] for q in ok])
+        svc = np.array([alone[(q.key, not q.factored)] for q in ok])
     finally:
         server.close()
     return {
         "workload": f"serving: Poisson requests over {keys} resident HODLR matrices, N log-uniform [1024, 32768], "
                     f"rank 32, leaf ~256, 1-64 rhs, 30% reuse, mixed precision + FP64 refinement to {tol:g}, "
-                    f"{slots} slots x {workers} workers, HBM budget {budget_gb} GB, matrices built on the "
+                    f"{slots} streams sharing {slots * workers} workers ({'worker share by N' if share else 'fixed slots'}), "
+                    f"{policy} admission, HBM budget {budget_gb} GB, matrices built on the "
                     f"{'device' if device_build else 'host'}",
         "sizes": sizes,
         "peak_rps": peak["throughput_rps"],
@@ -210,9 +239,14 @@ def run_serve(pp, torch, ctx, keys=12, duration=20.0, load=0.8, burst=48, slots=
         "throughput_rps": s["throughput_rps"],
         "p50_ms": s["p50_ms"], "p99_ms": s["p99_ms"],
         "p99_over_p50": s["p99_ms"] / s["p50_ms"] if s["p50_ms"] else None,
+        "alone_service_p50_ms": 1e3 * float(np.percentile(svc, 50)),
+        "alone_service_p99_ms": 1e3 * float(np.percentile(svc, 99)),
+        "alone_service_p99_over_p50": float(np.percentile(svc, 99) / np.percentile(svc, 50)),
+        "slowdown_p50": float(np.percentile(slow, 50)), "slowdown_p99": float(np.percentile(slow, 99)),
+        "slowdown_p99_over_p50": float(np.percentile(slow, 99) / np.percentile(slow, 50)),
         "requests": s["requests"], "errors": s["errors"], "factored": s["factored"],
         "max_residual": s["max_residual"], "converged": s["converged"],
-        "per_size": per_n, "evictions": server.evictions, "build_s": build_s,
+        "per_size": per_n, "evictions": server.evictions, "build_s": build_s, "setup_s": setup_s,
     }
 
 

@@ -223,3 +223,99 @@ def test_failed_factorization_evicts_the_entry():
     srv.close()
     assert again.error is None and again.factored
     assert FakeSolver.factor_calls == 2
+
+
+class TimedSolver(FakeSolver):
+    """Fake solver whose factorization takes time proportional to n and
+    records how many workers are on the device at once."""
+    active = 0
+    peak = 0
+    lock = threading.Lock()
+
+    def __init__(self, ctx, hm, ptr, precision, workers, wavefronts, nrhs_max):
+        super().__init__(ctx, hm, ptr, precision, workers, wavefronts, nrhs_max)
+        self.workers = workers
+
+    def factor(self, stream=None):
+        with TimedSolver.lock:
+            TimedSolver.active += self.workers
+            TimedSolver.peak = max(TimedSolver.peak, TimedSolver.active)
+        time.sleep(self.hm.n * 1e-6)
+        with TimedSolver.lock:
+            TimedSolver.active -= self.workers
+
+
+class TimedPP(FakePP):
+    Solver = TimedSolver
+
+
+def _timed_server(policy, slots, share, sizes):
+    ctx = FakeCtx()
+    mats = {f"m{n}": (FakeHM(n), Arena()) for n in sizes}
+    srv = serve.Server(TimedPP, FakeTorch, ctx, mats, FakePP.MIXED, budget_bytes=1 << 40, slots=slots,
+                       workers_per_slot=32, nrhs_max=2, share=share, policy=policy)
+    return srv
+
+
+def test_worker_pool_never_overcommitted():
+    """With per-request worker shares the workers of the factorizations on
+    the device never exceed the pool (slots x workers per slot), so every
+    persistent launch is co-resident."""
+    TimedSolver.active = TimedSolver.peak = 0
+    sizes = (1000, 4000, 8000, 16000, 32000)
+    share = lambda n: 8 if n < 5000 else 64  # noqa: E731
+    srv = _timed_server("sjf", 4, share, sizes)
+    assert srv.pool == 128
+    reqs = [serve.Request(i, f"m{sizes[i % len(sizes)]}", np.ones((5, 1))) for i in range(20)]
+    for r in reqs:
+        srv.submit(r)
+    for r in reqs:
+        assert r.event.wait(20)
+    srv.close()
+    assert all(r.error is None for r in reqs)
+    assert 0 < TimedSolver.peak <= 128
+    assert srv.free_workers == srv.pool
+
+
+def test_sjf_serves_short_requests_first():
+    """One slot, everything queued at once: shortest job first finishes the
+    small factorizations before the large one that arrived with them, and
+    FIFO keeps arrival order."""
+    sizes = (1000, 20000)
+    order = {}
+    for policy in ("fifo", "sjf"):
+        srv = _timed_server(policy, 1, None, sizes)
+        srv.lock.acquire()  # queue everything before the slot takes any
+        reqs = [serve.Request(0, "m20000", np.ones((5, 1)))]
+        reqs += [serve.Request(i, "m1000", np.ones((5, 1))) for i in range(1, 5)]
+        for r in reqs:
+            srv.queue.append(r)
+        srv.lock.release()
+        with srv.lock:
+            srv.lock.notify_all()
+        for r in reqs:
+            assert r.event.wait(20)
+        srv.close()
+        order[policy] = [r.rid for r in sorted(reqs, key=lambda r: r.done)]
+    assert order["fifo"][0] == 0
+    assert order["sjf"][-1] == 0
+
+
+def test_sjf_aging_serves_a_waiting_long_request():
+    """A long request is not starved by a steady stream of short ones: its
+    priority grows with its time in the queue."""
+    srv = _timed_server("sjf", 1, None, (1000, 20000))
+    srv.aging = 50.0
+    long_req = serve.Request(0, "m20000", np.ones((5, 1)))
+    srv.submit(long_req)
+    shorts = []
+    t0 = time.perf_counter()
+    while not long_req.event.is_set() and time.perf_counter() - t0 < 5.0:
+        r = serve.Request(len(shorts) + 1, "m1000", np.ones((5, 1)))
+        shorts.append(srv.submit(r))
+        time.sleep(0.0005)
+    assert long_req.event.wait(10)
+    for r in shorts:
+        r.event.wait(10)
+    srv.close()
+    assert long_req.error is None

@@ -74,6 +74,41 @@ static_assert(AccLds<float>::BYTES <= kLdsBytes - 16, "accum LDS layout (FP32)")
 static_assert(3 * acc_chunk_sketch<double>() * AccLds<double>::P <= AccLds<double>::S1, "sketch staging (FP64)");
 static_assert(3 * acc_chunk_sketch<float>() * AccLds<float>::P <= AccLds<float>::S1, "sketch staging (FP32)");
 
+// Rows of a chunk of the sketch in LDS when the sketch itself stays in the
+// worker scratch (accum_big)
+template <typename T>
+constexpr int acc_big_chunk() { return sizeof(T) == 8 ? 96 : 256; }
+
+// Layout of accum_big (elements of T): the chunk of Y [acc_big_chunk][P] at
+// 0 (the sketch's staging uses [0, 3 acc_chunk_sketch P) before it), S1, S2,
+// RT, G as in AccLds, the staging of U1 and U2 chunks, the status word.
+template <typename T>
+struct AccBig {
+  static constexpr int P = acc_pitch<T>(), CH = acc_big_chunk<T>();
+  static constexpr int YC = 0;
+  static constexpr int S1 = (3 * acc_chunk_sketch<T>() > CH ? 3 * acc_chunk_sketch<T>() : CH) * P;
+  static constexpr int S2 = S1 + NB * P;
+  static constexpr int RT = S2 + NB * P;
+  static constexpr int G = RT + NB * P;
+  static constexpr int STG = G + NB * (NB + 1);
+  static constexpr int END = STG + 2 * CH * P;
+  static constexpr int BYTES = END * (int)sizeof(T) + 16;
+  __device__ static __forceinline__ PP_LDS T* at(int off) { return (PP_LDS T*)lds_base() + off; }
+  __device__ static __forceinline__ volatile PP_LDS int* status() {
+    return (volatile PP_LDS int*)((PP_LDS T*)lds_base() + END);
+  }
+};
+static_assert(AccBig<double>::BYTES <= kLdsBytes - 16, "streamed accum LDS layout (FP64)");
+static_assert(AccBig<float>::BYTES <= kLdsBytes - 16, "streamed accum LDS layout (FP32)");
+
+// Whether accum_big handles an accumulation of these dimensions (larger
+// sketches than accum_lds holds; the sketch Y is kept in the worker scratch,
+// ny x k1 with pitch k1, as the step form keeps it).
+template <typename T>
+__device__ __forceinline__ bool accum_big_ok(int nx, int ny, int k1, int k2) {
+  return blockDim.x == 256 && k1 >= 1 && k1 <= NB && k2 >= 1 && k2 <= NB && ny > acc_max_rows<T>() && nx >= 1;
+}
+
 // Whether accum_lds handles an accumulation of these dimensions.
 template <typename T>
 __device__ __forceinline__ bool accum_lds_ok(int nx, int ny, int k1, int k2) {
@@ -501,6 +536,177 @@ __device__ __attribute__((noinline)) void accum_lds(T* U1, int ld_u1, T* V1, int
     const int i = e / k1, j = e - i * k1;
     gstore<T>(U1 + (int64_t)i * ld_u1 + j, Y[i * P + j]);
   }
+  if (tid == 0) *chol_fail_flag() = 0;
+  __syncthreads();
+}
+
+// Y chunk (LDS, rows [0, n) of the chunk at row r0) to the sketch in the
+// worker scratch (pitch k1: columns < k1 only).
+template <typename T>
+__device__ __forceinline__ void acc_chunk_out(T* Yg, int k1, int r0, int n, const PP_LDS T* Yc) {
+  constexpr int P = acc_pitch<T>();
+  for (int e = threadIdx.x; e < n * k1; e += blockDim.x) {
+    const int i = e / k1, j = e - i * k1;
+    gstore<T>(Yg + (int64_t)(r0 + i) * k1 + j, Yc[i * P + j]);
+  }
+}
+
+// The accumulation for sketches too tall for LDS: the same steps, products
+// and precisions as accum_lds, with Y (ny x k1) in the worker scratch and
+// streamed through LDS in chunks of acc_big_chunk rows.  Each chunk is
+// formed once (Y = U1 S1^T + U2 S2^T, stored, its Gram contribution added),
+// and each later CholeskyQR pass reads it once (Y <- Y R^{-1}, stored, the
+// next Gram contribution added); the projection reads the chunks of Q, U1
+// and U2 together and writes Q over U1's rows right after their products
+// are taken.  Failure handling as accum_lds (U1, V1 untouched; the flag
+// tells the caller to run the step form).
+template <typename T, int FM, int GM>
+__device__ __attribute__((noinline)) void accum_big(T* U1, int ld_u1, T* V1, int ld_v1, const T* U2, int ld_u2,
+                                                    const T* V2, int ld_v2, int nx, int ny, int k1, int k2,
+                                                    const T* rnd, T* Yg, bool retry_all) {
+  using L = AccBig<T>;
+  constexpr int P = L::P, CH = L::CH;
+  using PG = Mma<T, GM>;
+  using PF = Mma<T, FM>;
+  const int tid = threadIdx.x, nt = blockDim.x, w = wave_id();
+  const int qi = w >> 1, qj = w & 1;
+  PP_LDS T* Yc = L::at(L::YC);
+  PP_LDS T* S1 = L::at(L::S1);
+  PP_LDS T* S2 = L::at(L::S2);
+  PP_LDS T* RT = L::at(L::RT);
+  PP_LDS T* G = L::at(L::G);
+  PP_LDS T* stg = L::at(L::STG);
+  volatile PP_LDS int* status = L::status();
+  const T u = sizeof(T) == 8 ? T(1.1102230246251565e-16) : T(5.9604644775390625e-08);
+
+  // 1. sketch products S1 = Omega^T V1, S2 = Omega^T V2 (as accum_lds)
+  {
+    constexpr int CS = acc_chunk_sketch<T>();
+    typename PG::acc_t a1[2] = {PG::zero(), PG::zero()}, a2[2] = {PG::zero(), PG::zero()};
+    const T* src[3] = {rnd, V1, V2};
+    const int ld[3] = {k1, ld_v1, ld_v2}, cols[3] = {k1, k1, k2};
+    for (int r0 = 0; r0 < nx; r0 += CS) {
+      acc_stage<T, 3, CS>(Yc, src, ld, cols, r0, nx);
+      const int n = nx - r0 < CS ? nx - r0 : CS, kn = (n + PG::KW - 1) / PG::KW * PG::KW;
+      acc_quad<T, GM>(Yc, Yc + CS * P, kn, qi, qj, a1);
+      acc_quad<T, GM>(Yc, Yc + 2 * CS * P, kn, qi, qj, a2);
+      __syncthreads();
+    }
+    acc_quad_store<T, GM>(S1, P, qi, qj, a1);
+    acc_quad_store<T, GM>(S2, P, qi, qj, a2);
+    __syncthreads();
+  }
+  // 2. Y = U1 S1^T + U2 S2^T chunk by chunk, stored, with G = Y^T Y
+  typename PF::acc_t ag[2] = {PF::zero(), PF::zero()};
+  for (int r0 = 0; r0 < ny; r0 += CH) {
+    const int n = ny - r0 < CH ? ny - r0 : CH, npad = (n + 31) & ~31;
+    acc_rows_2<T, GM, true>(U1 + (int64_t)r0 * ld_u1, ld_u1, k1, U2 + (int64_t)r0 * ld_u2, ld_u2, k2, n, S1, S2, Yc,
+                            nullptr, 0, NB);
+    __syncthreads();
+    acc_chunk_out<T>(Yg, k1, r0, n, Yc);
+    acc_quad<T, FM>(Yc, Yc, npad, qi, qj, ag);
+    __syncthreads();
+  }
+  // 3. shifted CholeskyQR3 over the stored chunks
+  bool ok = true, orthonormal = false;
+  for (int p = 0; p < 3; ++p) {
+    acc_quad_store<T, FM>(G, NB + 1, qi, qj, ag);
+    __syncthreads();
+    if (p > 0) {
+      int off = 0;
+      for (int e = tid; e < NB * NB; e += nt) {
+        const int i = e >> 5, j = e & 31;
+        if (i < k1 && j < k1) off |= !(__builtin_fabs(G[i * (NB + 1) + j] - (i == j ? T(1) : T(0))) <= T(64) * u);
+      }
+      if (!__syncthreads_or(off)) {
+        orthonormal = true;
+        break;
+      }
+    }
+    const int attempts = p == 0 ? 2 : 1;
+    bool done = false;
+    for (int at = 0; at < attempts && !done; ++at) {
+      if (at == 1) {  // the shifted retry starts from the Gram matrix again
+        acc_quad_store<T, FM>(G, NB + 1, qi, qj, ag);
+        __syncthreads();
+      }
+      if (w == 0) {
+        if (at == 1) {
+          T tr = T(0);
+          for (int i = 0; i < k1; ++i) tr += G[i * (NB + 1) + i];
+          const T shift = T(11) * (T(ny) * T(k1) + T(k1) * T(k1 + 1)) * u * tr;
+          lds_order();
+          if (lane_id() < k1) G[lane_id() * (NB + 1) + lane_id()] += shift;
+        }
+        if (lane_id() >= k1 && lane_id() < NB) G[lane_id() * (NB + 1) + lane_id()] = T(1);
+        lds_order();
+        T drop = T(0);
+        if (retry_all && p > 0) {
+          T gmax = T(0);
+          for (int i = 0; i < k1; ++i) gmax = fmax(gmax, G[i * (NB + 1) + i]);
+          drop = chol_drop_tol<T>() * gmax;
+        }
+        const bool good = acc_chol_rinv<T>(G, RT, drop);
+        if (lane_id() == 0) *status = good ? 1 : 0;
+      }
+      __syncthreads();
+      done = *status != 0;
+      __syncthreads();
+    }
+    if (!done) {
+      ok = false;
+      break;
+    }
+    // Y <- Y R^{-1} chunk by chunk, with the next pass's Gram matrix
+    for (int i = 0; i < 2; ++i) ag[i] = PF::zero();
+    for (int r0 = 0; r0 < ny; r0 += CH) {
+      const int n = ny - r0 < CH ? ny - r0 : CH, npad = (n + 31) & ~31;
+      const T* src[1] = {Yg};
+      const int ld[1] = {k1}, cols[1] = {k1};
+      acc_stage<T, 1, CH>(Yc, src, ld, cols, r0, ny);
+      acc_rows_rinv<T, FM>(Yc, npad, RT);
+      __syncthreads();
+      acc_chunk_out<T>(Yg, k1, r0, n, Yc);
+      acc_quad<T, FM>(Yc, Yc, npad, qi, qj, ag);
+      __syncthreads();
+    }
+  }
+  (void)orthonormal;
+  if (!ok) {
+    if (tid == 0) *chol_fail_flag() = 1;
+    __syncthreads();
+    return;
+  }
+  // 4. projection P1 = Q^T U1, P2 = Q^T U2 chunk by chunk; Q's rows go over
+  // U1's once their products are taken
+  {
+    typename PG::acc_t a1[2] = {PG::zero(), PG::zero()}, a2[2] = {PG::zero(), PG::zero()};
+    for (int r0 = 0; r0 < ny; r0 += CH) {
+      const int n = ny - r0 < CH ? ny - r0 : CH, kn = (n + PG::KW - 1) / PG::KW * PG::KW;
+      {
+        const T* src[1] = {Yg};
+        const int ld[1] = {k1}, cols[1] = {k1};
+        acc_stage<T, 1, CH>(Yc, src, ld, cols, r0, ny);
+      }
+      {
+        const T* src[2] = {U1, U2};
+        const int ld[2] = {ld_u1, ld_u2}, cols[2] = {k1, k2};
+        acc_stage<T, 2, CH>(stg, src, ld, cols, r0, ny);
+      }
+      acc_quad<T, GM>(Yc, stg, kn, qi, qj, a1);
+      acc_quad<T, GM>(Yc, stg + CH * P, kn, qi, qj, a2);
+      for (int e = tid; e < n * k1; e += nt) {
+        const int i = e / k1, j = e - i * k1;
+        gstore<T>(U1 + (int64_t)(r0 + i) * ld_u1 + j, Yc[i * P + j]);
+      }
+      __syncthreads();
+    }
+    acc_quad_store<T, GM>(S1, P, qi, qj, a1);
+    acc_quad_store<T, GM>(S2, P, qi, qj, a2);
+    __syncthreads();
+  }
+  // V1 <- V1 P1^T + V2 P2^T (in place)
+  acc_rows_2<T, GM, false>(V1, ld_v1, k1, V2, ld_v2, k2, nx, S1, S2, nullptr, V1, ld_v1, k1);
   if (tid == 0) *chol_fail_flag() = 0;
   __syncthreads();
 }

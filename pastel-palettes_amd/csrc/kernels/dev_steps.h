@@ -558,15 +558,25 @@ __device__ __forceinline__ int exec_entry(const int32_t* pc, void* const* ptrs, 
                                           int32_t* fallbacks = nullptr) {
   const int len = entry_length(word(pc, 2));
   if (len < 0) return -1;
-  if (word(pc, 2) == accum && accum_lds_ok<T>(word(pc, 11), word(pc, 12), word(pc, 13), word(pc, 14))) {
-    // a small block: the whole accumulation with the sketch in LDS; only
-    // when its factorisations fail does the step sequence below run, from
-    // the start (its Cholesky passes fail the same way, then Householder)
-    accum_lds<T, FMODE, GMODE>(slot<T>(ptrs, pc, 3) + word(pc, 7), word(pc, 15), slot<T>(ptrs, pc, 4) + word(pc, 8),
-                               word(pc, 16), slot<T>(ptrs, pc, 5) + word(pc, 9), word(pc, 17),
-                               slot<T>(ptrs, pc, 6) + word(pc, 10), word(pc, 18), word(pc, 11), word(pc, 12),
-                               word(pc, 13), word(pc, 14), reinterpret_cast<const T*>(rnd), GMODE == MODE_BF16);
-    if (*chol_fail_flag() == 0) return len;
+  if (word(pc, 2) == accum) {
+    // the whole accumulation in one routine: the sketch in LDS (small
+    // blocks) or streamed through LDS from the worker scratch; only when its
+    // factorisations fail does the step sequence below run, from the start
+    // (its Cholesky passes fail the same way, then Householder)
+    const int nx = word(pc, 11), ny = word(pc, 12), k1 = word(pc, 13), k2 = word(pc, 14);
+    T* U1 = slot<T>(ptrs, pc, 3) + word(pc, 7);
+    T* V1 = slot<T>(ptrs, pc, 4) + word(pc, 8);
+    const T* U2 = slot<T>(ptrs, pc, 5) + word(pc, 9);
+    const T* V2 = slot<T>(ptrs, pc, 6) + word(pc, 10);
+    const int l1 = word(pc, 15), l2 = word(pc, 16), l3 = word(pc, 17), l4 = word(pc, 18);
+    const T* om = reinterpret_cast<const T*>(rnd);
+    if (accum_lds_ok<T>(nx, ny, k1, k2)) {
+      accum_lds<T, FMODE, GMODE>(U1, l1, V1, l2, U2, l3, V2, l4, nx, ny, k1, k2, om, GMODE == MODE_BF16);
+      if (*chol_fail_flag() == 0) return len;
+    } else if (accum_big_ok<T>(nx, ny, k1, k2)) {
+      accum_big<T, FMODE, GMODE>(U1, l1, V1, l2, U2, l3, V2, l4, nx, ny, k1, k2, om, tmp, GMODE == MODE_BF16);
+      if (*chol_fail_flag() == 0) return len;
+    }
   }
   for (int s = 0;; ++s) {
     Step<T> st;

@@ -230,29 +230,37 @@ __device__ __forceinline__ void gemm_tiles(int m, int n, int k, T alpha, T beta,
   auto step = [&](auto cur_c) {
     constexpr int cur = decltype(cur_c)::value;
     if (ci == 0) {
-      // the whole C tile is requested before any value is used (one memory
-      // round trip per tile); addresses are clamped as in OperandLane
-      T cv[2][2][4];
+      if (ks == 0 && beta != T(0)) {
+        // the whole C tile is requested before any value is used (one
+        // memory round trip per tile); addresses are clamped as in
+        // OperandLane.  With beta == 0 (or for the other K slices) C is
+        // not read at all.
+        T cv[2][2][4];
 #pragma unroll
-      for (int i = 0; i < 2; ++i)
+        for (int i = 0; i < 2; ++i)
 #pragma unroll
-        for (int j = 0; j < 2; ++j)
+          for (int j = 0; j < 2; ++j)
 #pragma unroll
-          for (int r = 0; r < 4; ++r) {
-            const int row = r0 + i * 16 + P::orow(l, r), col = c0 + j * 16 + li;
-            cv[i][j][r] = gload<T>(C + (int64_t)(row < m ? row : m - 1) * ldc + (col < n ? col : n - 1));
-          }
-      __builtin_amdgcn_sched_barrier(0);
+            for (int r = 0; r < 4; ++r) {
+              const int row = r0 + i * 16 + P::orow(l, r), col = c0 + j * 16 + li;
+              cv[i][j][r] = gload<T>(C + (int64_t)(row < m ? row : m - 1) * ldc + (col < n ? col : n - 1));
+            }
+        __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-      for (int i = 0; i < 2; ++i)
+        for (int i = 0; i < 2; ++i)
 #pragma unroll
-        for (int j = 0; j < 2; ++j)
+          for (int j = 0; j < 2; ++j)
 #pragma unroll
-          for (int r = 0; r < 4; ++r) {
-            const int row = r0 + i * 16 + P::orow(l, r), col = c0 + j * 16 + li;
-            const bool in = ks == 0 && beta != T(0) && row < m && col < n;
-            acc[i][j][r] = in ? ab * cv[i][j][r] : T(0);
-          }
+            for (int r = 0; r < 4; ++r) {
+              const int row = r0 + i * 16 + P::orow(l, r), col = c0 + j * 16 + li;
+              acc[i][j][r] = (row < m && col < n) ? ab * cv[i][j][r] : T(0);
+            }
+      } else {
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+          for (int j = 0; j < 2; ++j) acc[i][j] = P::zero();
+      }
     }
     int nti = ti, nci = ci + 1;
     if (nci == chunks_eff) {
@@ -297,9 +305,8 @@ __device__ __forceinline__ void gemm_tiles(int m, int n, int k, T alpha, T beta,
 }
 
 // C = beta * C + alpha * op(A) op(B); C m x n, op(A) m x k, op(B) k x n;
-// alpha must be +1 or -1.  With beta == 0 the old C is ignored (it is read,
-// so C must be allocated).  Uses LDS only for the split-K reduction.  Ends
-// with a barrier.
+// alpha must be +1 or -1.  With beta == 0 the old C is not read.  Uses LDS
+// only for the split-K reduction.  Ends with a barrier.
 template <typename T, int MODE>
 __device__ __forceinline__ void gemm_block(int m, int n, int k, T alpha, T beta, T* C, int ldc, const T* A, int lda,
                                            bool aT, const T* B, int ldb, bool bT) {

@@ -151,7 +151,8 @@ def test_trmm(ctx, orc, prec, ny, nx, pad):
     assert rel(got, want) < TOL[FP64 if prec == FP64 else FP32] * 10
 
 
-GEMM_SHAPES = [(256, 256, 256), (37, 53, 29), (4096, 32, 32), (32, 32, 4096), (1, 300, 17), (256, 256, 16),
+GEMM_SHAPES = [(256, 256, 256), (37, 53, 29), (4096, 32, 32), (32, 32, 4096), (1, 300, 17), (256, 256, 16), (256, 256, 32),
+               (200, 136, 24),
                (65, 33, 100)]
 
 

@@ -304,9 +304,155 @@ __device__ __forceinline__ void gemm_tiles(int m, int n, int k, T alpha, T beta,
   }
 }
 
+// ---------------------------------------------------------------- staged gemm
+// Updates whose operands are both k-contiguous and whose output tiles are all
+// whole (m, n multiples of 32, at least one tile per wave).  op(A) (m x k)
+// and op(B)^T (n x k) are copied into LDS once by LDS DMA
+// (global_load_lds_dwordx4: 64 lanes x 16 bytes per instruction, every copy
+// of the workgroup in flight at once), so the fragments of every output tile
+// come from LDS instead of a global round trip per tile; the C tiles are
+// requested through a ring of register buffers RING tiles ahead.  Chunk c of
+// staged row r holds the row's 16-byte chunk c ^ (r mod cpr), so the rows
+// read by one fragment load spread over the banks.
+// FP32 storage forms the transposed tile (op(B)^T op(A)^T on the MFMA): each
+// lane's four accumulators are then four consecutive columns of C, which
+// moves as 16-byte vectors.  FP64 keeps the product's orientation (sixteen
+// lanes cover a 128-byte row segment).  Every element sees the same MFMA
+// sequence as in gemm_tiles (C scaled first, K chunks in order), so results
+// are identical.
+constexpr int kStagedBytes = 128 * 1024;
+
+template <typename T, int MODE>
+__device__ __forceinline__ bool gemm_staged_ok(int m, int n, int k, const T* A, int lda, bool a_along, const T* B,
+                                               int ldb, bool b_along, const T* C, int ldc, int nw) {
+  constexpr int KW = Mma<T, MODE>::KW, ES = (int)sizeof(T);
+  const int cpr = k * ES / 16;  // 16-byte chunks per staged row
+  const int64_t bytes = (int64_t)(m + n) * k * ES;
+  return a_along && b_along && m % 32 == 0 && n % 32 == 0 && (m >> 5) * (n >> 5) >= nw && k % KW == 0 &&
+         cpr >= 2 && (cpr & (cpr - 1)) == 0 && bytes <= kStagedBytes && bytes % 1024 == 0 &&
+         (reinterpret_cast<uintptr_t>(A) & 15) == 0 && (reinterpret_cast<uintptr_t>(B) & 15) == 0 &&
+         (lda * ES) % 16 == 0 && (ldb * ES) % 16 == 0 &&
+         (ES == 8 || ((reinterpret_cast<uintptr_t>(C) & 15) == 0 && ldc % 4 == 0));
+}
+
+template <typename T, int MODE, bool BETA>
+__device__ __forceinline__ void gemm_staged(int m, int n, int k, T alpha, T beta, T* C, int ldc, const T* A, int lda,
+                                            const T* B, int ldb) {
+  using P = Mma<T, MODE>;
+  using acc_t = typename P::acc_t;
+  constexpr bool TR = sizeof(T) == 4;
+  constexpr int KW = P::KW, EPL = P::EPL, V = 16 / (int)sizeof(T), RING = 2;
+  typedef T vec_t __attribute__((ext_vector_type(V)));
+  const int nw = blockDim.x >> 6, w = wave_id(), l = lane_id(), g = l >> 4, li = l & 15;
+  const int tiles_n = n >> 5, ntiles = (m >> 5) * tiles_n;
+  const int my = (ntiles - w + nw - 1) / nw;  // >= 1: every wave has a tile
+  const int cpr = k / V, lg = __builtin_ctz(cpr), nch = k / KW;
+  const T ab = alpha * beta;
+  PP_LDS T* sa = (PP_LDS T*)lds_base();  // op(A) rows 0 .. m-1, op(B)^T rows m .. m+n-1, pitch k
+  acc_t cb[RING][2][2];
+  auto tile_rc = [&](int q, int& r0, int& c0) {
+    const int t = w + (q < my ? q : my - 1) * nw, tr = t / tiles_n;
+    r0 = tr * 32;
+    c0 = (t - tr * tiles_n) * 32;
+  };
+  // element r of accumulator (i, j): FP32 C[r0 + 16 j + li][c0 + 16 i + 4 g + r],
+  // FP64 C[r0 + 16 i + orow(l, r)][c0 + 16 j + li]
+  auto cptr = [&](int r0, int c0, int i, int j, int r) -> T* {
+    if constexpr (TR) return C + (int64_t)(r0 + 16 * j + li) * ldc + c0 + 16 * i + 4 * g + r;
+    else return C + (int64_t)(r0 + 16 * i + P::orow(l, r)) * ldc + c0 + 16 * j + li;
+  };
+  auto load_c = [&](auto s_c, int q) {
+    constexpr int s = decltype(s_c)::value;
+    int r0, c0;
+    tile_rc(q, r0, c0);
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+      for (int j = 0; j < 2; ++j) {
+        if constexpr (TR) {
+          cb[s][i][j] = gload<f4_t>(cptr(r0, c0, i, j, 0));
+        } else {
+#pragma unroll
+          for (int r = 0; r < 4; ++r) cb[s][i][j][r] = gload<T>(cptr(r0, c0, i, j, r));
+        }
+      }
+  };
+  // the first C tiles are requested before the copies; the copies are issued
+  // after them, so once the copies have landed (vmcnt 0) so have these
+  if constexpr (BETA) {
+    load_c(IC<0>{}, 0);
+    load_c(IC<1>{}, 1);
+  }
+  {
+    const int ninst = (m + n) * cpr / 64;
+    for (int i = w; i < ninst; i += nw) {
+      const int q = 64 * i + l, row = q >> lg, pc = q & (cpr - 1), lc = pc ^ (row & (cpr - 1));
+      const T* src = row < m ? A + (int64_t)row * lda + V * lc : B + (int64_t)(row - m) * ldb + V * lc;
+      __builtin_amdgcn_global_load_lds((const void*)src, (PP_LDS void*)(sa + 64 * V * i), 16, 0, 0);
+    }
+    __builtin_amdgcn_s_waitcnt(0);
+  }
+  __syncthreads();
+  auto body = [&](auto s_c, int q) {
+    constexpr int s = decltype(s_c)::value;
+    int r0, c0;
+    tile_rc(q, r0, c0);
+    acc_t acc[2][2];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+      for (int j = 0; j < 2; ++j) acc[i][j] = BETA ? ab * cb[s][i][j] : P::zero();
+    if constexpr (BETA) load_c(IC<s>{}, q + RING);
+    for (int c = 0; c < nch; ++c) {
+      T fa[2][EPL], fb[2][EPL];
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        const int ra = r0 + 16 * h + li, rb = m + c0 + 16 * h + li;
+#pragma unroll
+        for (int v = 0; v < EPL / V; ++v) {
+          const int ch = (c * KW + EPL * g) / V + v;
+          const vec_t x = *(const PP_LDS vec_t*)(sa + ra * k + V * (ch ^ (ra & (cpr - 1))));
+          const vec_t y = *(const PP_LDS vec_t*)(sa + rb * k + V * (ch ^ (rb & (cpr - 1))));
+#pragma unroll
+          for (int e = 0; e < V; ++e) {
+            fa[h][V * v + e] = x[e];
+            fb[h][V * v + e] = y[e];
+          }
+        }
+      }
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+          if constexpr (TR) acc[i][j] = P::mma_chunk(fb[i], fa[j], acc[i][j]);
+          else acc[i][j] = P::mma_chunk(fa[i], fb[j], acc[i][j]);
+        }
+    }
+    if (q < my) {
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+          if constexpr (TR) {
+            gstore<f4_t>(cptr(r0, c0, i, j, 0), alpha * acc[i][j]);
+          } else {
+#pragma unroll
+            for (int r = 0; r < 4; ++r) gstore<T>(cptr(r0, c0, i, j, r), alpha * acc[i][j][r]);
+          }
+        }
+    }
+  };
+  for (int q0 = 0; q0 < my; q0 += RING) {
+    body(IC<0>{}, q0);
+    body(IC<1>{}, q0 + 1);
+  }
+  __syncthreads();
+}
+
 // C = beta * C + alpha * op(A) op(B); C m x n, op(A) m x k, op(B) k x n;
 // alpha must be +1 or -1.  With beta == 0 the old C is not read.  Uses LDS
-// only for the split-K reduction.  Ends with a barrier.
+// for the staged operands (gemm_staged) or the split-K reduction.  Ends with
+// a barrier.
 template <typename T, int MODE>
 __device__ __forceinline__ void gemm_block(int m, int n, int k, T alpha, T beta, T* C, int ldc, const T* A, int lda,
                                            bool aT, const T* B, int ldb, bool bT) {
@@ -337,6 +483,11 @@ __device__ __forceinline__ void gemm_block(int m, int n, int k, T alpha, T beta,
   const int my_tiles = grp < ntiles ? (ntiles - grp + groups - 1) / groups : 0;
   const int my_chunks = ks < nch ? (nch - ks + gk - 1) / gk : 0;
   const bool a_along = !aT, b_along = bT, full = (k % KW) == 0;
+  if (gemm_staged_ok<T, MODE>(m, n, k, A, lda, a_along, B, ldb, b_along, C, ldc, nw)) {
+    if (beta != T(0)) gemm_staged<T, MODE, true>(m, n, k, alpha, beta, C, ldc, A, lda, B, ldb);
+    else gemm_staged<T, MODE, false>(m, n, k, alpha, beta, C, ldc, A, lda, B, ldb);
+    return;
+  }
   const bool a_vec = full && a_along && ((reinterpret_cast<uintptr_t>(A) & 15) == 0) && (lda % V) == 0;
   const bool b_vec = full && b_along && ((reinterpret_cast<uintptr_t>(B) & 15) == 0) && (ldb % V) == 0;
   acc_t acc[2][2];

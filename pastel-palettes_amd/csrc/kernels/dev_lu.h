@@ -75,12 +75,28 @@ __device__ __forceinline__ void diag_inverses(const T* F, int ldf, int n, T* out
   const int nw = blockDim.x >> 6, w = wave_id(), l = lane_id();
   const int half = l >> 5, lc = l & 31;
   const int nblk = (n + NB - 1) / NB;
+  constexpr int V = 16 / (int)sizeof(T);
+  typedef T vec_t __attribute__((ext_vector_type(V)));
+  const bool fvec = (n % NB) == 0 && (ldf % V) == 0 && (reinterpret_cast<uintptr_t>(F) & 15) == 0;
   PP_LDS T* s = (PP_LDS T*)lds_base() + (2 * w + half) * NB * (NB + 1);
   for (int b0 = 0; b0 < nblk; b0 += 2 * nw) {
     const int b = b0 + 2 * w + half;
     const int r0 = b * NB;
     const int nb = b < nblk ? ((n - r0) < NB ? (n - r0) : NB) : 0;
-    if (b < nblk) {
+    if (b < nblk && fvec) {
+      // lane lc loads row lc of the block as whole 16-byte vectors, all at
+      // once; the reciprocal of the diagonal is taken from LDS
+      T v[NB];
+#pragma unroll
+      for (int q = 0; q < NB / V; ++q) {
+        const vec_t x = gload<vec_t>(F + (int64_t)(r0 + lc) * ldf + r0 + q * V);
+#pragma unroll
+        for (int e = 0; e < V; ++e) v[q * V + e] = x[e];
+      }
+#pragma unroll
+      for (int j = 0; j < NB; ++j) s[lc * (NB + 1) + j] = v[j];
+      if (UPPER) s[lc * (NB + 1) + NB] = T(1) / (T)s[lc * (NB + 1) + lc];
+    } else if (b < nblk) {
       // eight loads in flight per lane, through the global address space
       // with clamped addresses (see narrow_solve's staging)
       for (int e0 = lc; e0 < NB * NB; e0 += 8 * 32) {
@@ -532,12 +548,36 @@ __device__ __forceinline__ void narrow_solve(T* Wg, int ldw, bool w_trans, int m
   PP_LDS T* Ws = (PP_LDS T*)lds_base();
   const int tid = threadIdx.x, nt = blockDim.x, wv = wave_id(), nw = nt >> 6, l = lane_id(), li = l & 15;
   // stage W (in the order that keeps global accesses coalesced); rows >= m
-  // and columns >= w are zero and stay zero
-  // Eight loads in flight per thread: the loads go through the global
-  // address space with clamped addresses (no branch), so the compiler may
-  // issue them ahead of the LDS stores of the same batch (a flat load could
-  // alias LDS and would wait for each store before it).
-  {
+  // and columns >= w are zero and stay zero.  The loads go through the
+  // global address space with clamped addresses (no branch), so the compiler
+  // may issue them ahead of the LDS stores of the same batch (a flat load
+  // could alias LDS and would wait for each store before it).
+  constexpr int V = 16 / (int)sizeof(T);
+  typedef T vec_t __attribute__((ext_vector_type(V)));
+  // W row-major with whole 16-byte vectors: every vector of the staged block
+  // is requested at once (one memory round trip for up to 16 vectors per
+  // thread); rows >= m and columns >= w read a clamped address and are zeroed
+  const bool wvec = !w_trans && (w % V) == 0 && (ldw % V) == 0 && (reinterpret_cast<uintptr_t>(Wg) & 15) == 0;
+  if (wvec) {
+    constexpr int U = 16;
+    const int vpr = wc / V, total = mr * vpr;
+    for (int e0 = tid; e0 < total; e0 += U * nt) {
+      vec_t v[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int e = e0 + u * nt, r = e / vpr, c = (e - r * vpr) * V;
+        const bool in = e < total && r < m && c < w;
+        v[u] = gload<vec_t>(Wg + (in ? (int64_t)r * ldw + c : 0));
+        if (!in) v[u] = vec_t{};
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int e = e0 + u * nt, r = e / vpr, c = (e - r * vpr) * V;
+        if (e < total) *(PP_LDS vec_t*)(Ws + r * wp + c) = v[u];
+      }
+    }
+  } else {
+    // element loads, eight in flight per thread
     constexpr int U = 8;
     const int total = mr * wc;
     for (int e0 = tid; e0 < total; e0 += U * nt) {
@@ -641,6 +681,15 @@ __device__ __forceinline__ void narrow_solve(T* Wg, int ldw, bool w_trans, int m
     __syncthreads();
   }
   // write W back
+  if (wvec) {
+    const int vpr = w / V, total = m * vpr;
+    for (int e = tid; e < total; e += nt) {
+      const int r = e / vpr, c = (e - r * vpr) * V;
+      gstore<vec_t>(Wg + (int64_t)r * ldw + c, *(const PP_LDS vec_t*)(Ws + r * wp + c));
+    }
+    __syncthreads();
+    return;
+  }
   for (int e = tid; e < m * w; e += nt) {
     if (w_trans) {
       const int c = e / m, r = e - c * m;

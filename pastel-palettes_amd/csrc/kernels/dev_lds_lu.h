@@ -7,11 +7,12 @@
 // there and copied back: no global round trip between the panels, which
 // dominates the leaf factorization when every panel and trailing update
 // goes through global memory.  Per 32-column panel:
-//   1. wave 0 factors the diagonal block from registers (pivot rows by
-//      v_readlane, as panel_lu), publishing multiplier columns and groups of
-//      final rows with the pivot reciprocals; wave 1 forms L11^{-1} from the
-//      multiplier columns; waves 2-3 eliminate the rows below against each
-//      published group (one row per thread);
+//   1. waves 0 and 1 factor the diagonal block from registers, each holding
+//      half of its columns (pivot rows by v_readlane, as panel_lu),
+//      publishing multiplier columns and groups of final rows with the pivot
+//      reciprocals; wave 0 then forms L11^{-1} from the multiplier columns;
+//      waves 2-3 eliminate the rows below against each published group (one
+//      row per thread);
 //   2. U12 = L11^{-1} A12 on MFMA, one 32-column tile per wave, in place;
 //   3. A22 -= L21 U12 on MFMA, 32 x 32 tiles dealt round-robin to the
 //      waves, both operands and C read from LDS.
@@ -31,8 +32,8 @@ constexpr int lds_lu_max() { return sizeof(T) == 8 ? 128 : 160; }
 // with pitch SP = MAXN + one 16-byte vector whatever the order (rows are
 // read and written as whole vectors), then the panel's work
 // area: lc [NB][NB] multiplier columns (column j at lc[j NB + i]), linv
-// [NB][NB] L11^{-1} (row-major), rinv [NB] pivot reciprocals, two
-// publication counters.
+// [NB][NB] L11^{-1} (row-major), rinv [NB] pivot reciprocals, three
+// publication counters (16 bytes).
 template <typename T, int MAXN = lds_lu_max<T>()>
 struct LdsLu {
   static constexpr int SP = MAXN + 16 / (int)sizeof(T);  // 16-byte aligned rows
@@ -64,13 +65,95 @@ __device__ __forceinline__ void lds_row_read(const PP_LDS T* p, T (&v)[NB]) {
   }
 }
 
+// Half HALF (columns HALF NB/2 .. of row l, lanes l < NB) of the diagonal
+// block of the panel, factored by one wave.  Pivot j belongs to half j / H.
+// In its own half the wave forms the multipliers of pivot j (publishing
+// multiplier column j, its reciprocal, and cnt[0] = j + 1) and updates its
+// later columns; wave 1 applies pivots j < H to its columns with the
+// published multipliers.  Each element sees the updates of the one-wave
+// elimination in the same order, so the factors are the same.  Rows whose
+// half is final are written to D in groups of GRP (cnt[1 + HALF]); wave 0's
+// columns are final after pivot H - 1, when it writes all remaining rows.
+template <typename T, int MAXN, int HALF>
+__device__ __forceinline__ void lds_diag_half(PP_LDS T* D, PP_LDS T* keep_rinv) {
+  using L = LdsLu<T, MAXN>;
+  constexpr int GRP = sizeof(T) == 4 ? 4 : 8;  // rows per publication
+  constexpr int H = NB / 2, C0 = HALF * H;
+  constexpr int sp = L::SP;
+  const int l = lane_id();
+  PP_LDS T* const lc = L::lc();
+  PP_LDS T* const rinvs = L::rinv();
+  volatile PP_LDS int* const cnt = L::cnt();
+  const bool own = l < NB;
+  T x[H];
+#pragma unroll
+  for (int c = 0; c < H; ++c) x[c] = own ? (T)D[l * sp + C0 + c] : T(0);
+  // reciprocal of the next pivot, formed from the broadcast pivot so that
+  // every lane takes the same branch of pivot_recip
+  T rinv = HALF == 0 ? pivot_recip(bcast_lane(x[0], 0)) : T(0);
+  int seen = 0;  // multiplier columns known to be published (HALF 1)
+#pragma unroll
+  for (int j = 0; j < (HALF == 0 ? H : NB); ++j) {
+    const bool act = own && l > j;
+    if (j / H == HALF) {
+      // pivot j is in this half, column jj of x
+      const int jj = j - C0;
+      const T rj = rinv;
+      // inactive lanes use a zero multiplier: x - 0 u = x
+      const T lv = act ? x[jj] * rj : T(0);
+      if (act) {
+        x[jj] = lv;
+        lc[j * NB + l] = lv;
+      }
+      if (l == 0) {
+        rinvs[j] = rj;
+        if (keep_rinv != nullptr) keep_rinv[j] = rj;
+      }
+      if (jj + 1 < H) {
+        const T u = bcast_lane(x[jj + 1], j);
+        x[jj + 1] -= lv * u;
+        rinv = pivot_recip(bcast_lane(x[jj + 1], j + 1));
+      }
+#pragma unroll
+      for (int c = jj + 2; c < H; ++c) x[c] -= lv * bcast_lane(x[c], j);
+      lds_order();
+      if (l == 0) cnt[0] = j + 1;
+    } else {
+      // HALF 1, pivot j < H: the multipliers come from wave 0
+      if (seen <= j) {
+        while ((seen = cnt[0]) <= j) __builtin_amdgcn_s_sleep(1);
+        lds_order();
+      }
+      const T lv = act ? (T)lc[j * NB + l] : T(0);
+#pragma unroll
+      for (int c = 0; c < H; ++c) x[c] -= lv * bcast_lane(x[c], j);
+      if (j == H - 1) rinv = pivot_recip(bcast_lane(x[0], H));
+    }
+    if ((j % GRP) == GRP - 1) {
+      // rows <= j have final values in this half (HALF 0 after pivot H - 1:
+      // every row)
+      const bool put = HALF == 0 && j == H - 1 ? (own && l > j - GRP) : (own && l <= j && l > j - GRP);
+      if (put) {
+#pragma unroll
+        for (int c = 0; c < H; ++c) D[l * sp + C0 + c] = x[c];
+      }
+      lds_order();
+      if (l == 0) cnt[1 + HALF] = HALF == 0 && j == H - 1 ? NB : j + 1;
+    }
+  }
+}
+
 // Step 1 for the panel at k0 of the np x np matrix.
 // keep_linv / keep_rinv (optional): L11^{-1} of the diagonal block
 // (row-major, pitch NB) and its pivot reciprocals are also left there.
+// Waves 0 and 1 factor the diagonal block (lds_diag_half); wave 0 then forms
+// L11^{-1} from the multiplier columns; waves 2-3 eliminate the rows below
+// against each published group of pivot rows.
 template <typename T, int MAXN>
 __device__ __forceinline__ void lds_panel(int k0, int np, PP_LDS T* keep_linv = nullptr, PP_LDS T* keep_rinv = nullptr) {
   using L = LdsLu<T, MAXN>;
   constexpr int GRP = sizeof(T) == 4 ? 4 : 8;  // rows per publication
+  constexpr int H = NB / 2;
   constexpr int sp = L::SP;
   const int tid = threadIdx.x, l = lane_id(), w = wave_id();
   PP_LDS T* D = L::mat() + k0 * sp + k0;
@@ -81,56 +164,19 @@ __device__ __forceinline__ void lds_panel(int k0, int np, PP_LDS T* keep_linv = 
   if (tid == 0) {
     cnt[0] = 0;
     cnt[1] = 0;
+    cnt[2] = 0;
   }
   __syncthreads();
   if (w == 0) {
-    const bool own = l < NB;
-    T x[NB];
-#pragma unroll
-    for (int c = 0; c < NB; ++c) x[c] = own ? (T)D[l * sp + c] : T(0);
-    // reciprocal of the next pivot, formed from the broadcast pivot so that
-    // every lane takes the same branch of pivot_recip
-    T rinv = pivot_recip(bcast_lane(x[0], 0));
-    T my_rinv = rinv;
-#pragma unroll
-    for (int j = 0; j < NB; ++j) {
-      const T rj = rinv;
-      const bool act = own && l > j;
-      // inactive lanes use a zero multiplier: x - 0 u = x
-      const T lv = act ? x[j] * rj : T(0);
-      if (act) {
-        x[j] = lv;
-        lc[j * NB + l] = lv;
-      }
-      if (j + 1 < NB) {
-        const T u = bcast_lane(x[j + 1], j);
-        x[j + 1] -= lv * u;
-        rinv = pivot_recip(bcast_lane(x[j + 1], j + 1));
-        if (l == j + 1) my_rinv = rinv;
-      }
-#pragma unroll
-      for (int c = j + 2; c < NB; ++c) x[c] -= lv * bcast_lane(x[c], j);
-      lds_order();
-      if (l == 0) cnt[0] = j + 1;
-      if ((j % GRP) == GRP - 1) {
-        if (own && l <= j && l > j - GRP) {
-#pragma unroll
-          for (int c = 0; c < NB; ++c) D[l * sp + c] = x[c];
-          rinvs[l] = my_rinv;
-          if (keep_rinv != nullptr) keep_rinv[l] = my_rinv;
-        }
-        lds_order();
-        if (l == 0) cnt[1] = j + 1;
-      }
-    }
-  } else if (w == 1) {
+    lds_diag_half<T, MAXN, 0>(D, keep_rinv);
+    // L11^{-1}, column by column as the multiplier columns are published
     const bool own = l < NB;
     T e[NB];
 #pragma unroll
     for (int c = 0; c < NB; ++c) e[c] = (c == l) ? T(1) : T(0);
 #pragma unroll
     for (int j = 0; j < NB; ++j) {
-      lds_wait_ge(&cnt[0], j + 1);  // multiplier column j is in LDS
+      if (j >= H) lds_wait_ge(&cnt[0], j + 1);  // multiplier column j is in LDS
       const bool act = own && l > j;
       const T lij = act ? (T)lc[j * NB + l] : T(0);
 #pragma unroll
@@ -144,6 +190,8 @@ __device__ __forceinline__ void lds_panel(int k0, int np, PP_LDS T* keep_linv = 
         for (int c = 0; c < NB; ++c) keep_linv[l * NB + c] = e[c];
       }
     }
+  } else if (w == 1) {
+    lds_diag_half<T, MAXN, 1>(D, keep_rinv);
   } else {
     const int bt = tid - 128, below = np - k0 - NB, pw = np - k0 < NB ? np - k0 : NB;
     const bool on = bt < below;
@@ -156,7 +204,12 @@ __device__ __forceinline__ void lds_panel(int k0, int np, PP_LDS T* keep_linv = 
       // pw == NB, but not known to the compiler: the unrolled steps are then
       // kept apart instead of interleaved (which spills xa)
       if (j >= pw) continue;
-      if ((j % GRP) == 0) lds_wait_ge(&cnt[1], j + GRP);
+      if ((j % GRP) == 0) {
+        // both halves of pivot rows j .. j + GRP - 1 (the left half of rows
+        // >= H is not read: only columns > j are)
+        if (j < H) lds_wait_ge(&cnt[1], j + GRP);
+        lds_wait_ge(&cnt[2], j + GRP);
+      }
       // the whole pivot row is requested before any of it is used (one LDS
       // round trip per step)
       T pr[NB];

@@ -475,8 +475,9 @@ __device__ __forceinline__ void lds_order() {
   __builtin_amdgcn_wave_barrier();
 }
 
-// Waits until the LDS counter reaches `target` (written only by wave 0 of the
-// panel, which never waits on the other waves, so the wait always ends).
+// Waits until the LDS counter reaches `target`.  Every such wait ends: the
+// wave that writes a counter waits for nothing, or only for counters whose
+// writers wait for nothing.
 __device__ __forceinline__ void lds_wait_ge(volatile PP_LDS int* c, int target) {
   while (*c < target) __builtin_amdgcn_s_sleep(1);
   lds_order();

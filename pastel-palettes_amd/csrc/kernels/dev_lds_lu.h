@@ -101,14 +101,14 @@ __device__ __forceinline__ void lds_diag_half(PP_LDS T* D, PP_LDS T* keep_rinv) 
       const T rj = rinv;
       // inactive lanes use a zero multiplier: x - 0 u = x
       const T lv = act ? x[jj] * rj : T(0);
-      if (act) {
-        x[jj] = lv;
-        lc[j * NB + l] = lv;
-      }
-      if (l == 0) {
-        rinvs[j] = rj;
-        if (keep_rinv != nullptr) keep_rinv[j] = rj;
-      }
+      x[jj] = act ? lv : x[jj];
+      // stores without lane masks: lanes <= j store a zero multiplier that
+      // is never read, lanes >= NB one into the next column (written at the
+      // next pivot; after column NB - 1, into linv, which wave 0 writes after
+      // the last column is published); every lane stores the same reciprocal
+      lc[j * NB + l] = lv;
+      rinvs[j] = rj;
+      if (keep_rinv != nullptr) keep_rinv[j] = rj;
       if (jj + 1 < H) {
         const T u = bcast_lane(x[jj + 1], j);
         x[jj + 1] -= lv * u;
@@ -117,14 +117,15 @@ __device__ __forceinline__ void lds_diag_half(PP_LDS T* D, PP_LDS T* keep_rinv) 
 #pragma unroll
       for (int c = jj + 2; c < H; ++c) x[c] -= lv * bcast_lane(x[c], j);
       lds_order();
-      if (l == 0) cnt[0] = j + 1;
+      cnt[0] = j + 1;
     } else {
       // HALF 1, pivot j < H: the multipliers come from wave 0
       if (seen <= j) {
         while ((seen = cnt[0]) <= j) __builtin_amdgcn_s_sleep(1);
         lds_order();
       }
-      const T lv = act ? (T)lc[j * NB + l] : T(0);
+      const T lcv = lc[j * NB + l];
+      const T lv = act ? lcv : T(0);
 #pragma unroll
       for (int c = 0; c < H; ++c) x[c] -= lv * bcast_lane(x[c], j);
       if (j == H - 1) rinv = pivot_recip(bcast_lane(x[0], H));
@@ -138,7 +139,7 @@ __device__ __forceinline__ void lds_diag_half(PP_LDS T* D, PP_LDS T* keep_rinv) 
         for (int c = 0; c < H; ++c) D[l * sp + C0 + c] = x[c];
       }
       lds_order();
-      if (l == 0) cnt[1 + HALF] = HALF == 0 && j == H - 1 ? NB : j + 1;
+      cnt[1 + HALF] = HALF == 0 && j == H - 1 ? NB : j + 1;
     }
   }
 }

@@ -10,9 +10,9 @@
 //   1. waves 0 and 1 factor the diagonal block from registers, each holding
 //      half of its columns (pivot rows by v_readlane, as panel_lu),
 //      publishing multiplier columns and groups of final rows with the pivot
-//      reciprocals; wave 0 then forms L11^{-1} from the multiplier columns;
-//      waves 2-3 eliminate the rows below against each published group (one
-//      row per thread);
+//      reciprocals, then each forms half of the columns of L11^{-1} from the
+//      multiplier columns; waves 2-3 eliminate the rows below against each
+//      published group (one row per thread);
 //   2. U12 = L11^{-1} A12 on MFMA, one 32-column tile per wave, in place;
 //   3. A22 -= L21 U12 on MFMA, 32 x 32 tiles dealt round-robin to the
 //      waves, both operands and C read from LDS.
@@ -75,7 +75,7 @@ __device__ __forceinline__ void lds_row_read(const PP_LDS T* p, T (&v)[NB]) {
 // half is final are written to D in groups of GRP (cnt[1 + HALF]); wave 0's
 // columns are final after pivot H - 1, when it writes all remaining rows.
 template <typename T, int MAXN, int HALF>
-__device__ __forceinline__ void lds_diag_half(PP_LDS T* D, PP_LDS T* keep_rinv) {
+__device__ __forceinline__ void lds_diag_half(PP_LDS T* D, PP_LDS T* keep_rinv, T (&x)[NB / 2]) {
   using L = LdsLu<T, MAXN>;
   constexpr int GRP = sizeof(T) == 4 ? 4 : 8;  // rows per publication
   constexpr int H = NB / 2, C0 = HALF * H;
@@ -85,7 +85,6 @@ __device__ __forceinline__ void lds_diag_half(PP_LDS T* D, PP_LDS T* keep_rinv) 
   PP_LDS T* const rinvs = L::rinv();
   volatile PP_LDS int* const cnt = L::cnt();
   const bool own = l < NB;
-  T x[H];
 #pragma unroll
   for (int c = 0; c < H; ++c) x[c] = own ? (T)D[l * sp + C0 + c] : T(0);
   // reciprocal of the next pivot, formed from the broadcast pivot so that
@@ -144,12 +143,56 @@ __device__ __forceinline__ void lds_diag_half(PP_LDS T* D, PP_LDS T* keep_rinv) 
   }
 }
 
+// Columns HALF NB/2 .. of L11^{-1} (row l in lane l) from the multiplier
+// columns, by the wave that factored that half of the diagonal block (x:
+// its half of the factored rows, the multipliers of its own pivots below
+// the diagonal).  Row j of the inverse is final after multiplier columns
+// < j and column j updates columns <= j, so wave 0 takes every multiplier
+// column (those of wave 1 from LDS once published) and wave 1 only its own
+// (the earlier ones leave its columns of the inverse at zero).  Each
+// element sees the updates of the one-wave formation in the same order.
+template <typename T, int MAXN, int HALF>
+__device__ __forceinline__ void lds_linv_half(PP_LDS T* keep_linv, const T (&x)[NB / 2]) {
+  using L = LdsLu<T, MAXN>;
+  constexpr int H = NB / 2, C0 = HALF * H;
+  const int l = lane_id();
+  PP_LDS T* const lc = L::lc();
+  PP_LDS T* const linv = L::linv();
+  volatile PP_LDS int* const cnt = L::cnt();
+  const bool own = l < NB;
+  T e[H];
+#pragma unroll
+  for (int c = 0; c < H; ++c) e[c] = (C0 + c == l) ? T(1) : T(0);
+#pragma unroll
+  for (int j = C0; j < NB; ++j) {
+    const bool act = own && l > j;
+    T lij;
+    if (j / H == HALF) {
+      lij = act ? x[j - C0] : T(0);
+    } else {
+      lds_wait_ge(&cnt[0], j + 1);  // multiplier column j is in LDS
+      lij = act ? (T)lc[j * NB + l] : T(0);
+    }
+#pragma unroll
+    for (int c = 0; c < H; ++c)
+      if (C0 + c <= j) e[c] -= lij * bcast_lane(e[c], j);
+  }
+  if (own) {
+#pragma unroll
+    for (int c = 0; c < H; ++c) linv[l * NB + C0 + c] = e[c];
+    if (keep_linv != nullptr) {
+#pragma unroll
+      for (int c = 0; c < H; ++c) keep_linv[l * NB + C0 + c] = e[c];
+    }
+  }
+}
+
 // Step 1 for the panel at k0 of the np x np matrix.
 // keep_linv / keep_rinv (optional): L11^{-1} of the diagonal block
 // (row-major, pitch NB) and its pivot reciprocals are also left there.
-// Waves 0 and 1 factor the diagonal block (lds_diag_half); wave 0 then forms
-// L11^{-1} from the multiplier columns; waves 2-3 eliminate the rows below
-// against each published group of pivot rows.
+// Waves 0 and 1 factor the diagonal block (lds_diag_half) and then form the
+// two column halves of L11^{-1} from the multiplier columns; waves 2-3
+// eliminate the rows below against each published group of pivot rows.
 template <typename T, int MAXN>
 __device__ __forceinline__ void lds_panel(int k0, int np, PP_LDS T* keep_linv = nullptr, PP_LDS T* keep_rinv = nullptr) {
   using L = LdsLu<T, MAXN>;
@@ -159,7 +202,6 @@ __device__ __forceinline__ void lds_panel(int k0, int np, PP_LDS T* keep_linv = 
   const int tid = threadIdx.x, l = lane_id(), w = wave_id();
   PP_LDS T* D = L::mat() + k0 * sp + k0;
   PP_LDS T* const lc = L::lc();
-  PP_LDS T* const linv = L::linv();
   PP_LDS T* const rinvs = L::rinv();
   volatile PP_LDS int* const cnt = L::cnt();
   if (tid == 0) {
@@ -168,31 +210,15 @@ __device__ __forceinline__ void lds_panel(int k0, int np, PP_LDS T* keep_linv = 
     cnt[2] = 0;
   }
   __syncthreads();
-  if (w == 0) {
-    lds_diag_half<T, MAXN, 0>(D, keep_rinv);
-    // L11^{-1}, column by column as the multiplier columns are published
-    const bool own = l < NB;
-    T e[NB];
-#pragma unroll
-    for (int c = 0; c < NB; ++c) e[c] = (c == l) ? T(1) : T(0);
-#pragma unroll
-    for (int j = 0; j < NB; ++j) {
-      if (j >= H) lds_wait_ge(&cnt[0], j + 1);  // multiplier column j is in LDS
-      const bool act = own && l > j;
-      const T lij = act ? (T)lc[j * NB + l] : T(0);
-#pragma unroll
-      for (int c = 0; c <= j; ++c) e[c] -= lij * bcast_lane(e[c], j);
+  if (w < 2) {
+    T x[H];
+    if (w == 0) {
+      lds_diag_half<T, MAXN, 0>(D, keep_rinv, x);
+      lds_linv_half<T, MAXN, 0>(keep_linv, x);
+    } else {
+      lds_diag_half<T, MAXN, 1>(D, keep_rinv, x);
+      lds_linv_half<T, MAXN, 1>(keep_linv, x);
     }
-    if (own) {
-#pragma unroll
-      for (int c = 0; c < NB; ++c) linv[l * NB + c] = e[c];
-      if (keep_linv != nullptr) {
-#pragma unroll
-        for (int c = 0; c < NB; ++c) keep_linv[l * NB + c] = e[c];
-      }
-    }
-  } else if (w == 1) {
-    lds_diag_half<T, MAXN, 1>(D, keep_rinv);
   } else {
     const int bt = tid - 128, below = np - k0 - NB, pw = np - k0 < NB ? np - k0 : NB;
     const bool on = bt < below;

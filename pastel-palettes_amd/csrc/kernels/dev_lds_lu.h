@@ -211,13 +211,16 @@ __device__ __forceinline__ void lds_panel(int k0, int np, PP_LDS T* keep_linv = 
   }
   __syncthreads();
   if (w < 2) {
+    // L11^{-1} serves the U12 product of this panel, so the last panel
+    // forms it only when the caller keeps it
+    const bool need_linv = k0 + NB < np || keep_linv != nullptr;
     T x[H];
     if (w == 0) {
       lds_diag_half<T, MAXN, 0>(D, keep_rinv, x);
-      lds_linv_half<T, MAXN, 0>(keep_linv, x);
+      if (need_linv) lds_linv_half<T, MAXN, 0>(keep_linv, x);
     } else {
       lds_diag_half<T, MAXN, 1>(D, keep_rinv, x);
-      lds_linv_half<T, MAXN, 1>(keep_linv, x);
+      if (need_linv) lds_linv_half<T, MAXN, 1>(keep_linv, x);
     }
   } else {
     const int bt = tid - 128, below = np - k0 - NB, pw = np - k0 < NB ? np - k0 : NB;

@@ -361,12 +361,17 @@ __device__ __forceinline__ T chol_rsqrt(T d) {
 // earlier ones to within rounding; its row of R and its reciprocal are set to
 // zero, so column j of R^{-1}, hence of Q, is zero (chol_upper's rule).
 // Returns false (RT untouched) on a non-finite pivot, or a non-positive one
-// when drop is 0.  Wave 0 only; no workgroup barrier.
+// when drop is 0.  Wave 0 only; no workgroup barrier.  Not inlined: with its
+// per-row branches inlined into accum_lds / accum_big their register demand
+// quadrupled those functions' scratch frames.
 template <typename T>
-__device__ __forceinline__ bool acc_chol_rinv(PP_LDS T* G, PP_LDS T* RT, T drop) {
+__device__ __attribute__((noinline)) bool acc_chol_rinv(PP_LDS T* G, PP_LDS T* RT, T drop) {
   constexpr int GP = NB + 1, P = acc_pitch<T>();
   const int l = lane_id(), c = l & 31, half = l >> 5;
   drop = bcast_lane<T>(drop, 0);  // the same on every lane
+  // the lane's rows a = half + 2 t with 2 t + 1 <= j are final and neither
+  // read nor written at step j (a branch of the whole wave per row pair; in
+  // the pair 2 t = j + 1 - half, row a = j is written back unchanged)
   for (int j = 0; j < NB; ++j) {
     T ga[NB / 2], gr[NB / 2];
     // the pivot is the same on every lane: taken as a scalar, so the tests
@@ -375,6 +380,7 @@ __device__ __forceinline__ bool acc_chol_rinv(PP_LDS T* G, PP_LDS T* RT, T drop)
     const T gjc = G[j * GP + c];
 #pragma unroll
     for (int t = 0; t < NB / 2; ++t) {
+      if (2 * t + 1 <= j) continue;
       const int a = half + 2 * t;
       ga[t] = G[a * GP + c];  // G[a][c]
       gr[t] = G[j * GP + a];  // G[j][a]
@@ -384,10 +390,11 @@ __device__ __forceinline__ bool acc_chol_rinv(PP_LDS T* G, PP_LDS T* RT, T drop)
     const T ri = dep ? T(0) : chol_rsqrt<T>(d);
     const T rj = (c < j || dep) ? T(0) : (c == j ? d * ri : gjc * ri);  // R[j][c]
     lds_order();
-    // every row of the lane is written back (rows a <= j unchanged), then
-    // row j of R over it: no branch per row
+    // the trailing rows of the lane (rows a <= j written back unchanged),
+    // then row j of R over it
 #pragma unroll
     for (int t = 0; t < NB / 2; ++t) {
+      if (2 * t + 1 <= j) continue;
       const int a = half + 2 * t;
       G[a * GP + c] = ga[t] - (a > j ? (gr[t] * ri) * rj : T(0));
     }
@@ -532,9 +539,19 @@ __device__ __attribute__((noinline)) void accum_lds(T* U1, int ld_u1, T* V1, int
   }
   // V1 <- V1 P1^T + V2 P2^T (in place), then U1 <- Q
   acc_rows_2<T, GM, false>(V1, ld_v1, k1, V2, ld_v2, k2, nx, S1, S2, nullptr, V1, ld_v1, k1);
-  for (int e = tid; e < ny * k1; e += nt) {
-    const int i = e / k1, j = e - i * k1;
-    gstore<T>(U1 + (int64_t)i * ld_u1 + j, Y[i * P + j]);
+  constexpr int V = 16 / (int)sizeof(T);
+  typedef T vec_t __attribute__((ext_vector_type(V)));
+  if ((k1 % V) == 0 && (ld_u1 % V) == 0 && (reinterpret_cast<uintptr_t>(U1) & 15) == 0 && (P % V) == 0) {
+    const int vpr = k1 / V;
+    for (int e = tid; e < ny * vpr; e += nt) {
+      const int i = e / vpr, j = (e - i * vpr) * V;
+      gstore<vec_t>(U1 + (int64_t)i * ld_u1 + j, *(const PP_LDS vec_t*)(Y + i * P + j));
+    }
+  } else {
+    for (int e = tid; e < ny * k1; e += nt) {
+      const int i = e / k1, j = e - i * k1;
+      gstore<T>(U1 + (int64_t)i * ld_u1 + j, Y[i * P + j]);
+    }
   }
   if (tid == 0) *chol_fail_flag() = 0;
   __syncthreads();

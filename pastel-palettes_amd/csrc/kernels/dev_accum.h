@@ -208,18 +208,34 @@ template <typename T, int MODE>
 __device__ __forceinline__ void acc_frag_rows_g(const T* X, int ld, int rows, int kv, int r0,
                                                 T (&fa)[NB / Mma<T, MODE>::KW][2][Mma<T, MODE>::EPL]) {
   using Pm = Mma<T, MODE>;
-  constexpr int KW = Pm::KW, EPL = Pm::EPL, CH = NB / KW;
+  constexpr int KW = Pm::KW, EPL = Pm::EPL, CH = NB / KW, V = 16 / (int)sizeof(T);
+  typedef T vec_t __attribute__((ext_vector_type(V)));
   const int l = lane_id(), g = l >> 4, li = l & 15;
+  // FP32, all NB columns valid with 16-byte aligned rows: whole-vector loads
+  // (in FP64 the vector form enlarges the accum functions' register frames
+  // more than it saves)
+  const bool vec = sizeof(T) == 4 && kv == NB && (ld % V) == 0 && (reinterpret_cast<uintptr_t>(X) & 15) == 0;
 #pragma unroll
   for (int i = 0; i < 2; ++i) {
     const int row = r0 + 16 * i + li, rs = row < rows ? row : rows - 1;
+    if (vec) {
 #pragma unroll
-    for (int c = 0; c < CH; ++c)
+      for (int c = 0; c < CH; ++c)
 #pragma unroll
-      for (int e = 0; e < EPL; ++e) {
-        const int k = c * KW + EPL * g + e;
-        fa[c][i][e] = gload<T>(X + (int64_t)rs * ld + (k < kv ? k : kv - 1));
-      }
+        for (int q = 0; q < EPL / V; ++q) {
+          const vec_t x = gload<vec_t>(X + (int64_t)rs * ld + c * KW + EPL * g + q * V);
+#pragma unroll
+          for (int e = 0; e < V; ++e) fa[c][i][q * V + e] = x[e];
+        }
+    } else {
+#pragma unroll
+      for (int c = 0; c < CH; ++c)
+#pragma unroll
+        for (int e = 0; e < EPL; ++e) {
+          const int k = c * KW + EPL * g + e;
+          fa[c][i][e] = gload<T>(X + (int64_t)rs * ld + (k < kv ? k : kv - 1));
+        }
+    }
   }
 #pragma unroll
   for (int c = 0; c < CH; ++c)
@@ -263,7 +279,8 @@ __device__ __forceinline__ void acc_tile_mma(const T (&fa)[NB / Mma<T, MODE>::KW
 // (OUT_LDS, rows past `rows` stored as zero up to the next multiple of 32)
 // or a global matrix (pitch ldo, only rows < rows and columns < kout
 // stored; it may be X1 itself: a tile's operands are loaded before it is
-// stored and tiles belong to one wave each).
+// stored, tiles belong to one wave each, and the tile a wave prefetches is
+// its own next one).
 template <typename T, int MODE, bool OUT_LDS>
 __device__ __forceinline__ void acc_rows_2(const T* X1, int ld1, int kv1, const T* X2, int ld2, int kv2, int rows,
                                            const PP_LDS T* S1, const PP_LDS T* S2, PP_LDS T* OutL, T* OutG, int ldo,
@@ -276,16 +293,27 @@ __device__ __forceinline__ void acc_rows_2(const T* X1, int ld1, int kv1, const 
   acc_frag_st<T, MODE>(S1, fb1);
   acc_frag_st<T, MODE>(S2, fb2);
   const int ntile = (rows + 31) >> 5;
-  for (int t = w; t < ntile; t += nw) {
+  // the fragments of the wave's next tile are requested before the current
+  // tile's products (past the last tile, the last tile again: rows clamped)
+  T fa1[2][CH][2][EPL], fa2[2][CH][2][EPL];
+  auto load = [&](auto b_c, int t) {
+    constexpr int bb = decltype(b_c)::value;
+    const int r0 = (t < ntile ? t : ntile - 1) * 32;
+    acc_frag_rows_g<T, MODE>(X1, ld1, rows, kv1, r0, fa1[bb]);
+    acc_frag_rows_g<T, MODE>(X2, ld2, rows, kv2, r0, fa2[bb]);
+  };
+  auto step = [&](auto b_c, int t, auto pre_c) {
+    constexpr int bb = decltype(b_c)::value;
     const int r0 = t * 32;
-    T fa1[CH][2][EPL], fa2[CH][2][EPL];
-    acc_frag_rows_g<T, MODE>(X1, ld1, rows, kv1, r0, fa1);
-    acc_frag_rows_g<T, MODE>(X2, ld2, rows, kv2, r0, fa2);
+    if constexpr (decltype(pre_c)::value) {
+      load(IC<bb ^ 1>{}, t + nw);
+      __builtin_amdgcn_sched_barrier(0);
+    }
     acc_t acc[2][2];
 #pragma unroll
     for (int i = 0; i < 2; ++i) acc[i][0] = acc[i][1] = Pm::zero();
-    acc_tile_mma<T, MODE>(fa1, fb1, acc);
-    acc_tile_mma<T, MODE>(fa2, fb2, acc);
+    acc_tile_mma<T, MODE>(fa1[bb], fb1, acc);
+    acc_tile_mma<T, MODE>(fa2[bb], fb2, acc);
 #pragma unroll
     for (int i = 0; i < 2; ++i)
 #pragma unroll
@@ -299,6 +327,20 @@ __device__ __forceinline__ void acc_rows_2(const T* X1, int ld1, int kv1, const 
             if (row < rows && col < kout) gstore<T>(OutG + (int64_t)row * ldo + col, acc[i][j][r]);
           }
         }
+  };
+  if constexpr (sizeof(T) == 4) {
+    if (w < ntile) load(IC<0>{}, w);
+    for (int t = w; t < ntile; t += 2 * nw) {
+      step(IC<0>{}, t, IC<1>{});
+      if (t + nw < ntile) step(IC<1>{}, t + nw, IC<1>{});
+    }
+  } else {
+    // FP64: no prefetch (the second buffer enlarges the register frames of
+    // the accum functions more than the overlap saves)
+    for (int t = w; t < ntile; t += nw) {
+      load(IC<0>{}, t);
+      step(IC<0>{}, t, IC<0>{});
+    }
   }
 }
 
@@ -561,7 +603,16 @@ __device__ __attribute__((noinline)) void accum_lds(T* U1, int ld_u1, T* V1, int
 // worker scratch (pitch k1: columns < k1 only).
 template <typename T>
 __device__ __forceinline__ void acc_chunk_out(T* Yg, int k1, int r0, int n, const PP_LDS T* Yc) {
-  constexpr int P = acc_pitch<T>();
+  constexpr int P = acc_pitch<T>(), V = 16 / (int)sizeof(T);
+  typedef T vec_t __attribute__((ext_vector_type(V)));
+  if ((k1 % V) == 0 && (reinterpret_cast<uintptr_t>(Yg) & 15) == 0) {
+    const int vpr = k1 / V;
+    for (int e = threadIdx.x; e < n * vpr; e += blockDim.x) {
+      const int i = e / vpr, j = (e - i * vpr) * V;
+      gstore<vec_t>(Yg + (int64_t)(r0 + i) * k1 + j, *(const PP_LDS vec_t*)(Yc + i * P + j));
+    }
+    return;
+  }
   for (int e = threadIdx.x; e < n * k1; e += blockDim.x) {
     const int i = e / k1, j = e - i * k1;
     gstore<T>(Yg + (int64_t)(r0 + i) * k1 + j, Yc[i * P + j]);

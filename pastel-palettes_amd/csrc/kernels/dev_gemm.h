@@ -320,7 +320,7 @@ __device__ __forceinline__ void gemm_tiles(int m, int n, int k, T alpha, T beta,
 // lanes cover a 128-byte row segment).  Every element sees the same MFMA
 // sequence as in gemm_tiles (C scaled first, K chunks in order), so results
 // are identical.
-constexpr int kStagedBytes = 128 * 1024;
+constexpr int kStagedBytes = 144 * 1024;  // the failure flag of accum keeps the last 16 bytes
 
 template <typename T, int MODE>
 __device__ __forceinline__ bool gemm_staged_ok(int m, int n, int k, const T* A, int lda, bool a_along, const T* B,

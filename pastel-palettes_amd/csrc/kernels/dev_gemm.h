@@ -335,9 +335,13 @@ __device__ __forceinline__ bool gemm_staged_ok(int m, int n, int k, const T* A, 
          (ES == 8 || ((reinterpret_cast<uintptr_t>(C) & 15) == 0 && ldc % 4 == 0));
 }
 
-template <typename T, int MODE, bool BETA>
-__device__ __forceinline__ void gemm_staged(int m, int n, int k, T alpha, T beta, T* C, int ldc, const T* A, int lda,
-                                            const T* B, int ldb) {
+// A_DMA false: rows 0 .. m-1 of the staged area (op(A)) are written by
+// fill_a(sa) (in the swizzled layout, after the copies of op(B)^T are
+// issued, any barriers inside it reached by every thread) instead of copied
+// from A.
+template <typename T, int MODE, bool BETA, bool A_DMA, typename FillA>
+__device__ __forceinline__ void gemm_staged_run(int m, int n, int k, T alpha, T beta, T* C, int ldc, const T* A,
+                                                int lda, const T* B, int ldb, FillA&& fill_a) {
   using P = Mma<T, MODE>;
   using acc_t = typename P::acc_t;
   constexpr bool TR = sizeof(T) == 4;
@@ -384,14 +388,15 @@ __device__ __forceinline__ void gemm_staged(int m, int n, int k, T alpha, T beta
     load_c(IC<1>{}, 1);
   }
   {
-    const int ninst = (m + n) * cpr / 64;
-    for (int i = w; i < ninst; i += nw) {
+    const int i0 = A_DMA ? 0 : m * cpr / 64, ninst = (m + n) * cpr / 64;
+    for (int i = i0 + w; i < ninst; i += nw) {
       const int q = 64 * i + l, row = q >> lg, pc = q & (cpr - 1), lc = pc ^ (row & (cpr - 1));
       const T* src = row < m ? A + (int64_t)row * lda + V * lc : B + (int64_t)(row - m) * ldb + V * lc;
       __builtin_amdgcn_global_load_lds((const void*)src, (PP_LDS void*)(sa + 64 * V * i), 16, 0, 0);
     }
-    __builtin_amdgcn_s_waitcnt(0);
   }
+  if constexpr (!A_DMA) fill_a(sa);
+  __builtin_amdgcn_s_waitcnt(0);
   __syncthreads();
   auto body = [&](auto s_c, int q) {
     constexpr int s = decltype(s_c)::value;
@@ -447,6 +452,12 @@ __device__ __forceinline__ void gemm_staged(int m, int n, int k, T alpha, T beta
     body(IC<1>{}, q0 + 1);
   }
   __syncthreads();
+}
+
+template <typename T, int MODE, bool BETA>
+__device__ __forceinline__ void gemm_staged(int m, int n, int k, T alpha, T beta, T* C, int ldc, const T* A, int lda,
+                                            const T* B, int ldb) {
+  gemm_staged_run<T, MODE, BETA, true>(m, n, k, alpha, beta, C, ldc, A, lda, B, ldb, [](PP_LDS T*) {});
 }
 
 // C = beta * C + alpha * op(A) op(B); C m x n, op(A) m x k, op(B) k x n;

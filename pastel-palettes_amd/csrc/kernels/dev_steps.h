@@ -553,6 +553,104 @@ __device__ __forceinline__ void exec_step(const Step<T>& st) {
 
 // Executes one execute entry; returns its length in words or -1.
 // fallbacks (optional): counts the Householder fallbacks of accum.
+// ---------------------------------------------------------------- fused gemm_4x
+// M -= (A (op(B) op(C))) op(D), the (A (B C)) D association of gemm_4x, with
+// inner dimensions k = l = o = NB, A row-major and op(D) = D^T (D n x o
+// row-major): t1 = op(B) op(C) (NB x NB) is formed by wave 0 into LDS, then
+// t2 = A t1 (m x NB) tile by tile straight into the staged operand rows of
+// gemm_staged_run while the copies of D are in flight, and M -= t2 op(D)
+// runs as the staged update.  Each product sees the MFMA sequence of the
+// step form (the products of t1 in one wave, its chunks in order).
+template <typename T, int MODE>
+__device__ __forceinline__ bool gemm4x_fused_ok(int m, int n, int k, int l, int o, bool aT, const T* D, int ldd, bool dT,
+                                                const T* M, int ldm) {
+  const int nw = blockDim.x >> 6;
+  return k == NB && l == NB && o == NB && !aT && dT &&
+         gemm_staged_ok<T, MODE>(m, n, o, D, o, true, D, ldd, true, M, ldm, nw) &&
+         ((int64_t)(m + n) * o + NB * (NB + 1)) * (int64_t)sizeof(T) <= kStagedBytes;
+}
+
+// Not inlined: inlined, its register demand made the VM kernels spill more
+// on every path.
+template <typename T, int MODE>
+__device__ __attribute__((noinline)) void gemm4x_fused(int m, int n, T* M, int ldm, const T* A, int lda, const T* B, int ldb,
+                                             bool bT, const T* Cm, int ldc, bool cT, const T* D, int ldd) {
+  using P = Mma<T, MODE>;
+  using acc_t = typename P::acc_t;
+  constexpr int KW = P::KW, EPL = P::EPL, CH = NB / KW, V = 16 / (int)sizeof(T), TP = NB + 1, CPR = NB / V;
+  const int nw = blockDim.x >> 6, w = wave_id(), l = lane_id(), g = l >> 4, li = l & 15;
+  auto fill = [&](PP_LDS T* sa) {
+    PP_LDS T* t1 = sa + (m + n) * NB;  // after the staged rows, pitch TP
+    if (w == 0) {
+      OperandLane<T, EPL> la[2], lb[2];
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        la[h].init(B, ldb, !bT, 16 * h + li, NB);
+        lb[h].init(Cm, ldc, cT, 16 * h + li, NB);
+      }
+      acc_t acc[2][2];
+#pragma unroll
+      for (int i = 0; i < 2; ++i) acc[i][0] = acc[i][1] = P::zero();
+#pragma unroll
+      for (int c = 0; c < CH; ++c) {
+        T fa[2][EPL], fb[2][EPL];
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+          la[h].template load<false>(fa[h], c * KW + EPL * g, NB);
+          lb[h].template load<false>(fb[h], c * KW + EPL * g, NB);
+        }
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+          for (int j = 0; j < 2; ++j) acc[i][j] = P::mma_chunk(fa[i], fb[j], acc[i][j]);
+      }
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+#pragma unroll
+          for (int r = 0; r < 4; ++r) t1[(16 * i + P::orow(l, r)) * TP + 16 * j + li] = acc[i][j][r];
+    }
+    __syncthreads();
+    for (int t = w; t < (m >> 5); t += nw) {
+      const int r0 = 32 * t;
+      OperandLane<T, EPL> la[2];
+#pragma unroll
+      for (int h = 0; h < 2; ++h) la[h].init(A, lda, true, r0 + 16 * h + li, m);
+      acc_t acc[2][2];
+#pragma unroll
+      for (int i = 0; i < 2; ++i) acc[i][0] = acc[i][1] = P::zero();
+#pragma unroll
+      for (int c = 0; c < CH; ++c) {
+        T fa[2][EPL], fb[2][EPL];
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+          la[h].template load<false>(fa[h], c * KW + EPL * g, NB);
+#pragma unroll
+          for (int e = 0; e < EPL; ++e) fb[h][e] = t1[(c * KW + EPL * g + e) * TP + 16 * h + li];
+        }
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+          for (int j = 0; j < 2; ++j) acc[i][j] = P::mma_chunk(fa[i], fb[j], acc[i][j]);
+      }
+      // t2's row r, column c into the staged layout (pitch NB, chunk c / V
+      // at c / V ^ (r mod CPR))
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            const int row = r0 + 16 * i + P::orow(l, r), col = 16 * j + li;
+            sa[row * NB + V * ((col / V) ^ (row & (CPR - 1))) + col % V] = acc[i][j][r];
+          }
+    }
+  };
+  gemm_staged_run<T, MODE, true, false>(m, n, NB, T(-1), T(1), M, ldm, static_cast<const T*>(nullptr), NB, D, ldd,
+                                        fill);
+}
+
 template <typename T, int FMODE, int GMODE>
 __device__ __forceinline__ int exec_entry(const int32_t* pc, void* const* ptrs, const void* rnd, T* tmp,
                                           int32_t* fallbacks = nullptr) {
@@ -576,6 +674,22 @@ __device__ __forceinline__ int exec_entry(const int32_t* pc, void* const* ptrs, 
     } else if (accum_big_ok<T>(nx, ny, k1, k2)) {
       accum_big<T, FMODE, GMODE>(U1, l1, V1, l2, U2, l3, V2, l4, nx, ny, k1, k2, om, tmp, GMODE == MODE_BF16);
       if (*chol_fail_flag() == 0) return len;
+    }
+  }
+  if (word(pc, 2) == gemm_4x && word(pc, 27) == 1) {
+    // the (A (B C)) D association with rank-32 inner dimensions: the two
+    // small products in LDS, feeding the staged update
+    const int m = word(pc, 13), n = word(pc, 14), k = word(pc, 15), l = word(pc, 16), o = word(pc, 17);
+    T* M = slot<T>(ptrs, pc, 3) + word(pc, 8);
+    const T* A = slot<T>(ptrs, pc, 4) + word(pc, 9);
+    const T* B = slot<T>(ptrs, pc, 5) + word(pc, 10);
+    const T* C = slot<T>(ptrs, pc, 6) + word(pc, 11);
+    const T* D = slot<T>(ptrs, pc, 7) + word(pc, 12);
+    const int ldm = word(pc, 18), lda = word(pc, 19), ldb = word(pc, 20), ldc = word(pc, 21), ldd = word(pc, 22);
+    const bool aT = word(pc, 23) != 0, bT = word(pc, 24) != 0, cT = word(pc, 25) != 0, dT = word(pc, 26) != 0;
+    if (gemm4x_fused_ok<T, GMODE>(m, n, k, l, o, aT, D, ldd, dT, M, ldm)) {
+      gemm4x_fused<T, GMODE>(m, n, M, ldm, A, lda, B, ldb, bT, C, ldc, cT, D, ldd);
+      return len;
     }
   }
   for (int s = 0;; ++s) {

@@ -371,7 +371,8 @@ def run_ops(pp, torch, ctx, reps=50, log=print):
     workgroup each, executed `reps` times in place (pp_exec_single_timed):
     256 x 256 (and 128, 160) diagonal-leaf getrfs, the 256 x 32 trsm of a low-rank
     factor (left, and right with transposed storage), and the update of a
-    leaf by a rank-32 product (gemm_4x, and the 256 x 256 x 32 gemm_plus)."""
+    leaf by a rank-32 product (gemm_4x in the ((A B) C) D and A ((B^T C) D^T)
+    associations, and the 256 x 256 x 32 gemm_plus)."""
     rng = np.random.default_rng(1)
     res = {}
     for name, prec in (("fp64", pp.FP64), ("fp32", pp.FP32), ("mixed", pp.MIXED)):
@@ -405,6 +406,10 @@ def run_ops(pp, torch, ctx, reps=50, log=print):
         row["gemm_4x_256x256_r32"] = 1e3 * ctx.time_single(
             prec, [1, 0, 7, 0, 1, 2, 3, 4, 0, 0, 0, 0, 0, n, n, r, r, r, n, r, r, r, r, 0, 0, 0, 1, 0,
                    (t1 + 31) // 32 * 32],
+            [M.data_ptr(), Ua.data_ptr(), C1.data_ptr(), C1.data_ptr(), C2.data_ptr()], reps=reps)
+        # the association the factorization's critical path uses: A ((B^T C) D^T)
+        row["gemm_4x_256x256_r32_abc"] = 1e3 * ctx.time_single(
+            prec, [1, 0, 7, 0, 1, 2, 3, 4, 0, 0, 0, 0, 0, n, n, r, r, r, n, r, r, r, r, 0, 1, 0, 1, 1, r * r],
             [M.data_ptr(), Ua.data_ptr(), C1.data_ptr(), C1.data_ptr(), C2.data_ptr()], reps=reps)
         res[name] = row
         log(f"[ops] {name}: " + ", ".join(f"{k} {v:.1f} us" for k, v in row.items()))

@@ -215,9 +215,13 @@ def test_gemm3(ctx, orc, prec, control, trans):
 
 @pytest.mark.parametrize("prec", [FP64, FP32, MIXED])
 @pytest.mark.parametrize("control", [0, 1, 2, 3, 4])
-def test_gemm4(ctx, orc, prec, control):
-    m, n, k, l, o = 64, 72, 24, 20, 28
-    rng = np.random.default_rng(control)
+@pytest.mark.parametrize("shape", [(64, 72, 24, 20, 28), (256, 256, 32, 32, 32), (224, 96, 32, 32, 32)])
+def test_gemm4(ctx, orc, prec, control, shape):
+    """Every association; the rank-32 shapes of the (A (B C)) D association
+    (control 1) take the fused form (two small products in LDS feeding the
+    staged update)."""
+    m, n, k, l, o = shape
+    rng = np.random.default_rng(control + m)
     mm = rng.standard_normal((m, n))
     a, b = _bf16_exact(rng, (m, k)), _bf16_exact(rng, (l, k))   # B transposed
     c, d = _bf16_exact(rng, (l, o)), _bf16_exact(rng, (n, o))   # D transposed

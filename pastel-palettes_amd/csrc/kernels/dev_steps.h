@@ -555,71 +555,98 @@ __device__ __forceinline__ void exec_step(const Step<T>& st) {
 // fallbacks (optional): counts the Householder fallbacks of accum.
 // ---------------------------------------------------------------- fused gemm_4x
 // M -= (A (op(B) op(C))) op(D), the (A (B C)) D association of gemm_4x, with
-// inner dimensions k = l = o = NB, A row-major and op(D) = D^T (D n x o
-// row-major): t1 = op(B) op(C) (NB x NB) is formed by wave 0 into LDS, then
-// t2 = A t1 (m x NB) tile by tile straight into the staged operand rows of
+// k = o = NB (l a multiple of the MFMA chunk), A row-major and op(D) = D^T
+// (D n x o row-major): t1 = op(B) op(C) (NB x NB, inner dimension l) is
+// formed in LDS with its chunks split over the waves and the partial sums
+// added in wave order (as the step form's split-K GEMM does), then t2 = A t1
+// (m x NB) tile by tile straight into the staged operand rows of
 // gemm_staged_run while the copies of D are in flight, and M -= t2 op(D)
-// runs as the staged update.  Each product sees the MFMA sequence of the
-// step form (the products of t1 in one wave, its chunks in order).
+// runs as the staged update.
 template <typename T, int MODE>
 __device__ __forceinline__ bool gemm4x_fused_ok(int m, int n, int k, int l, int o, bool aT, const T* D, int ldd, bool dT,
                                                 const T* M, int ldm) {
   const int nw = blockDim.x >> 6;
-  return k == NB && l == NB && o == NB && !aT && dT &&
+  return k == NB && o == NB && l >= Mma<T, MODE>::KW && l % Mma<T, MODE>::KW == 0 && !aT && dT &&
          gemm_staged_ok<T, MODE>(m, n, o, D, o, true, D, ldd, true, M, ldm, nw) &&
-         ((int64_t)(m + n) * o + NB * (NB + 1)) * (int64_t)sizeof(T) <= kStagedBytes;
+         ((int64_t)(m + n) * o + NB * (NB + 1)) * (int64_t)sizeof(T) <= kStagedBytes &&
+         m * NB >= 4 * 64 * 16;  // t1's partial sums fit in the t2 rows
 }
 
 // Not inlined: inlined, its register demand made the VM kernels spill more
 // on every path.
 template <typename T, int MODE>
-__device__ __attribute__((noinline)) void gemm4x_fused(int m, int n, T* M, int ldm, const T* A, int lda, const T* B, int ldb,
-                                             bool bT, const T* Cm, int ldc, bool cT, const T* D, int ldd) {
+__device__ __attribute__((noinline)) void gemm4x_fused(int m, int n, int l, T* M, int ldm, const T* A, int lda,
+                                                       const T* B, int ldb, bool bT, const T* Cm, int ldc, bool cT,
+                                                       const T* D, int ldd) {
   using P = Mma<T, MODE>;
   using acc_t = typename P::acc_t;
   constexpr int KW = P::KW, EPL = P::EPL, CH = NB / KW, V = 16 / (int)sizeof(T), TP = NB + 1, CPR = NB / V;
-  const int nw = blockDim.x >> 6, w = wave_id(), l = lane_id(), g = l >> 4, li = l & 15;
+  const int nw = blockDim.x >> 6, w = wave_id(), l0 = lane_id(), g = l0 >> 4, li = l0 & 15;
   auto fill = [&](PP_LDS T* sa) {
     PP_LDS T* t1 = sa + (m + n) * NB;  // after the staged rows, pitch TP
-    if (w == 0) {
+    PP_LDS T* red = sa;                // partial sums, in the t2 rows before t2 is formed
+    // t1: chunks w, w + gk, ... of the inner dimension on wave w < gk
+    const int nch = l / KW;
+    int gk = 1;
+    while (2 * gk <= nw && 2 * gk <= nch) gk *= 2;
+    acc_t acc[2][2];
+#pragma unroll
+    for (int i = 0; i < 2; ++i) acc[i][0] = acc[i][1] = P::zero();
+    if (w < gk) {
       OperandLane<T, EPL> la[2], lb[2];
 #pragma unroll
       for (int h = 0; h < 2; ++h) {
         la[h].init(B, ldb, !bT, 16 * h + li, NB);
         lb[h].init(Cm, ldc, cT, 16 * h + li, NB);
       }
-      acc_t acc[2][2];
-#pragma unroll
-      for (int i = 0; i < 2; ++i) acc[i][0] = acc[i][1] = P::zero();
-#pragma unroll
-      for (int c = 0; c < CH; ++c) {
+      for (int c = w; c < nch; c += gk) {
         T fa[2][EPL], fb[2][EPL];
 #pragma unroll
         for (int h = 0; h < 2; ++h) {
-          la[h].template load<false>(fa[h], c * KW + EPL * g, NB);
-          lb[h].template load<false>(fb[h], c * KW + EPL * g, NB);
+          la[h].template load<false>(fa[h], c * KW + EPL * g, l);
+          lb[h].template load<false>(fb[h], c * KW + EPL * g, l);
         }
 #pragma unroll
         for (int i = 0; i < 2; ++i)
 #pragma unroll
           for (int j = 0; j < 2; ++j) acc[i][j] = P::mma_chunk(fa[i], fb[j], acc[i][j]);
       }
+      if (w > 0) {
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+          for (int j = 0; j < 2; ++j)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) red[(w * 64 + l0) * 16 + (i * 2 + j) * 4 + r] = acc[i][j][r];
+      }
+    }
+    __syncthreads();
+    if (w == 0) {
+      for (int q = 1; q < gk; ++q) {
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+          for (int j = 0; j < 2; ++j)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) acc[i][j][r] += red[(q * 64 + l0) * 16 + (i * 2 + j) * 4 + r];
+      }
 #pragma unroll
       for (int i = 0; i < 2; ++i)
 #pragma unroll
         for (int j = 0; j < 2; ++j)
 #pragma unroll
-          for (int r = 0; r < 4; ++r) t1[(16 * i + P::orow(l, r)) * TP + 16 * j + li] = acc[i][j][r];
+          for (int r = 0; r < 4; ++r) t1[(16 * i + P::orow(l0, r)) * TP + 16 * j + li] = acc[i][j][r];
     }
     __syncthreads();
+    // t2 = A t1, tiles of 32 rows dealt to the waves, into the staged rows
     for (int t = w; t < (m >> 5); t += nw) {
       const int r0 = 32 * t;
       OperandLane<T, EPL> la[2];
 #pragma unroll
       for (int h = 0; h < 2; ++h) la[h].init(A, lda, true, r0 + 16 * h + li, m);
-      acc_t acc[2][2];
+      acc_t a2[2][2];
 #pragma unroll
-      for (int i = 0; i < 2; ++i) acc[i][0] = acc[i][1] = P::zero();
+      for (int i = 0; i < 2; ++i) a2[i][0] = a2[i][1] = P::zero();
 #pragma unroll
       for (int c = 0; c < CH; ++c) {
         T fa[2][EPL], fb[2][EPL];
@@ -632,7 +659,7 @@ __device__ __attribute__((noinline)) void gemm4x_fused(int m, int n, T* M, int l
 #pragma unroll
         for (int i = 0; i < 2; ++i)
 #pragma unroll
-          for (int j = 0; j < 2; ++j) acc[i][j] = P::mma_chunk(fa[i], fb[j], acc[i][j]);
+          for (int j = 0; j < 2; ++j) a2[i][j] = P::mma_chunk(fa[i], fb[j], a2[i][j]);
       }
       // t2's row r, column c into the staged layout (pitch NB, chunk c / V
       // at c / V ^ (r mod CPR))
@@ -642,8 +669,8 @@ __device__ __attribute__((noinline)) void gemm4x_fused(int m, int n, T* M, int l
         for (int j = 0; j < 2; ++j)
 #pragma unroll
           for (int r = 0; r < 4; ++r) {
-            const int row = r0 + 16 * i + P::orow(l, r), col = 16 * j + li;
-            sa[row * NB + V * ((col / V) ^ (row & (CPR - 1))) + col % V] = acc[i][j][r];
+            const int row = r0 + 16 * i + P::orow(l0, r), col = 16 * j + li;
+            sa[row * NB + V * ((col / V) ^ (row & (CPR - 1))) + col % V] = a2[i][j][r];
           }
     }
   };
@@ -677,8 +704,8 @@ __device__ __forceinline__ int exec_entry(const int32_t* pc, void* const* ptrs, 
     }
   }
   if (word(pc, 2) == gemm_4x && word(pc, 27) == 1) {
-    // the (A (B C)) D association with rank-32 inner dimensions: the two
-    // small products in LDS, feeding the staged update
+    // the (A (B C)) D association with rank-32 outer factors: the two small
+    // products in LDS, feeding the staged update
     const int m = word(pc, 13), n = word(pc, 14), k = word(pc, 15), l = word(pc, 16), o = word(pc, 17);
     T* M = slot<T>(ptrs, pc, 3) + word(pc, 8);
     const T* A = slot<T>(ptrs, pc, 4) + word(pc, 9);
@@ -688,7 +715,7 @@ __device__ __forceinline__ int exec_entry(const int32_t* pc, void* const* ptrs, 
     const int ldm = word(pc, 18), lda = word(pc, 19), ldb = word(pc, 20), ldc = word(pc, 21), ldd = word(pc, 22);
     const bool aT = word(pc, 23) != 0, bT = word(pc, 24) != 0, cT = word(pc, 25) != 0, dT = word(pc, 26) != 0;
     if (gemm4x_fused_ok<T, GMODE>(m, n, k, l, o, aT, D, ldd, dT, M, ldm)) {
-      gemm4x_fused<T, GMODE>(m, n, M, ldm, A, lda, B, ldb, bT, C, ldc, cT, D, ldd);
+      gemm4x_fused<T, GMODE>(m, n, l, M, ldm, A, lda, B, ldb, bT, C, ldc, cT, D, ldd);
       return len;
     }
   }

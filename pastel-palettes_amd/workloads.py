@@ -407,10 +407,11 @@ def run_ops(pp, torch, ctx, reps=50, log=print):
             prec, [1, 0, 7, 0, 1, 2, 3, 4, 0, 0, 0, 0, 0, n, n, r, r, r, n, r, r, r, r, 0, 0, 0, 1, 0,
                    (t1 + 31) // 32 * 32],
             [M.data_ptr(), Ua.data_ptr(), C1.data_ptr(), C1.data_ptr(), C2.data_ptr()], reps=reps)
-        # the association the factorization's critical path uses: A ((B^T C) D^T)
+        # the form on the factorization's critical path: M -= A ((B^T C) D^T),
+        # B and C 256 x 32 (the inner product over the leaf's 256 rows)
         row["gemm_4x_256x256_r32_abc"] = 1e3 * ctx.time_single(
-            prec, [1, 0, 7, 0, 1, 2, 3, 4, 0, 0, 0, 0, 0, n, n, r, r, r, n, r, r, r, r, 0, 1, 0, 1, 1, r * r],
-            [M.data_ptr(), Ua.data_ptr(), C1.data_ptr(), C1.data_ptr(), C2.data_ptr()], reps=reps)
+            prec, [1, 0, 7, 0, 1, 2, 3, 4, 0, 0, 0, 0, 0, n, n, r, n, r, n, r, r, r, r, 0, 1, 0, 1, 1, r * r],
+            [M.data_ptr(), Ua.data_ptr(), Vb.data_ptr(), C2.data_ptr(), C2.data_ptr()], reps=reps)
         res[name] = row
         log(f"[ops] {name}: " + ", ".join(f"{k} {v:.1f} us" for k, v in row.items()))
     return {"workload": "latency of one critical-path operation on one workgroup (microseconds, mean of "

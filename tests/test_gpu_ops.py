@@ -215,11 +215,12 @@ def test_gemm3(ctx, orc, prec, control, trans):
 
 @pytest.mark.parametrize("prec", [FP64, FP32, MIXED])
 @pytest.mark.parametrize("control", [0, 1, 2, 3, 4])
-@pytest.mark.parametrize("shape", [(64, 72, 24, 20, 28), (256, 256, 32, 32, 32), (224, 96, 32, 32, 32)])
+@pytest.mark.parametrize("shape", [(64, 72, 24, 20, 28), (256, 256, 32, 32, 32), (224, 96, 32, 32, 32),
+                                   (256, 256, 32, 256, 32), (256, 128, 32, 96, 32)])
 def test_gemm4(ctx, orc, prec, control, shape):
-    """Every association; the rank-32 shapes of the (A (B C)) D association
-    (control 1) take the fused form (two small products in LDS feeding the
-    staged update)."""
+    """Every association; the shapes with k = o = 32 of the (A (B C)) D
+    association (control 1) take the fused form (two small products in LDS
+    feeding the staged update), l = 256 as in the HODLR leaf updates."""
     m, n, k, l, o = shape
     rng = np.random.default_rng(control + m)
     mm = rng.standard_normal((m, n))

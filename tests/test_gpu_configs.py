@@ -1,28 +1,28 @@
 """Every benchmark configuration at its full size on the GPU, checked against
 the CPU oracle.
 
-  cfg1  FP64 H-LU, N = 16384 (Morton points, 1/(r + 1e-3) + N/4 on the
-        diagonal), HODLR 6 levels, leaf 256, rank 32, 16 right-hand sides,
-        256 workers: the factor arena equals the oracle's execution of the
-        same instructions (leaf products within 1e-9 relative, the accum
-        projections' conditioning); the refined solution's residual,
-        recomputed on the host with the oracle H-matvec, is <= 1e-10; it
-        agrees with the oracle's own FP64 solution.
-  cfg2  the same matrix in mixed precision (FP32 leaf routines, BF16
-        products), and N = 32768 with 7 levels and rank 48 (FP64 and mixed),
-        refined to 1e-12: host residuals <= 1e-12, and each solution's error
-        against the oracle's FP64 solution (CPU executor) is printed and
-        bounded.  The mixed factor arena is compared with the FP64 oracle at
-        the BF16 tolerance stated below.
-  cfg3  the 512-matrix batch (N log-uniform in [512, 8192], multiples of 256,
-        rank 16/24/32 drawn as the batch benchmark draws them, leaf 256, 4
-        right-hand sides, FP32 factorization replayed from a hipGraph, FP64
-        refinement to 1e-10): every member's residual recomputed on the host,
-        and the factors of the smallest, a middle and the largest member
-        against the oracle.
-  cfg4  a serving burst over a pool with N up to 32768, 1-64 right-hand
-        sides, 30 % reuse, mixed precision refined to 1e-10: every result's
-        residual on the host, and every result the same alone and in the mix.
+  * FP64 H-LU, N = 16384 (Morton points, 1/(r + 1e-3) + N/4 on the
+    diagonal), HODLR 6 levels, leaf 256, rank 32, 16 right-hand sides, 256
+    workers: the factor arena equals the oracle's execution of the same
+    instructions (leaf products within 1e-9 relative, the accum projections'
+    conditioning); the refined solution's residual, recomputed on the host
+    with the oracle H-matvec, is <= 1e-10; it agrees with the oracle's own
+    FP64 solution.
+  * The same matrix in mixed precision (FP32 leaf routines, BF16 products),
+    and N = 32768 with 7 levels and rank 48 (FP64 and mixed), refined to
+    1e-12: host residuals <= 1e-12, and each solution's error against the
+    oracle's FP64 solution (CPU executor) is printed and bounded.  The mixed
+    factor arena is compared with the FP64 oracle at the BF16 tolerance
+    stated below.
+  * The 512-matrix batch (N log-uniform in [512, 8192], multiples of 256,
+    rank 16/24/32 drawn as the batch benchmark draws them, leaf 256, 4
+    right-hand sides, FP32 factorization replayed from a hipGraph, FP64
+    refinement to 1e-10): every member's residual recomputed on the host,
+    and the factors of the smallest, a middle and the largest member against
+    the oracle.
+  * A serving burst over a pool with N up to 32768, 1-64 right-hand sides,
+    30 % reuse, mixed precision refined to 1e-10: every result's residual on
+    the host, and every result the same alone and in the mix.
 """
 import math
 
@@ -67,83 +67,83 @@ def gpu_solve(pp, ctx, hm, arena, b, precision, tol, workers=256):
 
 
 @pytest.fixture(scope="module")
-def cfg1(pp, orc):
+def hodlr16k(pp, orc):
     n = 16384
-    print(f"[case] cfg1 build n={n} levels=6 rank=32", flush=True)
+    print(f"[case] hodlr16k build n={n} levels=6 rank=32", flush=True)
     hm, kern, arena = pp.build_config(1, n, levels=6, rank=32)
     b = pp.rhs(1, n, 16)
     fac = H.oracle_factor(pp, orc, hm, arena)
     x, res, it = H.oracle_solution(pp, orc, hm, arena, b, 5e-13, fac=fac)
-    print(f"[cfg1] oracle solution: residual {res:.2e} after {it} sweeps", flush=True)
+    print(f"[hodlr16k] oracle solution: residual {res:.2e} after {it} sweeps", flush=True)
     assert res <= 5e-13
     return hm, arena, b, fac, x
 
 
-def test_cfg1_fp64_factor_parity(pp, orc, ctx, cfg1):
-    hm, arena, b, fac, _ = cfg1
-    print("[case] cfg1 FP64 factor n=16384 workers=256", flush=True)
+def test_hodlr16k_fp64_factor_parity(pp, orc, ctx, hodlr16k):
+    hm, arena, b, fac, _ = hodlr16k
+    print("[case] hodlr16k FP64 factor n=16384 workers=256", flush=True)
     got = gpu_factor(pp, ctx, hm, arena, 256, pp.FP64)
     worst = H.worst_leaf_diff(hm, got, fac)
-    print(f"[cfg1] worst leaf difference FP64 vs oracle {worst:.2e}", flush=True)
+    print(f"[hodlr16k] worst leaf difference FP64 vs oracle {worst:.2e}", flush=True)
     assert worst < 1e-9
 
 
-def test_cfg1_fp64_solve(pp, orc, ctx, cfg1):
-    hm, arena, b, fac, xo = cfg1
-    print("[case] cfg1 FP64 solve 16 rhs", flush=True)
+def test_hodlr16k_fp64_solve(pp, orc, ctx, hodlr16k):
+    hm, arena, b, fac, xo = hodlr16k
+    print("[case] hodlr16k FP64 solve 16 rhs", flush=True)
     x, rep = gpu_solve(pp, ctx, hm, arena, b, pp.FP64, 5e-11)
     res = H.residual(orc, hm, arena, x, b)
     err = float(np.linalg.norm(x - xo) / np.linalg.norm(xo))
-    print(f"[cfg1] sweeps {rep['iterations']} residual device {rep['residual']:.2e} host {res:.2e} "
+    print(f"[hodlr16k] sweeps {rep['iterations']} residual device {rep['residual']:.2e} host {res:.2e} "
           f"error vs oracle solution {err:.2e}", flush=True)
     assert rep["converged"] and res <= 1e-10 and err < 1e-8
 
 
-def test_cfg2_mixed_factor_parity(pp, orc, ctx, cfg1):
-    hm, arena, b, fac, _ = cfg1
-    print("[case] cfg2 mixed factor n=16384 workers=256", flush=True)
+def test_hodlr16k_mixed_factor_parity(pp, orc, ctx, hodlr16k):
+    hm, arena, b, fac, _ = hodlr16k
+    print("[case] hodlr16k_mixed mixed factor n=16384 workers=256", flush=True)
     got = gpu_factor(pp, ctx, hm, arena, 256, pp.MIXED)
     worst = H.worst_leaf_diff(hm, got, fac)
-    print(f"[cfg2] worst leaf difference mixed vs FP64 oracle {worst:.2e} (tolerance {MIXED_FACTOR_TOL})", flush=True)
+    print(f"[hodlr16k_mixed] worst leaf difference mixed vs FP64 oracle {worst:.2e} (tolerance {MIXED_FACTOR_TOL})", flush=True)
     assert worst < MIXED_FACTOR_TOL
 
 
-def test_cfg2_mixed_solve_1e12(pp, orc, ctx, cfg1):
-    hm, arena, b, fac, xo = cfg1
-    print("[case] cfg2 mixed solve n=16384 to 1e-12", flush=True)
+def test_hodlr16k_mixed_solve_1e12(pp, orc, ctx, hodlr16k):
+    hm, arena, b, fac, xo = hodlr16k
+    print("[case] hodlr16k_mixed mixed solve n=16384 to 1e-12", flush=True)
     x, rep = gpu_solve(pp, ctx, hm, arena, b, pp.MIXED, 5e-13)
     res = H.residual(orc, hm, arena, x, b)
     err = float(np.linalg.norm(x - xo) / np.linalg.norm(xo))
-    print(f"[cfg2] n=16384 mixed: sweeps {rep['iterations']} residual host {res:.2e} "
+    print(f"[hodlr16k_mixed] n=16384 mixed: sweeps {rep['iterations']} residual host {res:.2e} "
           f"error vs FP64 oracle solution {err:.2e}", flush=True)
     assert rep["converged"] and res <= 1e-12 and err < 1e-9
 
 
 @pytest.fixture(scope="module")
-def cfg2_large(pp, orc):
+def hodlr32k_r48(pp, orc):
     n = 32768
-    print(f"[case] cfg2 build n={n} levels=7 rank=48", flush=True)
+    print(f"[case] hodlr16k_mixed build n={n} levels=7 rank=48", flush=True)
     hm, kern, arena = pp.build_config(1, n, levels=7, rank=48)
     b = pp.rhs(1, n, 16)
     x, res, it = H.oracle_solution(pp, orc, hm, arena, b, 5e-13)
-    print(f"[cfg2] n=32768 oracle solution: residual {res:.2e} after {it} sweeps", flush=True)
+    print(f"[hodlr16k_mixed] n=32768 oracle solution: residual {res:.2e} after {it} sweeps", flush=True)
     assert res <= 5e-13
     return hm, arena, b, x
 
 
 @pytest.mark.parametrize("precision", [0, 2])
-def test_cfg2_n32768_rank48(pp, orc, ctx, cfg2_large, precision):
-    hm, arena, b, xo = cfg2_large
-    print(f"[case] cfg2 n=32768 rank=48 precision={precision} to 1e-12", flush=True)
+def test_hodlr32k_rank48(pp, orc, ctx, hodlr32k_r48, precision):
+    hm, arena, b, xo = hodlr32k_r48
+    print(f"[case] hodlr16k_mixed n=32768 rank=48 precision={precision} to 1e-12", flush=True)
     x, rep = gpu_solve(pp, ctx, hm, arena, b, precision, 5e-13)
     res = H.residual(orc, hm, arena, x, b)
     err = float(np.linalg.norm(x - xo) / np.linalg.norm(xo))
-    print(f"[cfg2] n=32768 precision={precision}: sweeps {rep['iterations']} residual host {res:.2e} "
+    print(f"[hodlr16k_mixed] n=32768 precision={precision}: sweeps {rep['iterations']} residual host {res:.2e} "
           f"error vs FP64 oracle solution {err:.2e}", flush=True)
     assert rep["converged"] and res <= 1e-12 and err < 1e-9
 
 
-def test_cfg3_batch_512(pp, orc, ctx):
+def test_batch_512(pp, orc, ctx):
     """The batch benchmark's 512 matrices in one hipGraph-replayed launch."""
     wl = __import__("importlib").import_module("pastel-palettes_amd.workloads")
     tune = __import__("importlib").import_module("pastel-palettes_amd.tune")
@@ -151,7 +151,7 @@ def test_cfg3_batch_512(pp, orc, ctx):
     count, nrhs = 512, 4
     sizes = wl.log_uniform_sizes(rng, count, 512, 8192, 256)
     ranks = [int(r) for r in rng.choice([16, 24, 32], count)]
-    print(f"[case] cfg3 batch of {count}: rows {sum(sizes)}, N {min(sizes)}..{max(sizes)}", flush=True)
+    print(f"[case] batch512 batch of {count}: rows {sum(sizes)}, N {min(sizes)}..{max(sizes)}", flush=True)
     hms, arenas = [], []
     for i, (n, r) in enumerate(zip(sizes, ranks)):
         hm = pp.HMatrix("hodlr", n, levels=tune.hodlr_levels(n, 256), rank=r)
@@ -172,7 +172,7 @@ def test_cfg3_batch_512(pp, orc, ctx):
         solver.factor(stream.cuda_stream)
     rep = solver.solve(db.data_ptr(), dx.data_ptr(), nrhs, tol=5e-11, max_iter=60, hip_stream=stream.cuda_stream)
     torch.cuda.synchronize()
-    print(f"[cfg3] factor {solver.last_factor_ms():.1f} ms, sweeps {rep['iterations']}, "
+    print(f"[batch512] factor {solver.last_factor_ms():.1f} ms, sweeps {rep['iterations']}, "
           f"residual {rep['residual']:.2e}", flush=True)
     assert rep["converged"]
     X = dx.cpu().numpy()
@@ -183,19 +183,19 @@ def test_cfg3_batch_512(pp, orc, ctx):
         a = arenas[i].cpu().numpy()
         x = X[m["row_off"]:m["row_off"] + hm.n]
         worst = max(worst, H.residual(orc, hm, a, x, bs[i]))
-    print(f"[cfg3] worst member residual on the host {worst:.2e}", flush=True)
+    print(f"[batch512] worst member residual on the host {worst:.2e}", flush=True)
     assert worst <= 1e-10
     order = np.argsort(sizes)
     for i in (int(order[0]), int(order[len(order) // 2]), int(order[-1])):
         a = arenas[i].cpu().numpy()
         want = H.oracle_factor(pp, orc, hms[i], a)
         d = H.worst_leaf_diff(hms[i], fac_all, want, base_got=members[i]["arena_off"])
-        print(f"[cfg3] member {i} n={sizes[i]} rank={ranks[i]}: FP32 factors vs FP64 oracle {d:.2e}", flush=True)
+        print(f"[batch512] member {i} n={sizes[i]} rank={ranks[i]}: FP32 factors vs FP64 oracle {d:.2e}", flush=True)
         assert d < 1e-4  # FP32 arithmetic (2^-24) through a few hundred dependent updates
     solver.close()
 
 
-def test_cfg4_serving_burst(pp, orc, ctx):
+def test_serving_burst(pp, orc, ctx):
     """A burst of the serving mix over a pool up to N = 32768: each result's
     residual on the host <= 1e-10, and the same bits alone and in the mix."""
     from importlib import import_module
@@ -204,7 +204,7 @@ def test_cfg4_serving_burst(pp, orc, ctx):
     rng = np.random.default_rng(5000)  # the serving benchmark's pool draw
     sizes = wl.log_uniform_sizes(rng, 12, 1024, 32768, 256)
     sizes[int(np.argmax(sizes))] = 32768  # the top of the range is always exercised
-    print(f"[case] cfg4 serving pool N = {sizes}", flush=True)
+    print(f"[case] serving serving pool N = {sizes}", flush=True)
     mats, host = {}, {}
     for i, n in enumerate(sizes):
         hm = pp.HMatrix("hodlr", n, levels=max(1, int(round(math.log2(n / 256)))), rank=32)
@@ -226,7 +226,7 @@ def test_cfg4_serving_burst(pp, orc, ctx):
                            workers_per_slot=256 // slots)
         reqs, wall = serve.replay(srv, trace, make_rhs, realtime=False)
         srv.close()
-        print(f"[cfg4] slots={slots}: {len(reqs)} requests in {wall:.2f} s", flush=True)
+        print(f"[serving] slots={slots}: {len(reqs)} requests in {wall:.2f} s", flush=True)
         runs[slots] = reqs
     worst = 0.0
     for mix, alone in zip(runs[4], runs[1]):
@@ -236,5 +236,5 @@ def test_cfg4_serving_burst(pp, orc, ctx):
         worst = max(worst, res)
         assert mix.report["converged"] and res <= 1e-10, (mix.rid, mix.key, res)
         assert np.array_equal(mix.x, alone.x), f"request {mix.rid} differs alone vs in the mix"
-    print(f"[cfg4] worst residual on the host {worst:.2e}; reuse requests "
+    print(f"[serving] worst residual on the host {worst:.2e}; reuse requests "
           f"{sum(r.reuse for r in runs[4])}", flush=True)

@@ -129,6 +129,49 @@ __device__ __forceinline__ void diag_inverses(const T* F, int ldf, int n, T* out
   }
 }
 
+// diag_inverses with the inverses left in LDS: block b at D + b NB (NB + 1)
+// (row-major, pitch NB + 1, in place of its loaded copy; column NB holds the
+// reciprocals of the diagonal in the upper case).  n a multiple of NB, at
+// most 2 NB blocks per wave (one pass); ends with a barrier.
+template <typename T, bool UPPER>
+__device__ __forceinline__ void diag_inverses_lds(const T* F, int ldf, int n, PP_LDS T* D) {
+  const int w = wave_id(), l = lane_id();
+  const int half = l >> 5, lc = l & 31;
+  const int nblk = n / NB;
+  constexpr int V = 16 / (int)sizeof(T);
+  typedef T vec_t __attribute__((ext_vector_type(V)));
+  const bool fvec = (ldf % V) == 0 && (reinterpret_cast<uintptr_t>(F) & 15) == 0;
+  const int b = 2 * w + half, r0 = b * NB;
+  PP_LDS T* s = D + b * NB * (NB + 1);
+  if (b < nblk) {
+    // lane lc loads row lc of the block (whole vectors when aligned)
+    T v[NB];
+    if (fvec) {
+#pragma unroll
+      for (int q = 0; q < NB / V; ++q) {
+        const vec_t x = gload<vec_t>(F + (int64_t)(r0 + lc) * ldf + r0 + q * V);
+#pragma unroll
+        for (int e = 0; e < V; ++e) v[q * V + e] = x[e];
+      }
+    } else {
+#pragma unroll
+      for (int j = 0; j < NB; ++j) v[j] = gload<T>(F + (int64_t)(r0 + lc) * ldf + r0 + j);
+    }
+#pragma unroll
+    for (int j = 0; j < NB; ++j) s[lc * (NB + 1) + j] = v[j];
+    if (UPPER) s[lc * (NB + 1) + NB] = T(1) / (T)s[lc * (NB + 1) + lc];
+  }
+  __syncthreads();
+  if (b < nblk) {
+    T y[NB];
+    tri_inverse_column<T, UPPER>(s, NB, lc, y);
+    // the half-wave has read the whole block (in order before these writes)
+#pragma unroll
+    for (int i = 0; i < NB; ++i) s[i * (NB + 1) + lc] = y[i];
+  }
+  __syncthreads();
+}
+
 // ---------------------------------------------------------------- panel update
 // Pitch of the LDS operand tiles of lu_update_lds: rows of NB elements padded
 // so that the 16-byte fragment reads of eight consecutive lanes fall in
@@ -539,15 +582,42 @@ __device__ __forceinline__ int narrow_solve_bytes(int m, int w) {
   return nblk * NB * wp * (int)sizeof(T) + nblk * (int)sizeof(int);  // W, then one ready flag per block
 }
 
+// LDS offset (bytes) of the diagonal inverses a narrow solve keeps in LDS
+// (after W and its ready flags), and whether it does: m a multiple of NB,
+// at most eight blocks, everything in LDS.
+template <typename T>
+__device__ __forceinline__ int narrow_dl_offset(int m, int w) {
+  return (narrow_solve_bytes<T>(m, w) + 15) & ~15;
+}
+template <typename T>
+__device__ __forceinline__ bool narrow_dl_ok(int m, int w) {
+  // FP32 only: in FP64 the fused form measured slower (larger register
+  // frames, the FP64 factorization 0.9 ms longer)
+  return sizeof(T) == 4 && w <= 64 && m % NB == 0 && m / NB <= 8 &&
+         narrow_dl_offset<T>(m, w) + (m / NB) * NB * (NB + 1) * (int)sizeof(T) <= kLdsBytes - 1024;
+}
+
+// dl = 0: the inverses of T's diagonal blocks come from dinv (global memory);
+// dl = 1 (unit lower T) or 2 (upper T): they are formed here into LDS
+// (diag_inverses_lds at narrow_dl_offset, which narrow_dl_ok must admit)
+// before W is staged, and read from there.
 template <typename T, int MODE>
 __device__ __forceinline__ void narrow_solve(T* Wg, int ldw, bool w_trans, int m, int w, const T* F, int ldf,
-                                             bool f_trans, const T* dinv, bool forward) {
+                                             bool f_trans, const T* dinv, bool forward, int dl = 0) {
   using P = Mma<T, MODE>;
   using acc_t = typename P::acc_t;
   const int ntc = (w + 31) >> 5, wc = ntc * 32, wp = wc + 16 / (int)sizeof(T);
   const int nblk = (m + NB - 1) / NB, mr = nblk * NB;
   PP_LDS T* Ws = (PP_LDS T*)lds_base();
   const int tid = threadIdx.x, nt = blockDim.x, wv = wave_id(), nw = nt >> 6, l = lane_id(), li = l & 15;
+  // (FP32 only: narrow_dl_ok admits no FP64 solve, and the FP64 kernels are
+  // compiled without this path)
+  PP_LDS T* Dl = nullptr;
+  if constexpr (sizeof(T) == 4) {
+    if (dl != 0) Dl = (PP_LDS T*)(lds_base() + narrow_dl_offset<T>(m, w));
+    if (dl == 1) diag_inverses_lds<T, false>(F, ldf, m, Dl);
+    else if (dl == 2) diag_inverses_lds<T, true>(F, ldf, m, Dl);
+  }
   // stage W (in the order that keeps global accesses coalesced); rows >= m
   // and columns >= w are zero and stay zero.  The loads go through the
   // global address space with clamped addresses (no branch), so the compiler
@@ -625,11 +695,25 @@ __device__ __forceinline__ void narrow_solve(T* Wg, int ldw, bool w_trans, int m
       const int b = forward ? ob : nblk - 1 - ob;
       const int rb = b * NB, bsb = (m - rb) < NB ? (m - rb) : NB;
       // D_b's fragments are requested first (they wait for nothing)
-      OperandLane<T, P::EPL> ld_[2];
-#pragma unroll
-      for (int h = 0; h < 2; ++h) ld_[h].init(dinv + (int64_t)b * NB * NB, NB, !f_trans, 16 * h + li, bsb);
       T fd[CH][2][P::EPL];
-      load_a_frags<T, MODE>(ld_, bsb, fd);
+      if (sizeof(T) == 4 && Dl != nullptr) {
+        // element (r, q) of D_b at Db[r (NB + 1) + q]; fragment rows 16 h + li
+        const PP_LDS T* Db = Dl + b * NB * (NB + 1);
+#pragma unroll
+        for (int c = 0; c < CH; ++c)
+#pragma unroll
+          for (int h = 0; h < 2; ++h)
+#pragma unroll
+            for (int e = 0; e < P::EPL; ++e) {
+              const int r = 16 * h + li, q = c * P::KW + P::EPL * (l >> 4) + e;
+              fd[c][h][e] = f_trans ? (T)Db[q * (NB + 1) + r] : (T)Db[r * (NB + 1) + q];
+            }
+      } else {
+        OperandLane<T, P::EPL> ld_[2];
+#pragma unroll
+        for (int h = 0; h < 2; ++h) ld_[h].init(dinv + (int64_t)b * NB * NB, NB, !f_trans, 16 * h + li, bsb);
+        load_a_frags<T, MODE>(ld_, bsb, fd);
+      }
       for (int oj = 0; oj < ob; ++oj) {
         const int j = forward ? oj : nblk - 1 - oj;
         const int r0 = j * NB, bsj = (m - r0) < NB ? (m - r0) : NB;

@@ -107,6 +107,7 @@ __device__ __forceinline__ void set_tsolve(Step<T>& st, T* W, int ldw, bool w_tr
   st.aT = f_trans;
   st.B = dinv;
   st.k = forward ? 1 : 0;
+  st.ldb = 0;  // narrow_solve's dl: inverses from dinv (exec_entry may set 1 or 2)
 }
 
 // Entry length in words, or -1 for an unknown operation.
@@ -538,7 +539,7 @@ __device__ __forceinline__ void exec_step(const Step<T>& st) {
       lu_update<T, FMODE>(st.C, st.ldc, st.m, st.n, st.lda, st.k, st.A);
       break;
     case ST_TSOLVE:
-      narrow_solve<T, FMODE>(st.C, st.ldc, st.bT, st.m, st.n, st.A, st.lda, st.aT, st.B, st.k != 0);
+      narrow_solve<T, FMODE>(st.C, st.ldc, st.bT, st.m, st.n, st.A, st.lda, st.aT, st.B, st.k != 0, st.ldb);
       break;
     case ST_LDSLU:
       getrf_lds<T, FMODE>(st.C, st.ldc, st.m);
@@ -719,12 +720,27 @@ __device__ __forceinline__ int exec_entry(const int32_t* pc, void* const* ptrs, 
       return len;
     }
   }
+  // a narrow triangular solve whose diagonal-block inverses fit in LDS next
+  // to its right-hand side forms them itself (dl): the separate step and the
+  // round trip through the worker scratch are skipped
+  int dl = 0;
+  {
+    const int opw = word(pc, 2);
+    if (opw == trsml || opw == trsmr || opw == trsmu) {
+      const int n = opw == trsmr ? word(pc, 7) : word(pc, 8), wd = opw == trsmr ? word(pc, 8) : word(pc, 7);
+      if (narrow_ok<T>(n, wd) && narrow_dl_ok<T>(n, wd)) dl = opw == trsml ? 1 : 2;
+    }
+  }
   for (int s = 0;; ++s) {
     Step<T> st;
     const int r = make_step<T>(pc, ptrs, s, tmp, reinterpret_cast<const T*>(rnd), st);
     if (r < 0) return -1;
     if (r == 0) break;
     if (st.kind == ST_QR && fallbacks != nullptr && threadIdx.x == 0) atomicAdd(fallbacks, 1);
+    if (dl != 0) {
+      if (st.kind == ST_DIAGINV) continue;
+      if (st.kind == ST_TSOLVE) st.ldb = dl;
+    }
     exec_step<T, FMODE, GMODE>(st);
   }
   return len;

@@ -447,6 +447,69 @@ __device__ __forceinline__ void gemm_staged_run(int m, int n, int k, T alpha, T 
         }
     }
   };
+  if constexpr (MODE == MODE_FP32) {
+    if (nch == 2) {
+      // FP32 MFMA with k = 32 (two chunks): the next tile's fragments are
+      // read from LDS before the current tile's MFMAs (the FP32 products are
+      // long enough to hide the LDS latency behind them)
+      T fr[2][2][2][2][EPL];  // [buffer][chunk][a/b][group][element]
+      auto load_fr = [&](auto b_c, int q) {
+        constexpr int bb = decltype(b_c)::value;
+        int r0, c0;
+        tile_rc(q, r0, c0);
+#pragma unroll
+        for (int c = 0; c < 2; ++c)
+#pragma unroll
+          for (int h = 0; h < 2; ++h) {
+            const int ra = r0 + 16 * h + li, rb = m + c0 + 16 * h + li;
+#pragma unroll
+            for (int v = 0; v < EPL / V; ++v) {
+              const int ch = (c * KW + EPL * g) / V + v;
+              const vec_t x = *(const PP_LDS vec_t*)(sa + ra * k + V * (ch ^ (ra & (cpr - 1))));
+              const vec_t y = *(const PP_LDS vec_t*)(sa + rb * k + V * (ch ^ (rb & (cpr - 1))));
+#pragma unroll
+              for (int e = 0; e < V; ++e) {
+                fr[bb][c][0][h][V * v + e] = x[e];
+                fr[bb][c][1][h][V * v + e] = y[e];
+              }
+            }
+          }
+      };
+      auto pbody = [&](auto s_c, int q) {
+        constexpr int s = decltype(s_c)::value;
+        constexpr int bb = s & 1;
+        int r0, c0;
+        tile_rc(q, r0, c0);
+        load_fr(IC<bb ^ 1>{}, q + 1);
+        acc_t acc[2][2];
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+          for (int j = 0; j < 2; ++j) acc[i][j] = BETA ? ab * cb[s][i][j] : P::zero();
+        if constexpr (BETA) load_c(IC<s>{}, q + RING);
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int c = 0; c < 2; ++c)
+#pragma unroll
+          for (int i = 0; i < 2; ++i)
+#pragma unroll
+            for (int j = 0; j < 2; ++j) acc[i][j] = P::mma_chunk(fr[bb][c][1][i], fr[bb][c][0][j], acc[i][j]);
+        if (q < my) {
+#pragma unroll
+          for (int i = 0; i < 2; ++i)
+#pragma unroll
+            for (int j = 0; j < 2; ++j) gstore<f4_t>(cptr(r0, c0, i, j, 0), alpha * acc[i][j]);
+        }
+      };
+      load_fr(IC<0>{}, 0);
+      for (int q0 = 0; q0 < my; q0 += RING) {
+        pbody(IC<0>{}, q0);
+        pbody(IC<1>{}, q0 + 1);
+      }
+      __syncthreads();
+      return;
+    }
+  }
   for (int q0 = 0; q0 < my; q0 += RING) {
     body(IC<0>{}, q0);
     body(IC<1>{}, q0 + 1);

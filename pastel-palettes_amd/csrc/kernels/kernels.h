@@ -14,6 +14,11 @@ struct MatvecBlock {
   int32_t row0, col0, ny, nx, rank;
   int64_t off, off_u, off_v;  // element offsets in the arena the plan is applied to
   int64_t t_off;              // low rank: offset of its rank x nrhs V^T x block
+  // low rank: V^T x is formed in nparts partial products over row parts of
+  // V (at most kPartRows rows each), part s at p_off + s * p_stride of the
+  // partial area
+  int32_t part0, nparts;
+  int64_t p_off, p_stride;
 };
 
 // A row range of the product and the leaves covering it.
@@ -26,16 +31,23 @@ struct MatvecPlanDev {
   const MatvecTile* tiles;
   const int* contrib;  // pairs (block index, unused)
   const int* lr_list;  // indices of low-rank blocks
-  int n_tiles, n_lowrank;
-  int64_t t_elems = 0;  // V^T x area (max t_off + rank * nrhs); 0: unsplit products
+  const int* parts;    // pairs (block index, part number) of the V^T x partial products
+  int n_tiles, n_lowrank, n_parts;
+  int max_rank;         // largest rank of a low-rank block
+  int64_t t_elems = 0;  // V^T x area (max t_off + rank * nrhs)
+  int64_t p_elems = 0;  // partial area that follows it
 };
 
-// V^T x of a low-rank block is formed by kHmvSplit workgroups over row
-// chunks of V (one workgroup per block would leave most of the GPU idle and
-// stream an 8192-row factor through one CU), whose partial products are
-// summed in chunk order by a second launch.  The t buffer of the matvec
-// launches then holds (1 + kHmvSplit) * t_elems elements.
-constexpr int kHmvSplit = 8;
+// Rows of V per V^T x partial product: a part is one 16 x 16 MFMA output tile
+// per wave pass whose loads for the whole part are in flight at once, and an
+// 8192-row factor is spread over 64 workgroups instead of streaming through
+// one CU.
+constexpr int kPartRows = 128;
+// Rows of a row tile of the product (a diagonal leaf's rows split four ways
+// at leaf 256); each tile is handled by up to four workgroups of 16 rows.
+constexpr int kTileRows = 64;
+// Rows per workgroup of the first stage of the per-member column norms.
+constexpr int kNormRows = 256;
 
 // ---- H-matrix construction (build.hip)
 constexpr int kBuildMaxRank = 256;
@@ -126,15 +138,35 @@ hipError_t launch_copy2d(void* dst, int dst_bytes, int ldd, const void* src, int
                          hipStream_t s);
 hipError_t launch_add_correction(double* x, int ldx, const void* d, int d_bytes, int ldd, int n, int nrhs,
                                  hipStream_t s);
+// Optional outputs of an H-matrix-vector product besides out (which may be
+// null): a second copy in 4- or 8-byte elements (out2, pitch ld2); an FP64
+// target xo set to (xmode 1) or incremented by (xmode 2) the product; the
+// per-workgroup column sums of squares of the product (nrm_part, n_tiles x
+// kTileRows / 16 rows of nrhs) for launch_rownorm_sum.
+struct HmvExtra {
+  void* out2 = nullptr;
+  int out2_bytes = 0, ld2 = 0;
+  double* xo = nullptr;
+  int ldxo = 0, xmode = 0;
+  double* nrm_part = nullptr;
+};
+// out = y - A_H x (FP64 residual)
 hipError_t launch_hmatvec(const MatvecPlanDev& p, const double* A, const double* x, int ldx, const double* y, int ldy,
-                          double* out, int ldo, int nrhs, double* t, hipStream_t s);
+                          double* out, int ldo, int nrhs, double* t, const HmvExtra& ex, hipStream_t s);
 // out = sum(D x) - sum(U (V^T x)) over the plan (a hierarchical inverse whose
 // off-diagonal blocks are stored as -U V^T), in the arena's element type
 // (8: FP64, 4: FP32).
 hipError_t launch_happly(const MatvecPlanDev& p, int elem_bytes, const void* A, const void* x, int ldx, void* out,
-                         int ldo, int nrhs, void* t, hipStream_t s);
-hipError_t launch_segnorm2(const double* r, int ldr, const int* seg, int nseg, int nrhs, double* out, hipStream_t s);
-hipError_t launch_colnorm2(const double* r, int ldr, int n, int nrhs, double* out, hipStream_t s);
+                         int ldo, int nrhs, void* t, const HmvExtra& ex, hipStream_t s);
+// out[g nrhs + c] = sum of squares of column c over member g's row tiles
+// (seg_tiles: pairs first tile, tile count) from a product's nrm_part.
+hipError_t launch_rownorm_sum(const double* part, const int* seg_tiles, int nseg, int nrhs, double* out,
+                              hipStream_t s);
+// Per (member, column) sums of squares of r: a first launch sums rows in
+// chunks of kNormRows into part (nseg x nchunks x nrhs), a second adds the
+// chunks of each member in order into out (nseg x nrhs).
+hipError_t launch_segnorm2(const double* r, int ldr, const int* seg, int nseg, int nrhs, int nchunks, double* part,
+                           double* out, hipStream_t s);
 hipError_t launch_mfma_probe(int mode, const double* a, const double* b, double* out, hipStream_t s);
 hipError_t launch_single_op(int precision, int threads, int reps, const SingleOp& op, void* scratch, const void* rnd,
                             int32_t* status, hipStream_t s);

@@ -194,6 +194,33 @@ void destroy_stream(pp_stream* s) {
 
 }  // namespace
 
+namespace ppr {
+
+int stream_status(pp_stream* s, const int32_t* st, int32_t* first_unsignalled) {
+  s->last_fallbacks = st[5];
+  if (st[0] == pp::VM_OK) return PP_OK;
+  if (first_unsignalled) {
+    uint32_t epoch = 0;
+    std::vector<uint32_t> flags(std::max(s->n_insts, 1));
+    HIP_TRY(hipMemcpy(&epoch, s->d_epoch, 4, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(flags.data(), s->d_flags, 4 * flags.size(), hipMemcpyDeviceToHost));
+    for (int i = 0; i < s->n_insts; ++i)
+      if ((int32_t)(flags[i] - epoch) < 0) {
+        *first_unsignalled = i;
+        break;
+      }
+  }
+  const char* what = st[0] == pp::VM_TIMEOUT ? "watchdog: wait timed out"
+                     : st[0] == pp::VM_BAD_OPCODE ? "bad opcode"
+                     : st[0] == pp::VM_BAD_OPERATION ? "bad operation"
+                                                     : "aborted";
+  return set_error(st[0] == pp::VM_TIMEOUT ? PP_ERR_TIMEOUT : PP_ERR_DEVICE,
+                   std::string("VM ") + what + " (worker " + std::to_string(st[1]) + ", id " + std::to_string(st[2]) +
+                       ")");
+}
+
+}  // namespace ppr
+
 extern "C" {
 
 int pp_ctx_create(int device, uint64_t budget_bytes, pp_ctx** out) {
@@ -393,26 +420,7 @@ int pp_wait(pp_ctx* ctx, pp_stream* s, int timeout_ms, int32_t* first_unsignalle
   }
   int32_t st[8];
   HIP_TRY(hipMemcpy(st, s->d_status, sizeof(st), hipMemcpyDeviceToHost));
-  s->last_fallbacks = st[5];
-  if (st[0] == pp::VM_OK) return PP_OK;
-  if (first_unsignalled) {
-    uint32_t epoch = 0;
-    std::vector<uint32_t> flags(std::max(s->n_insts, 1));
-    HIP_TRY(hipMemcpy(&epoch, s->d_epoch, 4, hipMemcpyDeviceToHost));
-    HIP_TRY(hipMemcpy(flags.data(), s->d_flags, 4 * flags.size(), hipMemcpyDeviceToHost));
-    for (int i = 0; i < s->n_insts; ++i)
-      if ((int32_t)(flags[i] - epoch) < 0) {
-        *first_unsignalled = i;
-        break;
-      }
-  }
-  const char* what = st[0] == pp::VM_TIMEOUT ? "watchdog: wait timed out"
-                     : st[0] == pp::VM_BAD_OPCODE ? "bad opcode"
-                     : st[0] == pp::VM_BAD_OPERATION ? "bad operation"
-                                                     : "aborted";
-  return set_error(st[0] == pp::VM_TIMEOUT ? PP_ERR_TIMEOUT : PP_ERR_DEVICE,
-                   std::string("VM ") + what + " (worker " + std::to_string(st[1]) + ", id " + std::to_string(st[2]) +
-                       ")");
+  return ppr::stream_status(s, st, first_unsignalled);
 }
 
 double pp_stream_last_ms(const pp_stream* s) {

@@ -108,6 +108,11 @@ int64_t single_op_sketch(const int32_t* words);
 // current); called once per device by pp_ctx_create.
 int noalloc_init(int device);
 
+// Interprets the 8 status words of a finished launch of s (records the
+// accum fallbacks; on a VM error optionally finds the first unsignalled
+// instruction) and returns PP_OK or the error.
+int stream_status(pp_stream* s, const int32_t* st, int32_t* first_unsignalled);
+
 // Enqueue the launch sequence of a stream (temp fills, epoch bump, VM) on st
 // without waiting; usable under hipGraph capture.
 int enqueue_launch(pp_ctx* ctx, pp_stream* s, int wavefronts_per_inst, hipStream_t st, bool record_events);

@@ -23,6 +23,7 @@
 #include <map>
 #include <memory>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../io/wire.h"
@@ -50,10 +51,16 @@ struct pp_solver {
   };
   std::map<int, SolveProg> sprogs;
   void* d_rhs = nullptr;    // n_total x nrhs_max, storage precision
-  double* d_r = nullptr;    // n_total x nrhs_max residual
   double* d_t = nullptr;    // matvec low-rank products
   double* d_nrm = nullptr;  // [members * nrhs_max] r norms, then the same for b
+  double* d_npart = nullptr;  // [members * norm_chunks * nrhs_max] first-stage norm sums
+  int norm_chunks = 1;        // kNormRows chunks of the largest member
+  double* h_nrm = nullptr;    // pinned host copy of d_nrm
+  int32_t* h_status = nullptr;  // pinned host copy of the factor launch's status words
   int* d_seg = nullptr;     // member row ranges (row0, rows)
+  int* d_seg_tiles = nullptr;  // member row-tile ranges of the residual plan (first, count)
+  double* d_rpart = nullptr;   // residual workgroups' column sums of squares
+  hipEvent_t ev_done = nullptr;  // end of the work a host wait covers
   pp::MatvecPlanDev plan{};  // A_H (FP64 arena): residual
   // Hierarchical-inverse solve (every member HODLR-shaped): x = U^{-1} L^{-1} b
   // as two H-matrix products over the inverse areas of the factor arena.
@@ -81,7 +88,8 @@ const Hierarchical* hier(const pp_hmatrix* h) { return h->h.get(); }
 
 // Uploads a block list as a matvec plan: one row tile per dense diagonal
 // block (they must tile the stacked rows), each with the blocks covering it.
-int upload_plan(pp_solver* s, const std::vector<pp::MatvecBlock>& blocks, pp::MatvecPlanDev* plan) {
+int upload_plan(pp_solver* s, std::vector<pp::MatvecBlock> blocks, pp::MatvecPlanDev* plan,
+                std::vector<pp::MatvecTile>* tiles_out = nullptr) {
   std::vector<int> lr;
   std::vector<std::pair<int64_t, int>> ranges;  // global row0, rows
   for (size_t i = 0; i < blocks.size(); ++i) {
@@ -91,10 +99,27 @@ int upload_plan(pp_solver* s, const std::vector<pp::MatvecBlock>& blocks, pp::Ma
   }
   if (s->n_total > 0x7fffffffLL) return set_error(PP_ERR_INVALID, "stacked rows exceed int32");
   std::sort(ranges.begin(), ranges.end());
-  // Row tiles of at most 64 rows (a diagonal leaf's rows split four ways at
-  // leaf 256): one workgroup each, so an N = 16384 product has 256 of them
-  // instead of 64.
-  constexpr int kTileRows = 64;
+  // V^T x partial products: parts of at most kPartRows rows of V, each
+  // block's partials contiguous (room for nrhs_max columns)
+  std::vector<int> parts;
+  int64_t p_elems = 0;
+  int max_rank = 0;
+  for (int i : lr) {
+    auto& b = blocks[i];
+    b.part0 = (int32_t)(parts.size() / 2);
+    b.nparts = b.rank > 0 && b.nx > 0 ? (b.nx + pp::kPartRows - 1) / pp::kPartRows : 0;
+    b.p_off = p_elems;
+    b.p_stride = (int64_t)b.rank * s->nrhs_max;
+    p_elems += b.nparts * b.p_stride;
+    for (int q = 0; q < b.nparts; ++q) {
+      parts.push_back(i);
+      parts.push_back(q);
+    }
+    max_rank = std::max(max_rank, b.nparts > 0 ? b.rank : 0);
+  }
+  // Row tiles of at most kTileRows rows, each product workgroup taking 16 of
+  // them: an N = 16384 product at leaf 256 has 1024 workgroups.
+  constexpr int kTileRows = pp::kTileRows;
   std::vector<pp::MatvecTile> tiles;
   std::vector<int> contrib;
   std::vector<int64_t> rows_of(blocks.size(), 0);  // rows of the tiles each block contributes to
@@ -126,7 +151,7 @@ int upload_plan(pp_solver* s, const std::vector<pp::MatvecBlock>& blocks, pp::Ma
       return set_error(PP_ERR_INVALID, "matvec plan: block " + std::to_string(i) + " is not aligned to the row tiles");
   pp::MatvecBlock* d_blocks = nullptr;
   pp::MatvecTile* d_tiles = nullptr;
-  int *d_contrib = nullptr, *d_lr = nullptr;
+  int *d_contrib = nullptr, *d_lr = nullptr, *d_parts = nullptr;
   int rc;
   if ((rc = ctx_alloc_t(s->ctx, blocks.size(), &d_blocks))) return rc;
   s->owned.push_back(d_blocks);
@@ -136,17 +161,25 @@ int upload_plan(pp_solver* s, const std::vector<pp::MatvecBlock>& blocks, pp::Ma
   s->owned.push_back(d_contrib);
   if ((rc = ctx_alloc_t(s->ctx, std::max<size_t>(lr.size(), 1), &d_lr))) return rc;
   s->owned.push_back(d_lr);
+  if ((rc = ctx_alloc_t(s->ctx, std::max<size_t>(parts.size(), 2), &d_parts))) return rc;
+  s->owned.push_back(d_parts);
   HIP_TRY(hipMemcpy(d_blocks, blocks.data(), sizeof(pp::MatvecBlock) * blocks.size(), hipMemcpyHostToDevice));
   HIP_TRY(hipMemcpy(d_tiles, tiles.data(), sizeof(pp::MatvecTile) * tiles.size(), hipMemcpyHostToDevice));
   if (!contrib.empty())
     HIP_TRY(hipMemcpy(d_contrib, contrib.data(), sizeof(int) * contrib.size(), hipMemcpyHostToDevice));
   if (!lr.empty()) HIP_TRY(hipMemcpy(d_lr, lr.data(), sizeof(int) * lr.size(), hipMemcpyHostToDevice));
+  if (!parts.empty()) HIP_TRY(hipMemcpy(d_parts, parts.data(), sizeof(int) * parts.size(), hipMemcpyHostToDevice));
   plan->blocks = d_blocks;
   plan->tiles = d_tiles;
   plan->contrib = d_contrib;
   plan->lr_list = d_lr;
   plan->n_tiles = (int)tiles.size();
   plan->n_lowrank = (int)lr.size();
+  plan->parts = d_parts;
+  plan->n_parts = (int)(parts.size() / 2);
+  plan->max_rank = max_rank;
+  plan->p_elems = p_elems;
+  if (tiles_out) *tiles_out = std::move(tiles);
   return PP_OK;
 }
 
@@ -177,15 +210,40 @@ int build_plan(pp_solver* s) {
     }
   }
   int rc;
-  if ((rc = upload_plan(s, blocks, &s->plan))) return rc;
+  std::vector<pp::MatvecTile> tiles;
+  if ((rc = upload_plan(s, blocks, &s->plan, &tiles))) return rc;
   s->plan.t_elems = t_off;
-  if ((rc = ctx_alloc_t(s->ctx, std::max<int64_t>((1 + pp::kHmvSplit) * t_off, 1), &s->d_t))) return rc;
+  if ((rc = ctx_alloc_t(s->ctx, std::max<int64_t>(t_off + s->plan.p_elems, 1), &s->d_t))) return rc;
   s->owned.push_back(s->d_t);
   std::vector<int> seg;
+  int max_rows = 1;
   for (size_t m = 0; m < s->hms.size(); ++m) {
     seg.push_back((int)s->row_off[m]);
     seg.push_back(hier(s->hms[m])->getNx_abs());
+    max_rows = std::max(max_rows, seg.back());
   }
+  s->norm_chunks = (max_rows + pp::kNormRows - 1) / pp::kNormRows;
+  // each member's row tiles are consecutive (tiles follow the stacked rows)
+  std::vector<int> seg_tiles;
+  for (size_t m = 0; m < s->hms.size(); ++m) {
+    const int64_t r0 = s->row_off[m], r1 = r0 + hier(s->hms[m])->getNx_abs();
+    int first = -1, count = 0;
+    for (size_t i = 0; i < tiles.size(); ++i)
+      if (tiles[i].row0 >= r0 && tiles[i].row0 < r1) {
+        if (first < 0) first = (int)i;
+        if ((int)i != first + count) return set_error(PP_ERR_INTERNAL, "matvec plan: member tiles not consecutive");
+        ++count;
+      }
+    seg_tiles.push_back(std::max(first, 0));
+    seg_tiles.push_back(count);
+  }
+  if ((rc = ctx_alloc_t(s->ctx, seg_tiles.size(), &s->d_seg_tiles))) return rc;
+  s->owned.push_back(s->d_seg_tiles);
+  HIP_TRY(hipMemcpy(s->d_seg_tiles, seg_tiles.data(), sizeof(int) * seg_tiles.size(), hipMemcpyHostToDevice));
+  if ((rc = ctx_alloc_t(s->ctx, (int64_t)tiles.size() * (pp::kTileRows / 16) * s->nrhs_max, &s->d_rpart))) return rc;
+  s->owned.push_back(s->d_rpart);
+  if ((rc = ctx_alloc_t(s->ctx, (int64_t)s->hms.size() * s->norm_chunks * s->nrhs_max, &s->d_npart))) return rc;
+  s->owned.push_back(s->d_npart);
   if ((rc = ctx_alloc_t(s->ctx, seg.size(), &s->d_seg))) return rc;
   s->owned.push_back(s->d_seg);
   HIP_TRY(hipMemcpy(s->d_seg, seg.data(), sizeof(int) * seg.size(), hipMemcpyHostToDevice));
@@ -250,7 +308,8 @@ int build_inverse_plans(pp_solver* s) {
   s->lplan.t_elems = tl;
   s->uplan.t_elems = tu;
   const int eb = elem_bytes(s->precision);
-  if ((rc = ctx_alloc(s->ctx, (uint64_t)std::max<int64_t>((1 + pp::kHmvSplit) * std::max(tl, tu), 1) * eb,
+  if ((rc = ctx_alloc(s->ctx,
+                      (uint64_t)std::max<int64_t>(std::max(tl + s->lplan.p_elems, tu + s->uplan.p_elems), 1) * eb,
                       &s->d_tinv)))
     return rc;
   s->owned.push_back(s->d_tinv);
@@ -270,6 +329,9 @@ void destroy_solver(pp_solver* s) {
     if (kv.second.prog) pp_program_destroy(kv.second.prog);
   }
   for (void* p : s->owned) (void)ctx_free(s->ctx, p);
+  if (s->h_nrm) (void)hipHostFree(s->h_nrm);
+  if (s->h_status) (void)hipHostFree(s->h_status);
+  if (s->ev_done) (void)hipEventDestroy(s->ev_done);
   delete s;
 }
 
@@ -346,10 +408,15 @@ int solver_create(pp_ctx* ctx, const pp_hmatrix* const* hs, int count, const dou
     s->owned.push_back(s->d_F);
     if ((rc = ctx_alloc(ctx, (uint64_t)s->n_total * nrhs_max * eb, &s->d_rhs))) break;
     s->owned.push_back(s->d_rhs);
-    if ((rc = ctx_alloc_t(ctx, s->n_total * nrhs_max, &s->d_r))) break;
-    s->owned.push_back(s->d_r);
     if ((rc = ctx_alloc_t(ctx, 2 * (int64_t)count * nrhs_max, &s->d_nrm))) break;
     s->owned.push_back(s->d_nrm);
+    hipError_t he = hipHostMalloc((void**)&s->h_nrm, sizeof(double) * 2 * (size_t)count * nrhs_max, 0);
+    if (he == hipSuccess) he = hipHostMalloc((void**)&s->h_status, 8 * sizeof(int32_t), 0);
+    if (he == hipSuccess) he = hipEventCreateWithFlags(&s->ev_done, hipEventDisableTiming);
+    if (he != hipSuccess) {
+      rc = hip_fail(he, "hipHostMalloc");
+      break;
+    }
     if ((rc = build_plan(s))) break;
     if (mode == PP_SOLVE_HINV && (rc = build_inverse_plans(s))) break;
     if (mode == PP_SOLVE_SUBST) rc = pp_program_factor_batch(hs, count, workers, &s->fprog);
@@ -408,12 +475,33 @@ int solve_prog(pp_solver* s, int nrhs, pp_solver::SolveProg** out) {
 
 // Check the device status words of a stream after its work completed.
 int check_status(pp_stream* st) {
-  int32_t v[4];
+  int32_t v[8];
   HIP_TRY(hipMemcpy(v, st->d_status, sizeof(v), hipMemcpyDeviceToHost));
-  if (v[0] == 0) return PP_OK;
-  return set_error(v[0] == 1 ? PP_ERR_TIMEOUT : PP_ERR_DEVICE,
-                   "VM error " + std::to_string(v[0]) + " (worker " + std::to_string(v[1]) + ", id " +
-                       std::to_string(v[2]) + ")");
+  return stream_status(st, v, nullptr);
+}
+
+// Waits for the work enqueued on st so far by polling an event (the refinement
+// loop waits once per sweep for a few hundred bytes of norms; a blocking
+// synchronisation would add its wake-up latency to every sweep).
+int wait_host(pp_solver* s, hipStream_t st) {
+  HIP_TRY(hipEventRecord(s->ev_done, st));
+  for (;;) {
+    const hipError_t q = hipEventQuery(s->ev_done);
+    if (q == hipSuccess) return PP_OK;
+    if (q != hipErrorNotReady) return hip_fail(q, "hipEventQuery");
+    std::this_thread::yield();
+  }
+}
+
+// Waits for the factorization launch on st, with its status words copied
+// into pinned host memory by the same stream (one wait).
+int finish_factor(pp_solver* s, hipStream_t st) {
+  HIP_TRY(hipMemcpyAsync(s->h_status, s->fstream->d_status, 8 * sizeof(int32_t), hipMemcpyDeviceToHost, st));
+  if (const int rc = wait_host(s, st)) return rc;
+  s->last_factor_ms = pp_stream_last_ms(s->fstream);
+  const int rc = stream_status(s->fstream, s->h_status, nullptr);
+  s->factored = rc == PP_OK;
+  return rc;
 }
 
 int enqueue_factor(pp_solver* s, hipStream_t st, bool events) {
@@ -501,13 +589,8 @@ int pp_solver_factor(pp_solver* s, void* hip_stream) {
     hipStream_t st = pick(s->ctx, hip_stream);
     s->factored = false;
     if (!s->use_graph) {
-      int rc = enqueue_factor(s, st, true);
-      if (rc) return rc;
-      rc = pp_wait(s->ctx, s->fstream, 0, nullptr);
-      if (rc) return rc;
-      s->last_factor_ms = pp_stream_last_ms(s->fstream);
-      s->factored = true;
-      return PP_OK;
+      const int rc = enqueue_factor(s, st, true);
+      return rc ? rc : finish_factor(s, st);
     }
     // capture once (restore + temp fills + epoch bump + VM), replay after
     if (s->factor_graph == nullptr || s->graph_stream != st) {
@@ -532,13 +615,8 @@ int pp_solver_factor(pp_solver* s, void* hip_stream) {
     HIP_TRY(hipEventRecord(s->fstream->ev0, st));
     HIP_TRY(hipGraphLaunch(s->factor_graph, st));
     HIP_TRY(hipEventRecord(s->fstream->ev1, st));
-    HIP_TRY(hipStreamSynchronize(st));
-    float ms = 0.f;
-    HIP_TRY(hipEventElapsedTime(&ms, s->fstream->ev0, s->fstream->ev1));
-    s->last_factor_ms = ms;
-    const int rc = check_status(s->fstream);
-    s->factored = rc == PP_OK;
-    return rc;
+    s->fstream->launched = true;
+    return finish_factor(s, st);
   } catch (...) {
     return from_exception();
   }
@@ -548,7 +626,7 @@ int pp_solver_residual(pp_solver* s, const double* b, const double* x, double* r
   if (s == nullptr || b == nullptr || x == nullptr || r == nullptr || nrhs < 1 || nrhs > s->nrhs_max || ld < nrhs)
     return set_error(PP_ERR_INVALID, "bad residual args");
   hipStream_t st = pick(s->ctx, hip_stream);
-  hipError_t e = pp::launch_hmatvec(s->plan, s->d_A, x, ld, b, ld, r, ld, nrhs, s->d_t, st);
+  hipError_t e = pp::launch_hmatvec(s->plan, s->d_A, x, ld, b, ld, r, ld, nrhs, s->d_t, pp::HmvExtra(), st);
   if (e != hipSuccess) return hip_fail(e, "hmatvec");
   HIP_TRY(hipStreamSynchronize(st));
   return PP_OK;
@@ -570,17 +648,32 @@ int pp_solver_solve(pp_solver* s, const double* b, double* x, int nrhs, int ld, 
     const int w = s->hinv ? nrhs : program_width(s, nrhs);
     int rc = PP_OK;
     if (!s->hinv && (rc = solve_prog(s, w, &sp))) return rc;
-    // d_rhs <- F^{-1} d_rhs (pitch w)
-    auto apply_inverse = [&]() -> int {
-      if (!s->hinv) return enqueue_launch(s->ctx, sp->stream, s->wpi, st, true);
-      hipError_t e2 = pp::launch_happly(s->lplan, eb, s->d_F, s->d_rhs, w, s->d_rhs2, w, nrhs, s->d_tinv, st);
+    // Hierarchical inverse: x = F^{-1} d_rhs (d_rhs, pitch w), the U^{-1}
+    // product going straight into x (xmode 1: set, 2: add).  VM solve
+    // programs solve in place in d_rhs, whose result is then moved into x.
+    auto apply_inverse = [&](int xmode) -> int {
+      hipError_t e2;
+      if (!s->hinv) {
+        if (const int r2 = enqueue_launch(s->ctx, sp->stream, s->wpi, st, true)) return r2;
+        e2 = xmode == 1 ? pp::launch_copy2d(x, 8, ld, s->d_rhs, eb, w, (int)s->n_total, nrhs, st)
+                        : pp::launch_add_correction(x, ld, s->d_rhs, eb, w, (int)s->n_total, nrhs, st);
+        return e2 == hipSuccess ? PP_OK : hip_fail(e2, "update x");
+      }
+      pp::HmvExtra to_x;
+      to_x.xo = x;
+      to_x.ldxo = ld;
+      to_x.xmode = xmode;
+      e2 = pp::launch_happly(s->lplan, eb, s->d_F, s->d_rhs, w, s->d_rhs2, w, nrhs, s->d_tinv, pp::HmvExtra(), st);
       if (e2 == hipSuccess)
-        e2 = pp::launch_happly(s->uplan, eb, s->d_F, s->d_rhs2, w, s->d_rhs, w, nrhs, s->d_tinv, st);
+        e2 = pp::launch_happly(s->uplan, eb, s->d_F, s->d_rhs2, w, nullptr, w, nrhs, s->d_tinv, to_x, st);
       return e2 == hipSuccess ? PP_OK : hip_fail(e2, "apply hierarchical inverse");
     };
-    std::vector<double> bn(M * nrhs), rn(M * nrhs);
+    // norms: r of member m, column c at h_nrm[m nrhs + c]; b at the same
+    // place + M nrhs_max (copied with the first residual's norms)
+    const double* rn = s->h_nrm;
+    const double* bn = s->h_nrm + (int64_t)M * s->nrhs_max;
     double* d_bn = s->d_nrm + (int64_t)M * s->nrhs_max;
-    hipError_t e = pp::launch_segnorm2(b, ld, s->d_seg, M, nrhs, d_bn, st);
+    hipError_t e = pp::launch_segnorm2(b, ld, s->d_seg, M, nrhs, s->norm_chunks, s->d_npart, d_bn, st);
     if (e != hipSuccess) return hip_fail(e, "norm b");
     // x = F^{-1} b
     if (w > nrhs) {
@@ -589,23 +682,27 @@ int pp_solver_solve(pp_solver* s, const double* b, double* x, int nrhs, int ld, 
     }
     e = pp::launch_copy2d(s->d_rhs, eb, w, b, 8, ld, (int)s->n_total, nrhs, st);
     if (e != hipSuccess) return hip_fail(e, "copy rhs");
-    if ((rc = apply_inverse())) return rc;
-    e = pp::launch_copy2d(x, 8, ld, s->d_rhs, eb, w, (int)s->n_total, nrhs, st);
-    if (e != hipSuccess) return hip_fail(e, "copy x");
-    HIP_TRY(hipMemcpyAsync(bn.data(), d_bn, 8 * bn.size(), hipMemcpyDeviceToHost, st));
-    HIP_TRY(hipStreamSynchronize(st));
-    if (sp != nullptr && (rc = check_status(sp->stream))) return rc;
+    if ((rc = apply_inverse(1))) return rc;
+    // the residual product leaves r in d_rhs (the next sweep's right-hand
+    // side) and its workgroups' column sums of squares in d_rpart
+    pp::HmvExtra to_rhs;
+    to_rhs.out2 = s->d_rhs;
+    to_rhs.out2_bytes = eb;
+    to_rhs.ld2 = w;
+    to_rhs.nrm_part = s->d_rpart;
     double res = 0.;
     int it = 0;
     bool conv = false;
     for (;;) {
       // r = b - A x (FP64)
-      e = pp::launch_hmatvec(s->plan, s->d_A, x, ld, b, ld, s->d_r, nrhs, nrhs, s->d_t, st);
+      e = pp::launch_hmatvec(s->plan, s->d_A, x, ld, b, ld, nullptr, nrhs, nrhs, s->d_t, to_rhs, st);
       if (e != hipSuccess) return hip_fail(e, "hmatvec");
-      e = pp::launch_segnorm2(s->d_r, nrhs, s->d_seg, M, nrhs, s->d_nrm, st);
+      e = pp::launch_rownorm_sum(s->d_rpart, s->d_seg_tiles, M, nrhs, s->d_nrm, st);
       if (e != hipSuccess) return hip_fail(e, "norm r");
-      HIP_TRY(hipMemcpyAsync(rn.data(), s->d_nrm, 8 * rn.size(), hipMemcpyDeviceToHost, st));
-      HIP_TRY(hipStreamSynchronize(st));
+      const int64_t ncopy = it == 0 ? 2LL * M * s->nrhs_max : (int64_t)M * nrhs;
+      HIP_TRY(hipMemcpyAsync(s->h_nrm, s->d_nrm, 8 * ncopy, hipMemcpyDeviceToHost, st));
+      if ((rc = wait_host(s, st))) return rc;
+      if (it == 0 && sp != nullptr && (rc = check_status(sp->stream))) return rc;
       res = 0.;
       for (int m = 0; m < M; ++m) {
         double worst = 0.;
@@ -622,11 +719,7 @@ int pp_solver_solve(pp_solver* s, const double* b, double* x, int nrhs, int ld, 
       }
       if (it >= max_iter) break;
       ++it;
-      e = pp::launch_copy2d(s->d_rhs, eb, w, s->d_r, 8, nrhs, (int)s->n_total, nrhs, st);
-      if (e != hipSuccess) return hip_fail(e, "copy residual");
-      if ((rc = apply_inverse())) return rc;
-      e = pp::launch_add_correction(x, ld, s->d_rhs, eb, w, (int)s->n_total, nrhs, st);
-      if (e != hipSuccess) return hip_fail(e, "update x");
+      if ((rc = apply_inverse(2))) return rc;
     }
     if (sp != nullptr && (rc = check_status(sp->stream))) return rc;
     if (rep) {

@@ -141,8 +141,9 @@ hipError_t launch_add_correction(double* x, int ldx, const void* d, int d_bytes,
 // Optional outputs of an H-matrix-vector product besides out (which may be
 // null): a second copy in 4- or 8-byte elements (out2, pitch ld2); an FP64
 // target xo set to (xmode 1) or incremented by (xmode 2) the product; the
-// per-workgroup column sums of squares of the product (nrm_part, n_tiles x
-// kTileRows / 16 rows of nrhs) for launch_rownorm_sum.
+// per-workgroup column sums of squares of the product (nrm_part: for column
+// c, the n_tiles kTileRows / 16 workgroup sums at nrm_part + c n_tiles
+// kTileRows / 16) for launch_rownorm_sum.
 struct HmvExtra {
   void* out2 = nullptr;
   int out2_bytes = 0, ld2 = 0;
@@ -160,7 +161,7 @@ hipError_t launch_happly(const MatvecPlanDev& p, int elem_bytes, const void* A, 
                          int ldo, int nrhs, void* t, const HmvExtra& ex, hipStream_t s);
 // out[g nrhs + c] = sum of squares of column c over member g's row tiles
 // (seg_tiles: pairs first tile, tile count) from a product's nrm_part.
-hipError_t launch_rownorm_sum(const double* part, const int* seg_tiles, int nseg, int nrhs, double* out,
+hipError_t launch_rownorm_sum(const double* part, const int* seg_tiles, int n_tiles, int nseg, int nrhs, double* out,
                               hipStream_t s);
 // Per (member, column) sums of squares of r: a first launch sums rows in
 // chunks of kNormRows into part (nseg x nchunks x nrhs), a second adds the

@@ -57,6 +57,32 @@ __global__ void convert_kernel(D* dst, const S* src, int64_t n) {
     dst[i] = (D)src[i];
 }
 
+// Four elements per thread, one thread each (16-byte aligned arrays; the
+// tail of n % 4 elements by the last threads): a large grid with every load
+// of the conversion in flight at once.
+template <typename D, typename S>
+__global__ __launch_bounds__(256) void convert4_kernel(D* dst, const S* src, int64_t n) {
+  const int64_t i = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) * 4;
+  if (i + 4 <= n) {
+    S v[4];
+    if constexpr (sizeof(S) == 8) {
+      const d2_t a = gload<d2_t>(src + i), b = gload<d2_t>(src + i + 2);
+      v[0] = a.x, v[1] = a.y, v[2] = b.x, v[3] = b.y;
+    } else {
+      const f4_t a = gload<f4_t>(src + i);
+      v[0] = a.x, v[1] = a.y, v[2] = a.z, v[3] = a.w;
+    }
+    if constexpr (sizeof(D) == 8) {
+      gstore<d2_t>(dst + i, d2_t{(double)v[0], (double)v[1]});
+      gstore<d2_t>(dst + i + 2, d2_t{(double)v[2], (double)v[3]});
+    } else {
+      gstore<f4_t>(dst + i, f4_t{(float)v[0], (float)v[1], (float)v[2], (float)v[3]});
+    }
+  } else {
+    for (int64_t j = i; j < n; ++j) dst[j] = (D)src[j];
+  }
+}
+
 // x[i, c] += d[i, c] for an n x nrhs block with pitches
 template <typename S>
 __global__ void add_correction_kernel(double* x, int ldx, const S* d, int ldd, int n, int nrhs) {
@@ -268,7 +294,8 @@ __global__ __launch_bounds__(256) void hmv_rows_kernel(const MatvecBlock* blocks
   if (c0 >= nrhs) return;  // whole workgroup
   if (r0 >= tl.rows) {      // whole workgroup: a row group past a short tile adds nothing to the norms
     if (ex.nrm_part != nullptr && threadIdx.x < 64 && c0 + (int)threadIdx.x < nrhs)
-      ex.nrm_part[(int64_t)(blockIdx.x * (kTileRows / 16) + blockIdx.y) * nrhs + c0 + threadIdx.x] = 0.;
+      ex.nrm_part[(int64_t)(c0 + threadIdx.x) * gridDim.x * (kTileRows / 16) + blockIdx.x * (kTileRows / 16) +
+                  blockIdx.y] = 0.;
     return;
   }
   const int rows = min(16, tl.rows - r0);
@@ -373,23 +400,33 @@ __global__ __launch_bounds__(256) void hmv_rows_kernel(const MatvecBlock* blocks
       const int cc = threadIdx.x >> 4, c = c0 + threadIdx.x;
       double a = sqs[cc * 16 + j];
       for (int r = 1; r < 16; ++r) a += sqs[(r * NC + cc) * 16 + j];
-      if (c < nrhs) ex.nrm_part[(int64_t)(blockIdx.x * (kTileRows / 16) + blockIdx.y) * nrhs + c] = a;
+      if (c < nrhs)
+        ex.nrm_part[(int64_t)c * gridDim.x * (kTileRows / 16) + blockIdx.x * (kTileRows / 16) + blockIdx.y] = a;
     }
   }
 }
 
 // Per (member, column) sums of squares from the per-workgroup column sums of
-// a residual product: member g's row tiles [first, first + count) (seg_tiles
-// pairs), kTileRows / 16 workgroups each, added in ordered_sum4 order.
-__global__ __launch_bounds__(256) void rownorm_sum_kernel(const double* part, const int* seg_tiles, int nseg, int nrhs,
+// a residual product (column-major: column c's n_wg = n_tiles kTileRows / 16
+// sums at part + c n_wg): one workgroup per (member g, column c) adds the
+// sums of member g's row tiles [first, first + count) (seg_tiles pairs), each
+// thread a fixed stride of them, then a fixed-order tree.
+__global__ __launch_bounds__(256) void rownorm_sum_kernel(const double* part, const int* seg_tiles, int n_wg, int nrhs,
                                                            double* out) {
   __shared__ double red[256];
-  const int n = nseg * nrhs, e = 64 * blockIdx.x + (threadIdx.x & 63), ec = min(e, n - 1);
-  const int g = ec / nrhs, c = ec % nrhs;
+  const int g = blockIdx.x / nrhs, c = blockIdx.x % nrhs;
   constexpr int G = kTileRows / 16;
-  const double v = ordered_sum4<double>(part + (int64_t)seg_tiles[2 * g] * G * nrhs + c, nrhs, seg_tiles[2 * g + 1] * G,
-                                        e < n, red);
-  if (threadIdx.x < 64 && e < n) out[e] = v;
+  const double* p = part + (int64_t)c * n_wg + (int64_t)seg_tiles[2 * g] * G;
+  const int n = seg_tiles[2 * g + 1] * G;
+  double v = 0.;
+  for (int i = threadIdx.x; i < n; i += 256) v += p[i];
+  red[threadIdx.x] = v;
+  __syncthreads();
+  for (int h = 128; h > 0; h >>= 1) {
+    if ((int)threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[blockIdx.x] = red[0];
 }
 
 // Per (member, column) sums of squares, stage 1: workgroup (chunk j, member
@@ -450,13 +487,31 @@ hipError_t launch_fill_zero(void* p, int64_t n, int elem_bytes, hipStream_t s) {
 
 hipError_t launch_convert(void* dst, int dst_bytes, const void* src, int src_bytes, int64_t n, hipStream_t s) {
   if (n <= 0) return hipSuccess;
+  const bool al = ((reinterpret_cast<uintptr_t>(dst) | reinterpret_cast<uintptr_t>(src)) & 15) == 0;
+  const int64_t g4 = (n + 1023) / 1024;  // 256 threads x 4 elements per workgroup
+  if (al && g4 <= 0x7fffffff) {
+    // kernels, not memcpy nodes, so graphs capture them plainly
+    if (dst_bytes == 4 && src_bytes == 8)
+      hipLaunchKernelGGL((convert4_kernel<float, double>), dim3((unsigned)g4), dim3(256), 0, s, (float*)dst,
+                         (const double*)src, n);
+    else if (dst_bytes == 8 && src_bytes == 4)
+      hipLaunchKernelGGL((convert4_kernel<double, float>), dim3((unsigned)g4), dim3(256), 0, s, (double*)dst,
+                         (const float*)src, n);
+    else if (dst_bytes == 8 && src_bytes == 8)
+      hipLaunchKernelGGL((convert4_kernel<double, double>), dim3((unsigned)g4), dim3(256), 0, s, (double*)dst,
+                         (const double*)src, n);
+    else
+      hipLaunchKernelGGL((convert4_kernel<float, float>), dim3((unsigned)g4), dim3(256), 0, s, (float*)dst,
+                         (const float*)src, n);
+    return hipGetLastError();
+  }
   if (dst_bytes == 4 && src_bytes == 8)
     hipLaunchKernelGGL((convert_kernel<float, double>), dim3(grid_for(n)), dim3(256), 0, s, (float*)dst,
                        (const double*)src, n);
   else if (dst_bytes == 8 && src_bytes == 4)
     hipLaunchKernelGGL((convert_kernel<double, float>), dim3(grid_for(n)), dim3(256), 0, s, (double*)dst,
                        (const float*)src, n);
-  else if (dst_bytes == 8 && src_bytes == 8)  // kernels, not memcpy nodes, so graphs capture them plainly
+  else if (dst_bytes == 8 && src_bytes == 8)
     hipLaunchKernelGGL((convert_kernel<double, double>), dim3(grid_for(n)), dim3(256), 0, s, (double*)dst,
                        (const double*)src, n);
   else
@@ -542,11 +597,11 @@ hipError_t launch_happly(const MatvecPlanDev& p, int elem_bytes, const void* A, 
   return hipErrorInvalidValue;
 }
 
-hipError_t launch_rownorm_sum(const double* part, const int* seg_tiles, int nseg, int nrhs, double* out,
+hipError_t launch_rownorm_sum(const double* part, const int* seg_tiles, int n_tiles, int nseg, int nrhs, double* out,
                               hipStream_t s) {
-  if (nseg < 1 || nrhs < 1) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(rownorm_sum_kernel, dim3((nseg * nrhs + 63) / 64), dim3(256), 0, s, part, seg_tiles, nseg, nrhs,
-                     out);
+  if (nseg < 1 || nrhs < 1 || n_tiles < 1) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(rownorm_sum_kernel, dim3(nseg * nrhs), dim3(256), 0, s, part, seg_tiles,
+                     n_tiles * (kTileRows / 16), nrhs, out);
   return hipGetLastError();
 }
 

@@ -697,7 +697,7 @@ int pp_solver_solve(pp_solver* s, const double* b, double* x, int nrhs, int ld, 
       // r = b - A x (FP64)
       e = pp::launch_hmatvec(s->plan, s->d_A, x, ld, b, ld, nullptr, nrhs, nrhs, s->d_t, to_rhs, st);
       if (e != hipSuccess) return hip_fail(e, "hmatvec");
-      e = pp::launch_rownorm_sum(s->d_rpart, s->d_seg_tiles, M, nrhs, s->d_nrm, st);
+      e = pp::launch_rownorm_sum(s->d_rpart, s->d_seg_tiles, s->plan.n_tiles, M, nrhs, s->d_nrm, st);
       if (e != hipSuccess) return hip_fail(e, "norm r");
       const int64_t ncopy = it == 0 ? 2LL * M * s->nrhs_max : (int64_t)M * nrhs;
       HIP_TRY(hipMemcpyAsync(s->h_nrm, s->d_nrm, 8 * ncopy, hipMemcpyDeviceToHost, st));

@@ -132,9 +132,10 @@ __device__ __forceinline__ void diag_inverses(const T* F, int ldf, int n, T* out
 // diag_inverses with the inverses left in LDS: block b at D + b NB (NB + 1)
 // (row-major, pitch NB + 1, in place of its loaded copy; column NB holds the
 // reciprocals of the diagonal in the upper case).  n a multiple of NB, at
-// most 2 NB blocks per wave (one pass); ends with a barrier.
+// most 2 NB blocks per wave (one pass); ends with a barrier.  In two halves:
+// diag_blocks_load (the blocks into LDS, no barrier) and diag_blocks_invert.
 template <typename T, bool UPPER>
-__device__ __forceinline__ void diag_inverses_lds(const T* F, int ldf, int n, PP_LDS T* D) {
+__device__ __forceinline__ void diag_blocks_load(const T* F, int ldf, int n, PP_LDS T* D) {
   const int w = wave_id(), l = lane_id();
   const int half = l >> 5, lc = l & 31;
   const int nblk = n / NB;
@@ -161,6 +162,14 @@ __device__ __forceinline__ void diag_inverses_lds(const T* F, int ldf, int n, PP
     for (int j = 0; j < NB; ++j) s[lc * (NB + 1) + j] = v[j];
     if (UPPER) s[lc * (NB + 1) + NB] = T(1) / (T)s[lc * (NB + 1) + lc];
   }
+}
+template <typename T, bool UPPER>
+__device__ __forceinline__ void diag_blocks_invert(int n, PP_LDS T* D) {
+  const int w = wave_id(), l = lane_id();
+  const int half = l >> 5, lc = l & 31;
+  const int nblk = n / NB;
+  const int b = 2 * w + half;
+  PP_LDS T* s = D + b * NB * (NB + 1);
   __syncthreads();
   if (b < nblk) {
     T y[NB];
@@ -170,6 +179,11 @@ __device__ __forceinline__ void diag_inverses_lds(const T* F, int ldf, int n, PP
     for (int i = 0; i < NB; ++i) s[i * (NB + 1) + lc] = y[i];
   }
   __syncthreads();
+}
+template <typename T, bool UPPER>
+__device__ __forceinline__ void diag_inverses_lds(const T* F, int ldf, int n, PP_LDS T* D) {
+  diag_blocks_load<T, UPPER>(F, ldf, n, D);
+  diag_blocks_invert<T, UPPER>(n, D);
 }
 
 // ---------------------------------------------------------------- panel update
@@ -575,6 +589,183 @@ __device__ __forceinline__ void tile_from_lds(const T (&fa)[NB / Mma<T, MODE>::K
       for (int j = 0; j < 2; ++j) acc[i][j] = P::mma_chunk(fa[c][i], fb[c][j], acc[i][j]);
 }
 
+// Chain form for FP32 solves of two to eight whole blocks with one 32-wide
+// column tile.  Wave 0 forms the blocks in solve order,
+//   X_b = D_b (W_b - S_b - T_{b,p} X_p)    (p: the block solved just before b),
+// keeping X_p in registers: in FP32 MFMA the accumulator layout of a 32-row
+// tile (lane l, register r: row 4 (l / 16) + r) is the B-operand layout of the
+// next product (k = 4 (l / 16) + e), so the two products of a block follow
+// each other with no LDS round trip.  Waves 1-3 form S_b, the sum of
+// T_{b,j} X_j over the blocks j solved before p (each X_j read from LDS after
+// wave 0 publishes it), and hand it to wave 0 through LDS (Ss, flag sready[b])
+// while wave 0 works on p.  Wave 0 waits only for S_b, whose producer waits
+// only for blocks wave 0 has already published, so every wait ends.  Not
+// inlined (own register allocation).
+template <typename T, int MODE>
+__device__ __attribute__((noinline)) void narrow_chain(PP_LDS T* Ws, int wp, PP_LDS T* Ss, const PP_LDS T* Ts,
+                                                       volatile PP_LDS int* ready, volatile PP_LDS int* sready, int m,
+                                                       int nblk, const T* F, int ldf, bool f_trans,
+                                                       const PP_LDS T* Dl, bool forward) {
+  using P = Mma<T, MODE>;
+  using acc_t = typename P::acc_t;
+  constexpr int KW = P::KW, EPL = P::EPL, CH = NB / KW;
+  static_assert(CH == 2 && EPL == 4 && KW == 16, "chain form: 16x16x4 MFMA, 32-row blocks");
+  const int l = lane_id(), wv = wave_id(), li = l & 15, g = l >> 4;
+  auto blk = [&](int o) { return forward ? o : nblk - 1 - o; };
+  // A fragments of T_{b,j} (rows of block b, k over the rows of block j)
+  auto load_t = [&](int b, int j, T(&fa)[CH][2][EPL]) {
+    OperandLane<T, EPL> la[2];
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      if (f_trans) la[h].init(F + (int64_t)j * NB * ldf, ldf, false, b * NB + 16 * h + li, m);
+      else la[h].init(F + j * NB, ldf, true, b * NB + 16 * h + li, m);
+    }
+    load_a_frags<T, MODE>(la, NB, fa);
+  };
+  // A fragments of D_b (inverses in LDS, element (r, q) at r (NB + 1) + q)
+  auto load_d = [&](int b, T(&fd)[CH][2][EPL]) {
+    const PP_LDS T* Db = Dl + b * NB * (NB + 1);
+#pragma unroll
+    for (int c = 0; c < CH; ++c)
+#pragma unroll
+      for (int h = 0; h < 2; ++h)
+#pragma unroll
+        for (int e = 0; e < EPL; ++e) {
+          const int r = 16 * h + li, q = c * KW + EPL * g + e;
+          fd[c][h][e] = f_trans ? (T)Db[q * (NB + 1) + r] : (T)Db[r * (NB + 1) + q];
+        }
+  };
+  // A fragments of the chain block T_{blk(o), blk(o - 1)} staged in LDS
+  auto load_ts = [&](int o, T(&fa)[CH][2][EPL]) {
+    const PP_LDS T* tb = Ts + (o - 1) * NB * (NB + 1);
+#pragma unroll
+    for (int c = 0; c < CH; ++c)
+#pragma unroll
+      for (int h = 0; h < 2; ++h)
+#pragma unroll
+        for (int e = 0; e < EPL; ++e) fa[c][h][e] = tb[(16 * h + li) * (NB + 1) + c * KW + EPL * g + e];
+  };
+  // out[h][jj] = A (fragments fa) x B, B given in accumulator layout:
+  // chunk c of B is the 16-row tile c (b[c][jj][e] = row 16 c + 4 g + e)
+  auto product = [&](const T(&fa)[CH][2][EPL], const acc_t(&b)[2][2], acc_t(&out)[2][2]) {
+#pragma unroll
+    for (int h = 0; h < 2; ++h)
+#pragma unroll
+      for (int jj = 0; jj < 2; ++jj) {
+        out[h][jj] = P::zero();
+#pragma unroll
+        for (int c = 0; c < CH; ++c) {
+          T fb[EPL];
+#pragma unroll
+          for (int e = 0; e < EPL; ++e) fb[e] = b[c][jj][e];
+          out[h][jj] = P::mma_chunk(fa[c][h], fb, out[h][jj]);
+        }
+      }
+  };
+  if (wv == 0) {
+    acc_t xp[2][2];
+    T fdn[CH][2][EPL], ftn[CH][2][EPL];
+    load_d(blk(0), fdn);
+    load_ts(1, ftn);
+    for (int ob = 0; ob < nblk; ++ob) {
+      const int b = blk(ob), rb = b * NB;
+      // fd = D_b, ft = T_{b,p} (requested one block ahead)
+      T fd[CH][2][EPL], ft[CH][2][EPL];
+#pragma unroll
+      for (int c = 0; c < CH; ++c)
+#pragma unroll
+        for (int h = 0; h < 2; ++h)
+#pragma unroll
+          for (int e = 0; e < EPL; ++e) {
+            fd[c][h][e] = fdn[c][h][e];
+            ft[c][h][e] = ftn[c][h][e];
+          }
+      if (ob + 1 < nblk) {
+        load_d(blk(ob + 1), fdn);
+        if (ob >= 1) load_ts(ob + 1, ftn);
+      }
+      acc_t w[2][2];
+#pragma unroll
+      for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int jj = 0; jj < 2; ++jj)
+#pragma unroll
+          for (int r = 0; r < 4; ++r) w[a][jj][r] = Ws[(rb + 16 * a + P::orow(l, r)) * wp + 16 * jj + li];
+      if (ob >= 1) {
+        acc_t pr[2][2];
+        product(ft, xp, pr);
+        if (ob >= 2) {
+          lds_wait_ge(sready + b, 1);
+#pragma unroll
+          for (int a = 0; a < 2; ++a)
+#pragma unroll
+            for (int jj = 0; jj < 2; ++jj)
+#pragma unroll
+              for (int r = 0; r < 4; ++r) w[a][jj][r] -= Ss[((b * 4 + a * 2 + jj) * 64 + l) * 4 + r];
+        }
+#pragma unroll
+        for (int a = 0; a < 2; ++a)
+#pragma unroll
+          for (int jj = 0; jj < 2; ++jj)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) w[a][jj][r] -= pr[a][jj][r];
+      }
+      product(fd, w, xp);
+#pragma unroll
+      for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int jj = 0; jj < 2; ++jj)
+#pragma unroll
+          for (int r = 0; r < 4; ++r) Ws[(rb + 16 * a + P::orow(l, r)) * wp + 16 * jj + li] = xp[a][jj][r];
+      lds_order();
+      if (l == 0) ready[b] = 1;
+    }
+  } else {
+    for (int ob = wv + 1; ob < nblk; ob += 3) {
+      const int b = blk(ob);
+      acc_t s[2][2];
+#pragma unroll
+      for (int a = 0; a < 2; ++a) s[a][0] = s[a][1] = P::zero();
+      T ft[CH][2][EPL];
+      load_t(b, blk(0), ft);
+      for (int oj = 0; oj + 1 < ob; ++oj) {
+        const int j = blk(oj);
+        T ftn[CH][2][EPL];
+        if (oj + 2 < ob) load_t(b, blk(oj + 1), ftn);
+        lds_wait_ge(ready + j, 1);
+#pragma unroll
+        for (int h = 0; h < 2; ++h)
+#pragma unroll
+          for (int jj = 0; jj < 2; ++jj)
+#pragma unroll
+            for (int c = 0; c < CH; ++c) {
+              T fb[EPL];
+#pragma unroll
+              for (int e = 0; e < EPL; ++e) fb[e] = Ws[(j * NB + c * KW + EPL * g + e) * wp + 16 * jj + li];
+              s[h][jj] = P::mma_chunk(ft[c][h], fb, s[h][jj]);
+            }
+        if (oj + 2 < ob) {
+#pragma unroll
+          for (int c = 0; c < CH; ++c)
+#pragma unroll
+            for (int h = 0; h < 2; ++h)
+#pragma unroll
+              for (int e = 0; e < EPL; ++e) ft[c][h][e] = ftn[c][h][e];
+        }
+      }
+#pragma unroll
+      for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int jj = 0; jj < 2; ++jj)
+#pragma unroll
+          for (int r = 0; r < 4; ++r) Ss[((b * 4 + a * 2 + jj) * 64 + l) * 4 + r] = s[a][jj][r];
+      lds_order();
+      if (l == 0) sready[b] = 1;
+    }
+  }
+  __syncthreads();
+}
+
 // LDS bytes of narrow_solve for m rows and w columns.
 template <typename T>
 __device__ __forceinline__ int narrow_solve_bytes(int m, int w) {
@@ -613,23 +804,84 @@ __device__ __forceinline__ void narrow_solve(T* Wg, int ldw, bool w_trans, int m
   // (FP32 only: narrow_dl_ok admits no FP64 solve, and the FP64 kernels are
   // compiled without this path)
   PP_LDS T* Dl = nullptr;
+  constexpr int V = 16 / (int)sizeof(T);
+  typedef T vec_t __attribute__((ext_vector_type(V)));
+  const bool wvec = !w_trans && (w % V) == 0 && (ldw % V) == 0 && (reinterpret_cast<uintptr_t>(Wg) & 15) == 0;
+  // the chain form (FP32 MFMA, two to eight whole blocks, one column tile,
+  // diagonal inverses formed here, W and F 16-byte rows): after W and the
+  // inverses, S (nblk blocks, C layout) and its flags, then the blocks
+  // T_{b,p} of the chain (pitch NB + 1)
+  bool chain = false;
+  int ss_off = 0, ts_off = 0;
+  if constexpr (sizeof(T) == 4 && MODE == MODE_FP32) {
+    ss_off = narrow_dl_offset<T>(m, w) + nblk * NB * (NB + 1) * (int)sizeof(T);
+    ts_off = (ss_off + nblk * NB * NB * (int)sizeof(T) + nblk * (int)sizeof(int) + 15) & ~15;
+    chain = dl != 0 && wvec && ntc == 1 && m % NB == 0 && nblk >= 2 && nblk <= 8 && nw == 4 &&
+            (ldf % V) == 0 && (reinterpret_cast<uintptr_t>(F) & 15) == 0 &&
+            ts_off + (nblk - 1) * NB * (NB + 1) * (int)sizeof(T) <= kLdsBytes - 1024;
+  }
   if constexpr (sizeof(T) == 4) {
     if (dl != 0) Dl = (PP_LDS T*)(lds_base() + narrow_dl_offset<T>(m, w));
-    if (dl == 1) diag_inverses_lds<T, false>(F, ldf, m, Dl);
-    else if (dl == 2) diag_inverses_lds<T, true>(F, ldf, m, Dl);
+    if (chain) {
+      // W, the chain blocks and the diagonal blocks requested together (one
+      // memory round trip), then stored, then the inverses formed
+      const int vpr = wc / V;  // 16-byte vectors per W row (all of them real: wvec)
+      vec_t wq[8], tq[7];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int e = tid + u * nt, r = e / vpr, c = (e - r * vpr) * V;
+        const bool in = e < mr * vpr && r < m && c < w;
+        wq[u] = gload<vec_t>(Wg + (in ? (int64_t)r * ldw + c : 0));
+        if (!in) wq[u] = vec_t{};
+      }
+#pragma unroll
+      for (int o = 1; o < 8; ++o) {
+        if (o < nblk) {
+          const int b = forward ? o : nblk - 1 - o, p = forward ? o - 1 : nblk - o;
+          const int i = tid >> 3, q = (tid & 7) * V;  // row (or, transposed, column) and first of V
+          tq[o - 1] = f_trans ? gload<vec_t>(F + (int64_t)(p * NB + i) * ldf + b * NB + q)
+                              : gload<vec_t>(F + (int64_t)(b * NB + i) * ldf + p * NB + q);
+        }
+      }
+      if (dl == 1) diag_blocks_load<T, false>(F, ldf, m, Dl);
+      else diag_blocks_load<T, true>(F, ldf, m, Dl);
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int e = tid + u * nt, r = e / vpr, c = (e - r * vpr) * V;
+        if (e < mr * vpr) *(PP_LDS vec_t*)(Ws + r * wp + c) = wq[u];
+      }
+      PP_LDS T* Ts = (PP_LDS T*)(lds_base() + ts_off);
+#pragma unroll
+      for (int o = 1; o < 8; ++o) {
+        if (o < nblk) {
+          PP_LDS T* tb = Ts + (o - 1) * NB * (NB + 1);
+          const int i = tid >> 3, q = (tid & 7) * V;
+#pragma unroll
+          for (int e = 0; e < V; ++e) {
+            if (f_trans) tb[(q + e) * (NB + 1) + i] = tq[o - 1][e];
+            else tb[i * (NB + 1) + q + e] = tq[o - 1][e];
+          }
+        }
+      }
+      if (dl == 1) diag_blocks_invert<T, false>(m, Dl);
+      else diag_blocks_invert<T, true>(m, Dl);
+    } else if (dl == 1) {
+      diag_inverses_lds<T, false>(F, ldf, m, Dl);
+    } else if (dl == 2) {
+      diag_inverses_lds<T, true>(F, ldf, m, Dl);
+    }
   }
   // stage W (in the order that keeps global accesses coalesced); rows >= m
   // and columns >= w are zero and stay zero.  The loads go through the
   // global address space with clamped addresses (no branch), so the compiler
   // may issue them ahead of the LDS stores of the same batch (a flat load
   // could alias LDS and would wait for each store before it).
-  constexpr int V = 16 / (int)sizeof(T);
-  typedef T vec_t __attribute__((ext_vector_type(V)));
   // W row-major with whole 16-byte vectors: every vector of the staged block
   // is requested at once (one memory round trip for up to 16 vectors per
   // thread); rows >= m and columns >= w read a clamped address and are zeroed
-  const bool wvec = !w_trans && (w % V) == 0 && (ldw % V) == 0 && (reinterpret_cast<uintptr_t>(Wg) & 15) == 0;
-  if (wvec) {
+  if (chain) {
+    // staged above
+  } else if (wvec) {
     constexpr int U = 16;
     const int vpr = wc / V, total = mr * vpr;
     for (int e0 = tid; e0 < total; e0 += U * nt) {
@@ -678,9 +930,19 @@ __device__ __forceinline__ void narrow_solve(T* Wg, int ldw, bool w_trans, int m
   }
   volatile PP_LDS int* ready = (volatile PP_LDS int*)(Ws + mr * wp);
   for (int e = tid; e < nblk; e += nt) ready[e] = 0;
-  __syncthreads();
   constexpr int CH = NB / P::KW;
-  {
+  if constexpr (sizeof(T) == 4 && MODE == MODE_FP32) {
+    if (chain) {
+      PP_LDS T* Ss = (PP_LDS T*)(lds_base() + ss_off);
+      volatile PP_LDS int* sready = (volatile PP_LDS int*)(Ss + nblk * NB * NB);
+      for (int e = tid; e < nblk; e += nt) sready[e] = 0;
+      __syncthreads();
+      narrow_chain<T, MODE>(Ws, wp, Ss, (const PP_LDS T*)(lds_base() + ts_off), ready, sready, m, nblk, F, ldf,
+                            f_trans, Dl, forward);
+    }
+  }
+  if (!chain) {
+    __syncthreads();
     // Row-block ownership: wave wv owns the blocks at distance wv, wv + nw,
     // ... from the first block of the solve order (increasing for forward,
     // decreasing for backward) and takes them in that order.  For its block

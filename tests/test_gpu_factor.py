@@ -166,17 +166,44 @@ def test_padded_solve_widths(pp, ctx):
         assert np.linalg.norm(got - want[:, :k]) <= 1e-10 * np.linalg.norm(want[:, :k])
 
 
-def test_matvec_matches_dense(pp, ctx):
+@pytest.mark.parametrize("nrhs", [4, 24, 80])
+def test_matvec_matches_dense(pp, ctx, nrhs):
+    """The FP64 residual product against numpy: one, two and (past 64
+    columns) two groups of four 16-wide column tiles."""
+    print(f"[case] matvec n=2048 nrhs={nrhs}", flush=True)
     hm, kern, arena = pp.build_config(1, 2048, levels=3, rank=16)
-    solver = pp.Solver(ctx, hm, arena, precision=0, workers=8, nrhs_max=4)
-    rng = np.random.default_rng(3)
-    x = rng.standard_normal((2048, 4))
-    b = rng.standard_normal((2048, 4))
+    solver = pp.Solver(ctx, hm, arena, precision=0, workers=8, nrhs_max=nrhs)
+    rng = np.random.default_rng(3 + nrhs)
+    x = rng.standard_normal((2048, nrhs))
+    b = rng.standard_normal((2048, nrhs))
     dx, db = torch.from_numpy(x).cuda(), torch.from_numpy(b).cuda()
     dr = torch.zeros_like(db)
-    solver.residual(db.data_ptr(), dx.data_ptr(), dr.data_ptr(), 4)
+    solver.residual(db.data_ptr(), dx.data_ptr(), dr.data_ptr(), nrhs)
     want = b - pp.densify(hm, arena) @ x
     assert np.max(np.abs(dr.cpu().numpy() - want)) < 1e-10 * np.max(np.abs(want))
+
+
+@pytest.mark.parametrize("precision", [0, 2])
+def test_solve_wide(pp, orc, ctx, precision):
+    """80 right-hand sides (two column groups in every H-matrix product and in
+    the residual norms): refined to 1e-10, checked against the dense FP64
+    oracle residual and solution."""
+    print(f"[case] wide solve n=2048 nrhs=80 precision={precision}", flush=True)
+    hm, kern, arena = pp.build_config(1, 2048, levels=3, rank=16)
+    nrhs = 80
+    solver = pp.Solver(ctx, hm, arena, precision=precision, workers=16, nrhs_max=nrhs)
+    solver.factor()
+    b = np.random.default_rng(11).standard_normal((2048, nrhs))
+    db = torch.from_numpy(b).cuda()
+    dx = torch.zeros_like(db)
+    rep = solver.solve(db.data_ptr(), dx.data_ptr(), nrhs, tol=1e-10, max_iter=40)
+    x = dx.cpu().numpy()
+    a = pp.densify(hm, arena)
+    res = orc.dense_residual(a, x, b)
+    assert rep["converged"] and rep["residual"] <= 1e-10
+    assert res <= 2e-10
+    xo = orc.dense_lu_solve(orc.dense_lu(a), b)
+    assert np.linalg.norm(x - xo) / np.linalg.norm(xo) < 1e-8
 
 
 def test_watchdog_reports_lost_signal(pp, ctx):
